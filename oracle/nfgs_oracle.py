@@ -1,0 +1,1004 @@
+"""CPU oracle for the NFGS plain iterate — TEST INFRASTRUCTURE ONLY.
+
+This module is a NumPy restatement of the reference solver's default NFGS
+path (Gaussian-separated background, frozen-linear-harmonic closure,
+staggered-face vertical current, direct poloidal update, gs_cfcf start,
+sigma_best stopping).  It exists to check the HIP product path.  Only
+``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
+may import it, and only as the checker or the timed CPU baseline; the
+product package never imports it.
+
+Every function cites the reference lines it follows (``nfgs_solver.py``).
+
+Parity pins (what checks each part; see tests/test_oracle_*.py):
+  * transforms           scipy.fft.dct/dst(type=1), dense closed-form sums,
+                         involution C.C = 2(n-1) I and S.S = 2(n+1) I
+  * lambda tables / FD   DCT2(lap_xy f) == lambda * DCT2(f); polynomial
+                         exactness of the FD stencils
+  * z solves             dense numpy.linalg.solve of the assembled systems
+  * Jz derivation        the reference's own known-answer tests
+                         (linear field curl 3.0; sign-preserving balance)
+  * everything composite golden fixtures written by tests/golden/make_golden.py
+                         (reference imported on CPU, FP64 and FP32)
+No function is left "parity unpinned".
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+# ----------------------------------------------------------------------------
+# Grid sizing (nfgs_solver.py:73-177, :771-819)
+# ----------------------------------------------------------------------------
+
+
+def mudpack_axis_size(n_obs: int) -> int:
+    """nfgs_solver.py:73-83."""
+    n_obs = int(n_obs)
+    if n_obs < 2:
+        return max(1, n_obs)
+    n = float(n_obs)
+    e2 = math.ceil(math.log((n - 1.0) / 2.0) / math.log(2.0)) + 1
+    e3 = math.ceil(math.log((n - 1.0) / 3.0) / math.log(2.0)) + 1
+    return min(int(2 * 2 ** (e2 - 1) + 1), int(3 * 2 ** (e3 - 1) + 1))
+
+
+def even_xy_pad_size(n_calc: int, n_obs: int) -> int:
+    """nfgs_solver.py:86-102."""
+    if n_calc < n_obs:
+        raise ValueError(f"calculation size {n_calc} is smaller than observed size {n_obs}")
+    return n_calc + 1 if (n_calc - n_obs) % 2 == 1 else n_calc
+
+
+def fortran_grid(nxo: int, nyo: int, nzo: int, symmetric_xy: bool = False):
+    """Sizes and pads of nfgs_solver.py:105-177 (spacings are not needed)."""
+    nx, ny, nz = mudpack_axis_size(nxo), mudpack_axis_size(nyo), mudpack_axis_size(nzo)
+    if symmetric_xy:
+        nx, ny = even_xy_pad_size(nx, nxo), even_xy_pad_size(ny, nyo)
+    return dict(nx=nx, ny=ny, nz=nz, bfx=(nx - nxo) // 2, bfy=(ny - nyo) // 2, bfz=nz - nzo)
+
+
+def calculation_grid(Nx, Ny, Nz, pad=True, mode="mudpack"):
+    """Calculation grid and lateral pads, nfgs_solver.py:771-819."""
+    mode = str(mode).lower().replace("-", "_")
+    if mode not in ("mudpack", "mudpack_legacy", "bottom_padding"):
+        raise ValueError(f"calculation_domain_mode invalid: {mode!r}")
+    if not pad:
+        return Nx, Ny, Nz, (0, 0)
+    if mode in ("mudpack", "mudpack_legacy"):
+        g = fortran_grid(Nx, Ny, Nz, symmetric_xy=(mode == "mudpack"))
+        return g["nx"], g["ny"], g["nz"], (g["bfx"], g["bfy"])
+    p = (Nx + Ny) // 6
+    cx, cy = Nx + 2 * p, Ny + 2 * p
+    cz = min(cx, cy)
+    if cz < Nz:
+        raise ValueError("bottom_padding calculation height cannot contain the input z domain")
+    return cx, cy, cz, (p, p)
+
+
+# ----------------------------------------------------------------------------
+# Type-I transforms via mirrored real FFTs (nfgs_solver.py:1019-1081)
+# ----------------------------------------------------------------------------
+
+
+def _move(x, axis):
+    return np.moveaxis(x, axis, -1)
+
+
+def dst1(x, axis):
+    """Unnormalised DST-I, nfgs_solver.py:1019-1045."""
+    x = _move(np.asarray(x), axis)
+    n = x.shape[-1]
+    y = np.zeros(x.shape[:-1] + (2 * (n + 1),), dtype=x.dtype)
+    y[..., 1:n + 1] = x
+    y[..., n + 2:] = -x[..., ::-1]
+    Y = np.fft.rfft(y, axis=-1)
+    return np.moveaxis((-Y.imag[..., 1:n + 1]).astype(x.dtype), -1, axis)
+
+
+def idst1(X, axis):
+    """nfgs_solver.py:1047-1050."""
+    n = np.asarray(X).shape[axis]
+    return dst1(X, axis) / (2.0 * (n + 1))
+
+
+def dct1(x, axis):
+    """Unnormalised DCT-I, nfgs_solver.py:1052-1074."""
+    x = _move(np.asarray(x), axis)
+    n = x.shape[-1]
+    if n == 1:
+        return np.moveaxis(x.copy(), -1, axis)
+    y = np.empty(x.shape[:-1] + (2 * (n - 1),), dtype=x.dtype)
+    y[..., :n] = x
+    y[..., n:] = x[..., n - 2:0:-1]
+    Y = np.fft.rfft(y, axis=-1)
+    return np.moveaxis(Y.real[..., :n].astype(x.dtype), -1, axis)
+
+
+def idct1(X, axis):
+    """nfgs_solver.py:1076-1081."""
+    n = np.asarray(X).shape[axis]
+    if n == 1:
+        return np.array(X, copy=True)
+    return dct1(X, axis) / (2.0 * (n - 1))
+
+
+def dct2(v):
+    """nfgs_solver.py:4354-4355."""
+    return dct1(dct1(v, -1), -2)
+
+
+def idct2(v):
+    """nfgs_solver.py:4459-4460."""
+    return idct1(idct1(v, -1), -2)
+
+
+def dst2(v):
+    return dst1(dst1(v, -1), -2)
+
+
+def idst2(v):
+    return idst1(idst1(v, -1), -2)
+
+
+# ----------------------------------------------------------------------------
+# Finite differences (nfgs_solver.py:1361-1465, :1711-1768)
+# ----------------------------------------------------------------------------
+
+
+def _d_axis(f, h, axis):
+    """Second-order centred derivative with one-sided ends (fdo=2 branch)."""
+    f = np.moveaxis(f, axis, -1)
+    out = np.zeros_like(f)
+    n = f.shape[-1]
+    if n < 2:
+        return np.moveaxis(out, -1, axis)
+    if n >= 3:
+        out[..., 1:-1] = (f[..., 2:] - f[..., :-2]) / (2.0 * h)
+        out[..., 0] = (-3.0 * f[..., 0] + 4.0 * f[..., 1] - f[..., 2]) / (2.0 * h)
+        out[..., -1] = (3.0 * f[..., -1] - 4.0 * f[..., -2] + f[..., -3]) / (2.0 * h)
+    else:
+        out[..., 0] = (f[..., 1] - f[..., 0]) / h
+        out[..., -1] = (f[..., -1] - f[..., -2]) / h
+    return np.moveaxis(out, -1, axis)
+
+
+def d_dx(f, dx):
+    """nfgs_solver.py:1361-1379 (also the 2-D twin :1428-1446)."""
+    return _d_axis(f, dx, -1)
+
+
+def d_dy(f, dy):
+    """nfgs_solver.py:1381-1399 (also :1448-1465)."""
+    return _d_axis(f, dy, -2)
+
+
+def d_dz(f, dz):
+    """nfgs_solver.py:1401-1419."""
+    return _d_axis(f, dz, 0)
+
+
+def lap_xy(f, dx, dy):
+    """Mirror-Neumann 5-point lateral Laplacian, nfgs_solver.py:1741-1749."""
+    lap = np.zeros_like(f)
+    lap[..., 1:-1] += (f[..., 2:] - 2 * f[..., 1:-1] + f[..., :-2]) / dx ** 2
+    lap[..., 1:-1, :] += (f[..., 2:, :] - 2 * f[..., 1:-1, :] + f[..., :-2, :]) / dy ** 2
+    lap[..., 0, :] += (2 * f[..., 1, :] - 2 * f[..., 0, :]) / dy ** 2
+    lap[..., -1, :] += (2 * f[..., -2, :] - 2 * f[..., -1, :]) / dy ** 2
+    lap[..., 0] += (2 * f[..., 1] - 2 * f[..., 0]) / dx ** 2
+    lap[..., -1] += (2 * f[..., -2] - 2 * f[..., -1]) / dx ** 2
+    return lap
+
+
+def d2_dxdz(f, dx, dz):
+    """nfgs_solver.py:1729-1733."""
+    d2 = np.zeros_like(f)
+    if f.shape[0] < 3 or f.shape[2] < 3:
+        return d2
+    d2[1:-1, :, 1:-1] = (f[2:, :, 2:] - f[:-2, :, 2:] - f[2:, :, :-2] + f[:-2, :, :-2]) / (4.0 * dx * dz)
+    return d2
+
+
+def d2_dydz(f, dy, dz):
+    """nfgs_solver.py:1735-1739."""
+    d2 = np.zeros_like(f)
+    if f.shape[0] < 3 or f.shape[1] < 3:
+        return d2
+    d2[1:-1, 1:-1, :] = (f[2:, 2:, :] - f[:-2, 2:, :] - f[2:, :-2, :] + f[:-2, :-2, :]) / (4.0 * dy * dz)
+    return d2
+
+
+def pad_or_crop_center_2d(a2, ty, tx):
+    """nfgs_solver.py:1593-1609; returns (out, (y0, y1, x0, x1))."""
+    ny, nx = a2.shape
+    out = np.zeros((ty, tx), dtype=a2.dtype)
+    sy0, sx0 = max(0, (ny - ty) // 2), max(0, (nx - tx) // 2)
+    sy1, sx1 = sy0 + min(ny, ty), sx0 + min(nx, tx)
+    dy0, dx0 = max(0, (ty - ny) // 2), max(0, (tx - nx) // 2)
+    dy1, dx1 = dy0 + (sy1 - sy0), dx0 + (sx1 - sx0)
+    out[dy0:dy1, dx0:dx1] = a2[sy0:sy1, sx0:sx1]
+    return out, (dy0, dy1, dx0, dx1)
+
+
+# ----------------------------------------------------------------------------
+# Bottom Jz (nfgs_solver.py:1498-1567 legacy helper; :4489-4546 dispatch)
+# ----------------------------------------------------------------------------
+
+
+def _cell_sum(a):
+    return 0.25 * (a[1:-1, 2:].sum() + a[2:, 1:-1].sum() + a[1:-1, 1:-1].sum() + a[2:, 2:].sum())
+
+
+def _balance(jz, mode, legacy):
+    pmask = jz >= 0.0
+    pos = np.where(jz >= 0.0, jz, 0.0).astype(jz.dtype)
+    neg = np.where(jz < 0.0, jz, 0.0).astype(jz.dtype)
+    tot, ps, ns = _cell_sum(jz), _cell_sum(pos), _cell_sum(neg)
+    if mode == 2:
+        tgt = ps - 0.5 * tot
+    elif mode == 3:
+        tgt = max(ps, abs(ns))
+    elif mode == 4:
+        tgt = min(ps, abs(ns))
+    else:
+        return jz
+    if legacy:
+        # masks are taken before either rescale (nfgs_solver.py:1556-1561)
+        if abs(ps) > 0.0:
+            jz = np.where(pmask, (tgt / ps) * jz, jz)
+        if abs(ns) > 0.0:
+            jz = np.where(~pmask, (tgt / abs(ns)) * jz, jz)
+        return jz
+    fp = tgt / ps if abs(ps) > 0 else 1.0
+    fn = tgt / abs(ns) if abs(ns) > 0 else 1.0
+    return np.where(jz >= 0, fp * jz, fn * jz)
+
+
+def derive_bottom_jz_legacy(bx, by, dx, dy, nz, ijzbcor=3):
+    """Legacy helper-grid Jz, nfgs_solver.py:1498-1567."""
+    nyo, nxo = bx.shape
+    g = fortran_grid(nxo, nyo, max(2, int(nz)))
+    bxb = np.zeros((g["ny"], g["nx"]), dtype=bx.dtype)
+    byb = np.zeros_like(bxb)
+    y1, x1 = g["bfy"] + nyo, g["bfx"] + nxo
+    bxb[g["bfy"]:y1, g["bfx"]:x1] = bx
+    byb[g["bfy"]:y1, g["bfx"]:x1] = by
+    jzb = np.zeros_like(bxb)
+    if g["nx"] >= 3 and g["ny"] >= 3:
+        jzb[1:-1, 1:-1] = ((byb[1:-1, 2:] - byb[1:-1, :-2]) / (2.0 * dx)
+                           - (bxb[2:, 1:-1] - bxb[:-2, 1:-1]) / (2.0 * dy))
+        if int(ijzbcor) >= 2:
+            jzb = _balance(jzb, int(ijzbcor), legacy=True).astype(bx.dtype)
+    jzb[0, :] = 0.0
+    jzb[-1, :] = 0.0
+    jzb[:, 0] = 0.0
+    jzb[:, -1] = 0.0
+    return jzb[g["bfy"]:y1, g["bfx"]:x1].copy()
+
+
+def derive_bottom_jz_calc(bx, by, dx, dy, ijzbcor=3):
+    """Unpadded-domain Jz on the calculation grid, nfgs_solver.py:4519-4546."""
+    jz = np.zeros_like(bx)
+    ny, nx = bx.shape
+    if nx >= 3 and ny >= 3:
+        jz[1:-1, 1:-1] = ((by[1:-1, 2:] - by[1:-1, :-2]) / (2 * dx)
+                          - (bx[2:, 1:-1] - bx[:-2, 1:-1]) / (2 * dy))
+        if int(ijzbcor) >= 2:
+            jz = _balance(jz, int(ijzbcor), legacy=False).astype(bx.dtype)
+    return jz
+
+
+# ----------------------------------------------------------------------------
+# Batched z solves (nfgs_solver.py:535-608, :5512-5625)
+# ----------------------------------------------------------------------------
+
+
+def thomas_dirichlet(rhs, u0, utop, lam, dz):
+    """_thomas_solve_batched_lowmem with Dirichlet top, nfgs_solver.py:535-608."""
+    Nz = rhs.shape[0]
+    dz2 = dz * dz
+    a = c = 1.0 / dz2
+    b = (lam - 2.0 / dz2).astype(rhs.dtype)
+    u = np.zeros_like(rhs)
+    u[0] = u0
+    u[Nz - 1] = utop
+    n = Nz - 2
+    if n <= 0:
+        return u
+    cs = np.empty((n,) + b.shape, dtype=rhs.dtype)
+    ds = np.empty_like(cs)
+    d0 = rhs[1] - a * u0
+    if n == 1:
+        cs[0] = 0.0
+        ds[0] = (d0 - c * utop) / b
+    else:
+        cs[0] = c / b
+        ds[0] = d0 / b
+        for i in range(1, n):
+            d = rhs[i + 1]
+            ce = c
+            if i == n - 1:
+                d = d - c * utop
+                ce = 0.0
+            den = b - a * cs[i - 1]
+            cs[i] = ce / den
+            ds[i] = (d - a * ds[i - 1]) / den
+    for i in range(n - 2, -1, -1):
+        ds[i] = ds[i] - cs[i] * ds[i + 1]
+    u[1:1 + n] = ds
+    return u
+
+
+def modal_robin_from_hat(src_hat, bottom_hat, lam, k, dz, rho=0.0, dtarget_hat=None, top_hat=None):
+    """Modal (d_zz + lam) u = s, bottom Dirichlet (optionally rho-mixed first
+    row) and eliminated top Robin row; nfgs_solver.py:5512-5625 (returns the
+    modal solution, before the inverse transform)."""
+    s = src_hat.copy()
+    bh = bottom_hat.copy()
+    s[:, 0, 0] = 0.0
+    bh[0, 0] = 0.0
+    g = np.zeros_like(bh) if top_hat is None else top_hat.copy()
+    g[0, 0] = 0.0
+    if rho > 0.0:
+        q = dtarget_hat.copy()
+        q[0, 0] = 0.0
+    Nz = s.shape[0]
+    dz2 = dz * dz
+    lo = up = 1.0 / dz2
+    diag = lam - 2.0 / dz2
+    nu = Nz - 1
+    cp = np.empty((nu,) + lam.shape, dtype=s.dtype)
+    dp = np.empty_like(cp)
+    if rho > 0.0:
+        kb = (1.0 + 0.5 * rho) / dz2
+        fd = (1.0 - rho) * lam - 2.0 / dz2
+        fu = (1.0 - 0.5 * rho) / dz2
+        fr = (1.0 - rho) * s[1]
+        fr = fr - rho * q / dz
+        r0 = fr - kb * bh
+    else:
+        fd, fu = diag, up
+        r0 = s[1] - lo * bh
+    cp[0] = fu / fd
+    dp[0] = r0 / fd
+    for r in range(1, nu - 1):
+        den = diag - lo * cp[r - 1]
+        cp[r] = up / den
+        dp[r] = (s[r + 1] - lo * dp[r - 1]) / den
+    rl = -lam * dz2 - 2.0
+    rd = 2.0 + 2.0 * dz * k
+    rr = 2.0 * dz * g - s[-2] * dz2
+    last = nu - 1
+    den = rd - rl * cp[last - 1]
+    cp[last] = 0.0
+    dp[last] = (rr - rl * dp[last - 1]) / den
+    sol = np.empty_like(s)
+    sol[0] = bh
+    sol[-1] = dp[last]
+    for r in range(last - 1, -1, -1):
+        sol[r + 1] = dp[r] - cp[r] * sol[r + 2]
+    sol[:, 0, 0] = 0.0
+    return sol
+
+
+# ----------------------------------------------------------------------------
+# The solver restatement
+# ----------------------------------------------------------------------------
+
+
+@dataclass
+class OracleResult:
+    history: dict
+    B: np.ndarray
+    J: np.ndarray
+    Phi_c: np.ndarray
+    Az: np.ndarray
+    extras: dict = field(default_factory=dict)
+
+
+class OracleNFGS:
+    """NumPy restatement of NFGSSolverGPU's default path (nfgs_solver.py:3905-7196)."""
+
+    def __init__(self, Nx, Ny, Nz, dx=1.0, dy=1.0, dz=1.0, *, dtype=np.float64,
+                 bottom_divergence_robin_weight=0.0, pad_calculation_domain=True,
+                 calculation_domain_mode="mudpack", weak_field_floor_b2=1e-20,
+                 alpha_limit_factor=1.0, source_zero_mode_policy="error",
+                 source_zero_mode_tolerance=1e-2, d0_zero_mode_policy="error",
+                 mean_mode_tolerance=1e-6):
+        self.input_shape_zyx = (int(Nz), int(Ny), int(Nx))
+        cx, cy, cz, pad_xy = calculation_grid(int(Nx), int(Ny), int(Nz), pad_calculation_domain,
+                                              calculation_domain_mode)
+        if cz < 3:
+            raise ValueError("NFGS requires Nz >= 3")
+        self.Nx, self.Ny, self.Nz = cx, cy, cz
+        self.pad_xy = pad_xy
+        self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+        self.dtype = np.dtype(dtype)
+        self.rho = float(bottom_divergence_robin_weight)
+        self.floor_b2 = float(weak_field_floor_b2)
+        self.alpha_max = float(alpha_limit_factor) / min(self.dx, self.dy, self.dz)
+        self.src_policy = source_zero_mode_policy
+        self.src_tol = float(source_zero_mode_tolerance)
+        self.d0_policy = d0_zero_mode_policy
+        self.mean_tol = float(mean_mode_tolerance)
+        self.zero_mode_history = []
+
+    # -- tables (nfgs_solver.py:1083-1118, :4357-4383) --
+    def lam_neumann(self):
+        kx = np.arange(self.Nx, dtype=self.dtype)
+        ky = np.arange(self.Ny, dtype=self.dtype)
+        lx = -4.0 * np.sin(np.pi * kx / (2.0 * max(self.Nx - 1, 1))) ** 2 / self.dx ** 2
+        ly = -4.0 * np.sin(np.pi * ky / (2.0 * max(self.Ny - 1, 1))) ** 2 / self.dy ** 2
+        return (ly[:, None] + lx[None, :]).astype(self.dtype)
+
+    def lam_dirichlet(self):
+        nxi, nyi = self.Nx - 2, self.Ny - 2
+        kx = np.arange(1, nxi + 1, dtype=self.dtype)
+        ky = np.arange(1, nyi + 1, dtype=self.dtype)
+        lx = -4.0 * np.sin(np.pi * kx / (2.0 * (nxi + 1))) ** 2 / self.dx ** 2
+        ly = -4.0 * np.sin(np.pi * ky / (2.0 * (nyi + 1))) ** 2 / self.dy ** 2
+        return (ly[:, None] + lx[None, :]).astype(self.dtype)
+
+    def alpha_x(self):
+        kx = np.arange(self.Nx, dtype=self.dtype)
+        return (2.0 * np.sin(np.pi * kx / (2.0 * max(self.Nx - 1, 1))) / self.dx).astype(self.dtype)
+
+    def alpha_y(self):
+        ky = np.arange(self.Ny, dtype=self.dtype)
+        return (2.0 * np.sin(np.pi * ky / (2.0 * max(self.Ny - 1, 1))) / self.dy).astype(self.dtype)
+
+    def _const_mode(self, hat):
+        """nfgs_solver.py:4462-4468."""
+        return hat[..., 0, 0] / (4.0 * max(self.Nx - 1, 1) * max(self.Ny - 1, 1))
+
+    # -- spectral horizontal operators (nfgs_solver.py:4385-4457) --
+    def spectral_dx_2d(self, v):
+        hat = dct2(v)
+        sine = -self.alpha_x() * hat
+        tmp = idct1(sine, -2)
+        out = np.zeros_like(v)
+        if v.shape[-1] >= 3:
+            out[..., 1:-1] = idst1(tmp[..., 1:-1], -1)
+        return out
+
+    def spectral_dy_2d(self, v):
+        hat = dct2(v)
+        sine = -self.alpha_y()[:, None] * hat
+        tmp = idct1(sine, -1)
+        out = np.zeros_like(v)
+        if v.shape[-2] >= 3:
+            out[..., 1:-1, :] = idst1(tmp[..., 1:-1, :], -2)
+        return out
+
+    def dstx_dcty_hat(self, bx):
+        out = np.zeros_like(bx)
+        if bx.shape[-1] < 3:
+            return out
+        tmp = dct1(bx, -2)
+        out[..., 1:-1] = dst1(tmp[..., 1:-1], -1)
+        return out
+
+    def dctx_dsty_hat(self, by):
+        out = np.zeros_like(by)
+        if by.shape[-2] < 3:
+            return out
+        tmp = dct1(by, -1)
+        out[..., 1:-1, :] = dst1(tmp[..., 1:-1, :], -2)
+        return out
+
+    def spectral_div_hat(self, bx, by):
+        return self.alpha_x() * self.dstx_dcty_hat(bx) + self.alpha_y()[:, None] * self.dctx_dsty_hat(by)
+
+    # -- 2-D Poisson (nfgs_solver.py:1132-1179) --
+    def poisson2d_dirichlet(self, src):
+        out = np.zeros_like(src)
+        if self.Nx < 3 or self.Ny < 3:
+            return out
+        h = dst2(src[..., 1:-1, 1:-1])
+        h = h / self.lam_dirichlet()
+        out[..., 1:-1, 1:-1] = idst1(idst1(h, -1), -2)
+        return out
+
+    def poisson2d_neumann(self, src):
+        h = dct2(src)
+        ls = self.lam_neumann().copy()
+        ls[0, 0] = 1.0
+        h = h / ls
+        h[..., 0, 0] = 0.0
+        return idct1(idct1(h, -1), -2)
+
+    # ------------------------------------------------------------------
+    # Boundary setup (nfgs_solver.py:1999-2110, :3024-3057, :4548-4919)
+    # ------------------------------------------------------------------
+    def set_bottom_vector_boundary(self, Bx, By, Bz, ijzbcor=3):
+        dt = self.dtype
+        bx, by, bz = (np.asarray(a, dtype=dt) for a in (Bx, By, Bz))
+        if bz.ndim == 3:
+            bx, by, bz = bx[0], by[0], bz[0]
+        if bz.shape != (self.Ny, self.Nx):
+            obs_ny, obs_nx = bz.shape
+            bz, fov = pad_or_crop_center_2d(bz, self.Ny, self.Nx)
+            bx, _ = pad_or_crop_center_2d(bx, self.Ny, self.Nx)
+            by, _ = pad_or_crop_center_2d(by, self.Ny, self.Nx)
+            self.fov = fov
+            self.observed_zyx = (min(self.Nz, self.input_shape_zyx[0]), obs_ny, obs_nx)
+        else:
+            self.fov = (0, self.Ny, 0, self.Nx)
+            self.observed_zyx = (self.Nz, self.Ny, self.Nx)
+        # padding dispatch of nfgs_solver.py:4503-4516
+        calc = (self.Nz, self.Ny, self.Nx)
+        padded = any(c > o for c, o in zip(calc, self.input_shape_zyx)) or \
+            any(c > o for c, o in zip(calc, self.observed_zyx)) or any(p > 0 for p in self.pad_xy)
+        if padded:
+            self.bottom_current_grid_mode = "legacy_helper"
+            jz = derive_bottom_jz_legacy(bx, by, self.dx, self.dy, self.Nz, ijzbcor)
+        else:
+            self.bottom_current_grid_mode = "calculation_grid"
+            jz = derive_bottom_jz_calc(bx, by, self.dx, self.dy, ijzbcor)
+        self.Bx_obs, self.By_obs, self.Bz_obs, self.Jz_obs = bx, by, bz, jz.astype(dt)
+        # Az_bottom and the toroidal bottom current (nfgs_solver.py:2105, :3037-3056)
+        self.Az_bottom = self.poisson2d_dirichlet(-self.Jz_obs)
+        jzt = -lap_xy(self.Az_bottom, self.dx, self.dy)
+        jzt[:, 0] = 0.0
+        jzt[:, -1] = 0.0
+        jzt[0, :] = 0.0
+        jzt[-1, :] = 0.0
+        self.Jz_T = jzt
+        rms = lambda v: float(np.sqrt(np.mean(np.asarray(v, dtype=np.float64) ** 2)))
+        self.toroidal_diagnostics = {
+            "raw_jz_rms": float(np.sqrt(np.mean(self.Jz_obs ** 2))),
+            "t_implied_jz_rms": float(np.sqrt(np.mean(jzt ** 2))),
+            "raw_vs_t_implied_jz_rms": float(np.sqrt(np.mean((self.Jz_obs - jzt) ** 2))),
+        }
+        self._gaussian_background()
+        self.zero_mode_history = []
+        self.last_zero_mode_by_height = []
+
+    def _gaussian_background(self):
+        """nfgs_solver.py:4671-4919 (uniform_open, spectral operator, no external field)."""
+        dt = self.dtype
+        bxh, byh, bzh = dct2(self.Bx_obs), dct2(self.By_obs), dct2(self.Bz_obs)
+        self.B_mean = np.array([self._const_mode(bxh), self._const_mode(byh), self._const_mode(bzh)])
+        rbz_hat = dct2(self.Bz_obs)
+        self.Bz_open_mean = dt.type(self._const_mode(rbz_hat))
+        flux_imb = float(abs(self.Bz_obs.sum()) / max(float(np.abs(self.Bz_obs).sum()), 1e-30))
+        nonmean = rbz_hat.copy()
+        nonmean[0, 0] = 0.0
+        d_hat = self.spectral_div_hat(self.Bx_obs, self.By_obs)
+        self.D0 = idct2(d_hat)
+        d0_mean = self._const_mode(d_hat)
+        d0_rms = np.sqrt(np.mean(self.D0 ** 2))
+        d0_ratio = float(abs(d0_mean) / max(d0_rms, 1e-30))
+        self._zero_mode_policy(d0_ratio, self.mean_tol, self.d0_policy,
+                               "The bottom vector field is incompatible with the zero-mean "
+                               "endpoint-Neumann GS model")
+        d_hat[0, 0] = 0.0
+        lam = self.lam_neumann()
+        k = np.sqrt(np.clip(-lam, 0.0, None)).astype(dt)
+        ks = k.copy()
+        ks[0, 0] = 1.0
+        ls = lam.copy()
+        ls[0, 0] = 1.0
+        self.k = k
+        dphi_total_hat = -d_hat / ls
+        dphi_total_hat[0, 0] = 0.0
+        bzb = 0.5 * (nonmean + d_hat / ks)
+        bza = 0.5 * (nonmean - d_hat / ks)
+        bzb[0, 0] = 0.0
+        bza[0, 0] = 0.0
+        pbh = bzb / ls
+        pah = bza / ls
+        pbh[0, 0] = 0.0
+        pah[0, 0] = 0.0
+        self.phi_below_hat, self.phi_above_hat = pbh, pah
+        self.Bz_below = idct2(bzb)
+        self.Bz_above = idct2(bza)
+        self.Phi_below_bottom = idct2(pbh)
+        self.Phi_above_bottom = idct2(pah)
+        z = np.arange(self.Nz, dtype=dt) * dt.type(self.dz)
+        decay = np.exp(-z[:, None, None] * k[None, :, :])
+        phi_below = idct2(decay * pbh[None])
+        bz_below = idct2(decay * bzb[None])
+        dphi_dz = idct2((-k)[None] * decay * pbh[None])
+        gx, gy = self.spectral_dx_2d(dphi_dz), self.spectral_dy_2d(dphi_dz)
+        bg = np.stack([-gx, -gy, bz_below])
+        bg[2] = bg[2] + self.Bz_open_mean  # B_below + B_open (+ zero external)
+        self.B_background = bg.astype(dt)
+        self.dphi_target = (idct2(dphi_total_hat) - d_dz(phi_below[:3], self.dz)[0]).astype(dt)
+        recon = self.B_background[2, 0] + self.Bz_above - self.Bz_obs
+        b_from_phi = np.stack([-gx, -gy, lap_xy(phi_below, self.dx, self.dy)])
+        b_from_phi[2] = b_from_phi[2] + self.Bz_open_mean
+        inv = b_from_phi - self.B_background
+        self.gaussian_diagnostics = {
+            "bottom_reconstruction_rms": float(np.sqrt(np.mean(recon ** 2))),
+            "bottom_reconstruction_max": float(np.abs(recon).max()),
+            "dct_zero_mode_Bx": float(self.B_mean[0]),
+            "dct_zero_mode_By": float(self.B_mean[1]),
+            "dct_zero_mode_Bz": float(self.B_mean[2]),
+            "Bz_open_mean": float(self.Bz_open_mean),
+            "flux_imbalance_fraction": flux_imb,
+            "D0_zero_mode": float(d0_mean),
+            "D0_zero_mode_ratio": d0_ratio,
+            "d0_removed_constant_mode": float(d0_mean),
+            "background_scalar_plus_explicit_rms": float(np.sqrt(np.mean(inv ** 2))),
+        }
+        self.gaussian_diagnostics.update(self._harmonic_diagnostics())
+        # frozen linear harmonic (nfgs_solver.py:5069-5096)
+        self.H = np.repeat(self.H_target[:, None], self.Nz, axis=1).astype(dt)
+
+    def _harmonic_diagnostics(self):
+        """nfgs_solver.py:4990-5056."""
+        dphi = idct2(self.k * (self.phi_above_hat - self.phi_below_hat))
+        gx, gy = self.spectral_dx_2d(dphi), self.spectral_dy_2d(dphi)
+        btx = d_dy(self.Az_bottom, self.dy)
+        bty = -d_dx(self.Az_bottom, self.dx)
+        rec = np.stack([-gx + btx, -gy + bty])
+        obs = np.stack([self.Bx_obs, self.By_obs])
+        res = obs - rec
+        const = np.array([self._const_mode(dct2(res[0])), self._const_mode(dct2(res[1]))],
+                         dtype=self.dtype)
+        self.H_target = const
+        nonconst = res - const[:, None, None]
+        div = idct2(self.spectral_div_hat(res[0], res[1]))
+        curl = d_dx(res[1], self.dx) - d_dy(res[0], self.dy)
+        rms = lambda v: float(np.sqrt(np.mean(v ** 2)))
+        ratio = rms(res) / max(rms(obs), 1e-30)
+        return {
+            "bottom_horizontal_reconstruction_rms": rms(res),
+            "bottom_horizontal_reconstruction_ratio": ratio,
+            "harmonic_residual_constant_rms": rms(const),
+            "harmonic_residual_nonconstant_rms": rms(nonconst),
+            "harmonic_residual_divergence_rms": rms(div),
+            "harmonic_residual_curl_rms": rms(curl),
+        }
+
+    def _zero_mode_policy(self, ratio, tol, policy, message):
+        """nfgs_solver.py:4477-4487."""
+        if ratio <= tol:
+            return
+        detail = f"{message} ratio={ratio:.6e} exceeds tolerance={tol:.6e}."
+        if policy == "error":
+            raise ValueError(detail)
+        if policy == "warn_project":
+            import warnings
+            warnings.warn(detail + " The incompatible zero mode will be projected out.",
+                          RuntimeWarning, stacklevel=3)
+
+    # ------------------------------------------------------------------
+    # Scalar solves (nfgs_solver.py:3263-3306, :5457-5476, :5627-5672)
+    # ------------------------------------------------------------------
+    def diagnose_zero_mode(self, src):
+        means = self._const_mode(dct2(src))
+        src_rms = np.sqrt(np.mean(src ** 2))
+        zero_rms = np.sqrt(np.mean(means ** 2))
+        ratio = float(zero_rms / max(src_rms, 1e-30))
+        self.last_zero_mode_by_height = [float(v) for v in means]
+        self.zero_mode_history.append(ratio)
+        self._zero_mode_policy(ratio, self.src_tol, self.src_policy,
+                               "The Phi_c nonlinear source has an incompatible zero mode")
+        return ratio
+
+    def solve_phi_c(self, src):
+        lam = self.lam_neumann()
+        ls = lam.copy()
+        ls[0, 0] = 1.0
+        self.diagnose_zero_mode(src)
+        sh = dct2(src) / ls[None]
+        sh[:, 0, 0] = 0.0
+        bh = dct2(self.Phi_above_bottom)
+        bh[0, 0] = 0.0
+        qh = None
+        if self.rho > 0.0:
+            qh = dct2(self.dphi_target)
+            qh[0, 0] = 0.0
+        sol = modal_robin_from_hat(sh, bh, lam, self.k, self.dz, self.rho, qh)
+        phi = idct2(sol)
+        if self.rho > 0.0:
+            phi[0] = self.Phi_above_bottom
+        return phi
+
+    def solve_az(self, src):
+        Az = np.zeros_like(src)
+        Az[0] = self.Az_bottom
+        Az[-1] = 0.0
+        lam = self.lam_dirichlet()
+        sh = dst2(src[:, 1:-1, 1:-1])
+        rh = -sh / lam
+        bh = dst2(self.Az_bottom[1:-1, 1:-1])
+        th = np.zeros_like(bh)
+        ah = thomas_dirichlet(rh, bh, th, lam, self.dz)
+        Az[:, 1:-1, 1:-1] = idst2(ah)
+        Az[0] = self.Az_bottom
+        Az[-1] = 0.0
+        self._az_lateral_zero(Az)
+        return Az
+
+    @staticmethod
+    def _az_lateral_zero(Az):
+        Az[:, :, 0] = 0.0
+        Az[:, :, -1] = 0.0
+        Az[:, 0, :] = 0.0
+        Az[:, -1, :] = 0.0
+
+    # ------------------------------------------------------------------
+    # Staggered-face current (nfgs_solver.py:5738-5839)
+    # ------------------------------------------------------------------
+    def face_state(self, B, Jz):
+        dz = self.dz
+        Bf = 0.5 * (B[:, 1:] + B[:, :-1])
+        c = B - self.B_background
+        cf = 0.5 * (c[:, 1:] + c[:, :-1])
+        dbx = (c[0, 1:] - c[0, :-1]) / dz
+        dby = (c[1, 1:] - c[1, :-1]) / dz
+        jx = d_dy(cf[2], self.dy) - dby
+        jy = dbx - d_dx(cf[2], self.dx)
+        jz = 0.5 * (Jz[1:] + Jz[:-1])
+        return Bf, np.stack([jx, jy, jz])
+
+    @staticmethod
+    def faces_to_centres(v):
+        out = np.empty((v.shape[0], v.shape[1] + 1) + v.shape[2:], dtype=v.dtype)
+        out[:, 0] = v[:, 0]
+        out[:, -1] = v[:, -1]
+        out[:, 1:-1] = 0.5 * (v[:, :-1] + v[:, 1:])
+        return out
+
+    def parallel_current(self, J, B):
+        Bf, Jf = self.face_state(B, J[2])
+        b2 = np.maximum(np.sum(Bf * Bf, axis=0), self.floor_b2)
+        alpha = np.sum(Jf * Bf, axis=0) / b2
+        alpha = np.clip(alpha, -self.alpha_max, self.alpha_max)
+        return self.faces_to_centres(alpha[None] * Bf)
+
+    # ------------------------------------------------------------------
+    # Metrics (nfgs_solver.py:2656-2697, :5882-5895)
+    # ------------------------------------------------------------------
+    def window(self, nz_faces):
+        nzo, nyo, nxo = self.observed_zyx
+        bfy, bfx = self.fov[0], self.fov[2]
+        bfz = max(0, nz_faces - int(nzo))
+        ax, ay, az = nxo // 8 + bfx, nyo // 8 + bfy, nzo // 4 + bfz
+        sz = slice(1, max(1, nz_faces - 1 - az + 1))
+        sy = slice(ay, max(ay + 1, self.Ny - 1 - ay + 1))
+        sx = slice(ax, max(ax + 1, self.Nx - 1 - ax + 1))
+        return sz, sy, sx
+
+    def sigma(self, B, J):
+        Bf, Jf = self.face_state(B, J[2])
+        sz, sy, sx = self.window(Bf.shape[1])
+        b = Bf[:, sz, sy, sx]
+        j = Jf[:, sz, sy, sx]
+        bn = np.maximum(np.sqrt(np.sum(b * b, axis=0)), 1e-10)
+        jn = np.sqrt(np.sum(j * j, axis=0))
+        jxb = np.cross(np.moveaxis(j, 0, -1), np.moveaxis(b, 0, -1))
+        num = (np.sqrt(np.sum(jxb * jxb, axis=-1)) / bn).sum()
+        den = max(jn.sum(), 1e-30)
+        return float(num / den)
+
+    def divB(self, B):
+        div = d_dx(B[0], self.dx) + d_dy(B[1], self.dy) + d_dz(B[2], self.dz)
+        bm = np.sqrt(B[0] ** 2 + B[1] ** 2 + B[2] ** 2)
+        delta = min(self.dx, self.dy, self.dz)
+        return float(np.abs(div).mean() / (6.0 * bm.mean() / delta + 1e-12))
+
+    def energy(self, B):
+        return float((B[0] ** 2 + B[1] ** 2 + B[2] ** 2).sum() * self.dx * self.dy * self.dz / 2.0)
+
+    # ------------------------------------------------------------------
+    # Reconstruction, init and one step (nfgs_solver.py:3311-3341,
+    # :5901-5942, :6069-6078, :6613-6935)
+    # ------------------------------------------------------------------
+    def _apply_bc(self, B, J):
+        J[:, :, :, 0] = 0.0
+        J[:, :, :, -1] = 0.0
+        J[:, :, 0, :] = 0.0
+        J[:, :, -1, :] = 0.0
+        J[:, -1] = 0.0
+        B[2, 0] = self.Bz_obs
+        J[2, 0] = self.Jz_T
+
+    def k0_current(self):
+        """nfgs_solver.py:5400-5455 (centre derivative of the constant profile)."""
+        p = self.H
+        d = np.zeros_like(p)
+        face = (p[:, 1:] - p[:, :-1]) / self.dz
+        d[:, 1:-1] = 0.5 * (face[:, :-1] + face[:, 1:])
+        d[:, 0] = (-3.0 * p[:, 0] + 4.0 * p[:, 1] - p[:, 2]) / (2.0 * self.dz)
+        d[:, -1] = (3.0 * p[:, -1] - 4.0 * p[:, -2] + p[:, -3]) / (2.0 * self.dz)
+        return d
+
+    def reconstruct(self, Phi_c, Az, Bc_z):
+        dx, dy, dz = self.dx, self.dy, self.dz
+        dphi = d_dz(Phi_c, dz)
+        bcx = -d_dx(dphi, dx) + d_dy(Az, dy)
+        bcy = -d_dy(dphi, dy) - d_dx(Az, dx)
+        jz = -lap_xy(Az, dx, dy)
+        B = self.B_background + np.stack([bcx, bcy, Bc_z])
+        # Jx/Jy of the PT reconstruction (nfgs_solver.py:1760-1768) are
+        # replaced by the staggered-face current below, so zeros stand in.
+        J = np.stack([np.zeros_like(jz), np.zeros_like(jz), jz])
+        self._apply_bc(B, J)
+        d = self.k0_current()
+        B[0] = B[0] + self.H[0][:, None, None]
+        B[1] = B[1] + self.H[1][:, None, None]
+        B[2, 0] = self.Bz_obs
+        J[2, 0] = self.Jz_T
+        J[:, -1] = 0.0
+        del d  # contributes only to Jx/Jy, which are overwritten
+        Bf, Jf = self.face_state(B, J[2])
+        Js = self.faces_to_centres(Jf)
+        J[:2] = Js[:2]
+        J[:, :, :, 0] = 0.0
+        J[:, :, :, -1] = 0.0
+        J[:, :, 0, :] = 0.0
+        J[:, :, -1, :] = 0.0
+        J[:, -1] = 0.0
+        J[2, 0] = self.Jz_T
+        self.last_face_diag = {
+            "face_current_rms": float(np.sqrt(np.mean(Jf.astype(np.float64) ** 2))),
+            "projected_cell_current_rms": float(np.sqrt(np.mean(Js.astype(np.float64) ** 2))),
+        }
+        return B, J
+
+    def initial_state(self):
+        z = np.zeros((self.Nz, self.Ny, self.Nx), dtype=self.dtype)
+        phi = self.solve_phi_c(z)
+        az = self.solve_az(z)
+        B, J = self.reconstruct(phi, az, lap_xy(phi, self.dx, self.dy))
+        return B, J, phi, az
+
+    def sources(self, B, J):
+        """Parallel current, bottom row, and both Poisson sources (nfgs_solver.py:6694-6745)."""
+        jp = self.parallel_current(J, B)
+        jp[:, 0] = J[:, 0]
+        s_bcz = d_dy(jp[0], self.dy) - d_dx(jp[1], self.dx)
+        s_jz = lap_xy(jp[2], self.dx, self.dy) - d2_dxdz(jp[0], self.dx, self.dz) \
+            - d2_dydz(jp[1], self.dy, self.dz)
+        return jp, s_bcz, s_jz
+
+    def step(self, B, J, Phi_c, Az):
+        _, s_bcz, s_jz = self.sources(B, J)
+        phi = self.solve_phi_c(s_bcz)
+        az = self.solve_az(s_jz)
+        phi[0] = self.Phi_above_bottom
+        az[0] = self.Az_bottom
+        az[-1] = 0.0
+        self._az_lateral_zero(az)
+        B2, J2 = self.reconstruct(phi, az, lap_xy(phi, self.dx, self.dy))
+        return B2, J2, phi, az
+
+    # ------------------------------------------------------------------
+    # Driver (nfgs_solver.py:3522-3850, :6937-7009)
+    # ------------------------------------------------------------------
+    def iterate(self, max_iter=200, tol=1e-5, patience=1, sigma_improvement_tol=1e-5,
+                callback=None):
+        B, J, phi, az = self.initial_state()
+        s0, d0, e0 = self.sigma(B, J), self.divB(B), self.energy(B)
+        h = {k: [] for k in ("sigma", "divB", "energy", "mu", "rollback", "sigma_relative_change",
+                             "energy_relative_change", "fixed_point_criteria_met")}
+        best_sigma, best_iter = s0, 0
+        best = (B.copy(), J.copy(), phi.copy(), az.copy())
+        stall = flat = 0
+        need = max(int(patience), 1)
+        prev = s0
+        use_imp = sigma_improvement_tol is not None and float(sigma_improvement_tol) > 0.0
+        eps = float(sigma_improvement_tol) if use_imp else 0.0
+        last_imp = None
+        reason = "max_iter"
+        final_iter = 0
+        for n in range(1, max_iter + 1):
+            B, J, phi, az = self.step(B, J, phi, az)
+            s = self.sigma(B, J)
+            d, e = self.divB(B), self.energy(B)
+            pe = h["energy"][-1] if h["energy"] else e0
+            h["sigma"].append(s)
+            h["divB"].append(d)
+            h["energy"].append(e)
+            h["mu"].append(1.0)
+            h["rollback"].append(0)
+            h["sigma_relative_change"].append(abs(s - prev) / max(abs(prev), 1e-30))
+            h["energy_relative_change"].append(abs(e - pe) / max(abs(pe), 1e-30))
+            h["fixed_point_criteria_met"].append(0)
+            final_iter = n
+            if callback is not None:
+                callback(n, s, B, J)
+            imp = prev - s
+            last_imp = imp
+            prev = s
+            flat = flat + 1 if imp < eps else 0
+            if s < best_sigma:
+                best_sigma, best_iter = s, n
+                best = (B.copy(), J.copy(), phi.copy(), az.copy())
+                stall = 0
+            else:
+                stall += 1
+            if s < tol:
+                reason = "tol"
+                break
+            if use_imp and flat >= need:
+                reason = "sigma_improvement_tol"
+                break
+            if patience > 0 and stall >= patience:
+                reason = "sigma_stall"
+                break
+        final = (B, J)
+        fvb = np.sqrt(np.mean((final[0].astype(np.float64) - best[0].astype(np.float64)) ** 2)) / \
+            max(np.sqrt(np.mean(best[0].astype(np.float64) ** 2)), 1e-30)
+        h.update(best_sigma=best_sigma, best_iter=best_iter, rollbacks=0, stop_reason=reason,
+                 stop_mode="sigma_best", selected_state="best_sigma", final_iter=final_iter,
+                 final_sigma=float(self.sigma(*final)), final_vs_best_B_relative=float(fvb),
+                 sigma_improvement_tol=eps if use_imp else None, last_sigma_improvement=last_imp,
+                 initial_sigma=s0, initial_divB=d0, initial_energy=e0)
+        self.B, self.J, self.Phi_c, self.Az = best
+        # post-selection staggered diagnostic (nfgs_solver.py:7005-7007)
+        Bf, Jf = self.face_state(self.B, self.J[2])
+        Js = self.faces_to_centres(Jf)
+        self.last_face_diag = {
+            "face_current_rms": float(np.sqrt(np.mean(Jf.astype(np.float64) ** 2))),
+            "projected_cell_current_rms": float(np.sqrt(np.mean(Js.astype(np.float64) ** 2))),
+        }
+        h["source_diagnostics"] = self.source_diagnostics(self.B)
+        return h
+
+    def fields_to_numpy(self, crop_to_observed_fov=True):
+        """nfgs_solver.py:2881-2903."""
+        B, J = self.B, self.J
+        if crop_to_observed_fov:
+            nzo = self.observed_zyx[0]
+            y0, y1, x0, x1 = self.fov
+            B = B[:, :nzo, y0:y1, x0:x1]
+            J = J[:, :nzo, y0:y1, x0:x1]
+        return {"Bx": B[0].copy(), "By": B[1].copy(), "Bz": B[2].copy(),
+                "Jx": J[0].copy(), "Jy": J[1].copy(), "Jz": J[2].copy()}
+
+    # ------------------------------------------------------------------
+    # Source diagnostics (nfgs_solver.py:7014-7196; numeric entries)
+    # ------------------------------------------------------------------
+    def source_diagnostics(self, B):
+        dz = self.dz
+        bcz = B[2] - self.B_background[2]
+        recon = self.B_background[2, 0] + self.Bz_above - self.Bz_obs
+        corr = bcz[0] - self.Bz_above
+        bc_hat = dct2(bcz[-3:])
+        der = (3.0 * bc_hat[-1] - 4.0 * bc_hat[-2] + bc_hat[-3]) / (2.0 * dz)
+        rh = der + self.k * bc_hat[-1]
+        rh[0, 0] = 0.0
+        robin = idct2(rh)
+        phi = np.empty_like(bcz)
+        phi[0] = self.Phi_above_bottom
+        phi[1:] = self.poisson2d_neumann(bcz[1:])
+        ph = dct2(phi[-3:])
+        pder = (3.0 * ph[-1] - 4.0 * ph[-2] + ph[-3]) / (2.0 * dz)
+        prh = pder + self.k * ph[-1]
+        prh[0, 0] = 0.0
+        probin = idct2(prh)
+        ks = self.k.copy()
+        ks[0, 0] = 1.0
+        uh = 0.5 * (bc_hat[-1] + der / ks)
+        uh[0, 0] = 0.0
+        up = idct2(uh)
+        rms = lambda v: float(np.sqrt(np.mean(np.asarray(v) ** 2)))
+        div = d_dx(B[0, 0], self.dx) + d_dy(B[1, 0], self.dy)
+        derr = div - self.D0
+        dscale = max(np.sqrt(np.mean(self.D0 ** 2)), 1e-30)
+        dphi = (-3.0 * phi[0] + 4.0 * phi[1] - phi[2]) / (2.0 * dz)
+        tor = self.toroidal_diagnostics
+        return {
+            "bottom_reconstruction_rms": rms(recon),
+            "bottom_reconstruction_max": float(np.abs(recon).max()),
+            "correction_bottom_rms": rms(corr),
+            "top_robin_rms": rms(robin),
+            "top_phi_robin_rms": rms(probin),
+            "top_upward_source_rms": rms(up),
+            "top_upward_source_ratio": float(rms(up) / max(rms(bcz[-1]), 1e-30)),
+            "source_zero_mode_ratio_last": self.zero_mode_history[-1] if self.zero_mode_history else 0.0,
+            "source_zero_mode_ratio_max": max(self.zero_mode_history) if self.zero_mode_history else 0.0,
+            "bottom_divergence_rms": rms(derr),
+            "bottom_divergence_relative": float(np.sqrt(np.mean(derr ** 2)) / dscale),
+            "bottom_dphi_residual_rms": rms(dphi - self.dphi_target),
+            "Jz_projected_out_rms": tor["raw_vs_t_implied_jz_rms"],
+            "B_harm_over_Bobs_rms": self.gaussian_diagnostics["bottom_horizontal_reconstruction_ratio"],
+        }

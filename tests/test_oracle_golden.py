@@ -1,0 +1,65 @@
+"""The oracle against the reference's golden vectors (tests/golden/*.npz)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.nfgs_oracle import OracleNFGS
+from tests.golden_cases import case_files, load, rel_l2
+
+CASES = case_files()
+
+
+def _oracle(meta):
+    c = meta["case"]
+    nxo, nyo, nz = c["observed"]
+    return OracleNFGS(nxo, nyo, nz, 1.0, 1.0, 1.0, dtype=np.dtype(c["dtype"]),
+                      bottom_divergence_robin_weight=c["rho"], pad_calculation_domain=c["pad"],
+                      calculation_domain_mode=c["mode"], source_zero_mode_policy="project",
+                      d0_zero_mode_policy="project")
+
+
+@pytest.mark.parametrize("path", CASES, ids=[os.path.basename(p)[:-4] for p in CASES])
+def test_oracle_matches_reference_fixture(path):
+    g, meta = load(path)
+    c = meta["case"]
+    tol = 1e-10 if c["dtype"] == "float64" else 2e-4
+    o = _oracle(meta)
+    o.set_bottom_vector_boundary(g["in_bx"], g["in_by"], g["in_bz"])
+    assert (o.Nz, o.Ny, o.Nx) == tuple(g["calc_shape_zyx"])
+    assert tuple(o.fov) == tuple(g["fov_yx"])
+    assert tuple(o.observed_zyx) == tuple(g["observed_zyx"])
+    for key, val in [("Jz_obs", o.Jz_obs), ("Az_bottom", o.Az_bottom), ("Jz_T", o.Jz_T),
+                     ("B_background", o.B_background), ("Phi_above_bottom", o.Phi_above_bottom),
+                     ("Bz_above", o.Bz_above), ("D0", o.D0), ("dphi_target", o.dphi_target)]:
+        assert rel_l2(val, g[key]) < tol, key
+    # H is a small residual constant: compare absolutely against the field scale
+    assert np.max(np.abs(o.H - g["H_k0"])) < tol * np.max(np.abs(g["Bz_obs"]))
+    st = o.initial_state()
+    for k, v in zip(("B", "J", "Phi_c", "Az"), st):
+        assert rel_l2(v, g["init_" + k]) < tol, "init_" + k
+    for step in (1, 2):
+        st = o.step(*st)
+        for k, v in zip(("B", "J", "Phi_c", "Az"), st):
+            assert rel_l2(v, g[f"step{step}_{k}"]) < tol, f"step{step}_{k}"
+        assert abs(o.sigma(st[0], st[1]) - float(g[f"step{step}_sigma"])) < tol
+        assert abs(o.divB(st[0]) - float(g[f"step{step}_divB"])) < tol * 10
+        assert abs(o.energy(st[0]) / float(g[f"step{step}_energy"]) - 1) < tol
+        assert abs(o.zero_mode_history[-1] - float(g[f"step{step}_zero_mode_ratio"])) < max(tol, 1e-6)
+    it = c["iterate"]
+    h = o.iterate(max_iter=it["max_iter"], tol=it["tol"], patience=it["patience"],
+                  sigma_improvement_tol=it["sigma_improvement_tol"])
+    assert h["best_iter"] == meta["best_iter"]
+    assert h["stop_reason"] == meta["stop_reason"]
+    assert h["final_iter"] == meta["final_iter"]
+    for key in ("sigma", "divB", "energy"):
+        assert rel_l2(h[key], g["hist_" + key]) < tol, key
+    out = o.fields_to_numpy()
+    for k, v in out.items():
+        assert v.shape == g["out_" + k].shape
+        assert rel_l2(v, g["out_" + k]) < tol, k
+    if c["dtype"] == "float64":
+        sd = h["source_diagnostics"]
+        for k, v in meta["source_diagnostics"].items():
+            assert abs(sd[k] - v) <= 1e-8 * max(1.0, abs(v)), k
+        assert o.bottom_current_grid_mode == ("calculation_grid" if not c["pad"] else "legacy_helper")
