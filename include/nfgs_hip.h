@@ -1,0 +1,231 @@
+/*
+ * nfgs_hip.h — C-ABI of the MI355X (gfx950) NFGS plain-iterate library.
+ *
+ * The reference exposes this path only as the Python class NFGSSolverGPU
+ * (nfgs_solver.py:3905; public API __all__ at :29).  Each entry point below
+ * replaces one stage of that class, cited per function.  All buffers are
+ * caller-owned device pointers (plain pointers, element counts, strides in
+ * elements), arrays are C-contiguous in (component, z, y, x) order with x
+ * fastest, and the element type is the plan's dtype (float or double).
+ * Work is enqueued on the given HIP stream (NULL = default stream) and is
+ * asynchronous; nothing here allocates device memory after plan creation.
+ *
+ * Every function returns NFGS_OK (0) or a negative error code; the text of
+ * the last error on the calling thread is returned by nfgs_last_error().
+ * A plan is not re-entrant; distinct plans may be used from distinct threads.
+ */
+#ifndef NFGS_HIP_H
+#define NFGS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  NFGS_OK = 0,
+  NFGS_ERR_ARG = -1,   /* invalid argument (maps to ValueError)            */
+  NFGS_ERR_HIP = -2,   /* HIP runtime failure                              */
+  NFGS_ERR_STATE = -3, /* call-order violation (maps to RuntimeError)      */
+  NFGS_ERR_ALLOC = -4  /* plan table allocation failed                     */
+};
+
+enum { NFGS_F32 = 0, NFGS_F64 = 1 };
+enum { NFGS_DCT1 = 0, NFGS_DST1 = 1 };
+enum { NFGS_FORWARD = 0, NFGS_INVERSE = 1 };
+
+/* Metric slots written by nfgs_reduce() (doubles). */
+enum {
+  NFGS_Q_SIGMA_NUM = 0,  /* sum |J_f x B_f| / max(|B_f|,1e-10) over the face window */
+  NFGS_Q_SIGMA_DEN = 1,  /* sum |J_f| over the face window                          */
+  NFGS_Q_DIV_ABS = 2,    /* sum |div B| over all nodes                              */
+  NFGS_Q_BMAG = 3,       /* sum |B| over all nodes                                  */
+  NFGS_Q_B2 = 4,         /* sum |B|^2 over all nodes                                */
+  NFGS_Q_SRC2 = 5,       /* sum S_Bcz^2 (source of the Phi_c solve)                 */
+  NFGS_Q_ZERO_MODE = 6,  /* sum over heights of (DCT2(S_Bcz)[z,0,0] / norm)^2        */
+  NFGS_Q_FACE_J2 = 7,    /* sum of squared face currents (final J pass only)        */
+  NFGS_Q_CELL_J2 = 8,    /* sum of squared centre currents (final J pass only)      */
+  NFGS_NQ = 9
+};
+
+typedef struct nfgs_plan nfgs_plan;
+
+/* Grid and constants of one solve; mirrors NFGSSolverGPU.__init__
+ * (nfgs_solver.py:3925-4349, :736-937) after calculation-grid padding. */
+typedef struct {
+  int nx, ny, nz;                 /* calculation grid (each >= 3)                     */
+  double dx, dy, dz;
+  int dtype;                      /* NFGS_F32 or NFGS_F64                             */
+  double rho;                     /* bottom_divergence_robin_weight in [0,1]          */
+  double alpha_max;               /* alpha clamp, alpha_limit_factor/min(dx,dy,dz)    */
+  double weak_field_floor_b2;     /* floor of |B_f|^2 in the alpha projection         */
+  int win_z0, win_z1;             /* sigma window on vertical faces, half-open        */
+  int win_y0, win_y1;             /* (nfgs_solver.py:2656-2676 on the face grid)      */
+  int win_x0, win_x1;
+} nfgs_grid_desc;
+
+/* Fixed per-boundary device arrays produced by the setup stage
+ * (set_bottom_vector_boundary, nfgs_solver.py:4548-4919). */
+typedef struct {
+  const void* b_bg;           /* 3*N  B_background = B_below + B_open              */
+  const void* bz_obs;         /* ny*nx observed Bz on the calculation grid          */
+  const void* jz_t;           /* ny*nx toroidal bottom current -lap_xy(Az_bottom)    */
+  const void* phi_bottom;     /* ny*nx Phi_above_bottom                              */
+  const void* az_bottom;      /* ny*nx Az_bottom                                     */
+  const void* phi_bottom_hat; /* ny*nx DCT2(Phi_above_bottom), (0,0) zeroed          */
+  const void* q_hat;          /* ny*nx DCT2(bottom dPhi_c/dz target), (0,0) zeroed   */
+  const void* az_bottom_hat;  /* (ny-2)*(nx-2) DST2 of the interior of Az_bottom     */
+  const void* h_profile;      /* 2*nz frozen linear harmonic H_k0                    */
+} nfgs_boundary;
+
+/* Caller-owned work buffers of one solve. */
+typedef struct {
+  void* jp;         /* 3*N  parallel current J_par of the current state          */
+  void* s_bcz;      /* N    source of the Bc_z / Phi_c solve                      */
+  void* s_jz;       /* nz*(ny-2)*(nx-2) interior source of the Az solve           */
+  void* hat0;       /* N    transform work                                        */
+  void* hat1;       /* N    transform work                                        */
+  void* cscr;       /* N    Thomas c' scratch                                      */
+  double* partials; /* nfgs_partials_count(plan) doubles                          */
+  double* metrics;  /* NFGS_NQ doubles                                            */
+} nfgs_workspace;
+
+const char* nfgs_last_error(void);
+
+/* Plan: transform matrices (paired DCT-I/DST-I halves), lambda/k tables.
+ * Replaces the constructor's table caches (nfgs_solver.py:1083-1118,
+ * :4357-4383). */
+int nfgs_plan_create(const nfgs_grid_desc* desc, nfgs_plan** out);
+int nfgs_plan_destroy(nfgs_plan* plan);
+/* Number of doubles the partials buffer must hold. */
+long nfgs_partials_count(const nfgs_plan* plan);
+/* Device pointer of a plan table in the plan dtype: 0 endpoint-Neumann
+ * lambda (ny*nx), 1 k = sqrt(-lambda) (ny*nx), 2 endpoint-Dirichlet lambda
+ * ((ny-2)*(nx-2)), 3 alpha_x (nx), 4 alpha_y (ny). */
+const void* nfgs_plan_table(const nfgs_plan* plan, int which);
+
+/* Unnormalised type-I transform (forward) or its inverse along one axis of a
+ * batch of vectors: vector g = p*nvec + q starts at p*plane_stride +
+ * q*vec_stride and element j sits at j*elem_stride.  Either both element
+ * strides or both vector strides must be 1.  n must be nx or ny for DCT-I and
+ * nx-2 or ny-2 for DST-I.  Replaces _dct1/_idct1/_dst1/_idst1
+ * (nfgs_solver.py:1019-1081). */
+int nfgs_transform(const nfgs_plan* plan, int kind, int direction, const void* in,
+                   long in_elem_stride, long in_vec_stride, long in_plane_stride, void* out,
+                   long out_elem_stride, long out_vec_stride, long out_plane_stride, int n,
+                   int nvec, int nplanes, void* stream);
+
+/* Modal Robin z-solve of (d_zz + lambda) Phi = S/lambda per endpoint-Neumann
+ * mode, rho-mixed first row and eliminated top Robin row.  src_hat is the
+ * raw DCT2 of the source (nz*ny*nx); out_hat receives the modal solution
+ * (plane 0 = phi_bottom_hat).  Optionally accumulates the zero-mode sum into
+ * partials.  Replaces _solve_phi_c_robin/_solve_modal_robin_from_hat
+ * (nfgs_solver.py:5627-5672, :5512-5625). */
+int nfgs_robin_solve(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* src_hat,
+                     void* out_hat, void* scratch, double* partials, void* stream);
+
+/* Dirichlet z-solve (d_zz + lambda_D) Az = -S/lambda_D per interior DST-I
+ * mode, Az(0) = az_bottom_hat, Az(top) = 0, on compact (nz,ny-2,nx-2)
+ * arrays.  Replaces _solve_az_from_jz_poisson_source's modal part and
+ * _thomas_solve_batched_lowmem (nfgs_solver.py:3263-3306, :535-608). */
+int nfgs_dirichlet_solve(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* src_hat,
+                         void* out_hat, void* scratch, void* stream);
+
+/* B = B_bg + (-Dx Dz Phi + Dy Az, -Dy Dz Phi - Dx Az, lap_xy Phi) + H, with
+ * Bz(0) = Bz_obs.  Replaces the B half of _reconstruct_state_from_scalars
+ * (nfgs_solver.py:5901-5942, :3311-3341, :1711-1715). */
+int nfgs_reconstruct_b(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* phi,
+                       const void* az, void* b_out, void* stream);
+
+/* Vertical-face pass over B: face current of the carrier B - B_bg, alpha
+ * projection, faces-to-centres parallel current J_par with the bottom row
+ * taken from the state current (jp_out, may be NULL); the state current J
+ * (j_out, may be NULL); and block partials of sigma, divB and energy
+ * (partials, may be NULL).  Jz is -lap_xy(Az) with its boundary values.
+ * Replaces compute_parallel_currents (:5811-5839), the staggered J of
+ * reconstruction (:5917-5941), compute_sigma (:5882-5895), compute_divB
+ * and compute_energy (:2689-2697). */
+int nfgs_face_pass(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* b,
+                   const void* az, void* jp_out, void* j_out, double* partials, void* stream);
+
+/* Sources S_Bcz = Dy Jpx - Dx Jpy (full grid) and S_Jz = lap_xy Jpz -
+ * D2xz Jpx - D2yz Jpy (interior, compact), plus the S_Bcz^2 partial.
+ * Replaces nfgs_solver.py:6726-6745. */
+int nfgs_sources(const nfgs_plan* plan, const void* jp, void* s_bcz, void* s_jz,
+                 double* partials, void* stream);
+
+/* Deterministic final reduction of the partials into NFGS_NQ metrics. */
+int nfgs_reduce(const nfgs_plan* plan, const double* partials, double* metrics, void* stream);
+
+/* One full plain iteration at mu = 1 from ws->jp (the parallel current of the
+ * current state): sources, Phi_c and Az solves into phi_out/az_out (whose
+ * fixed planes must already hold Phi_above_bottom, Az_bottom and zeros),
+ * reconstruction of B into b_out, and the face pass that leaves the next
+ * J_par in ws->jp and the metrics of the new state in ws->metrics.
+ * Replaces NFGSSolverGPU.iteration_step (nfgs_solver.py:6647-6935) plus the
+ * per-iteration monitor calls of the driver (:3694-3702).  With
+ * zero_source != 0 the sources are zero (the gs_cfcf start, :6613-6645). */
+int nfgs_advance(const nfgs_plan* plan, const nfgs_boundary* bnd, const nfgs_workspace* ws,
+                 void* phi_out, void* az_out, void* b_out, int zero_source, void* stream);
+
+/* --- boundary setup (set_bottom_vector_boundary, once per boundary) --- */
+
+/* Fortran readin+botbal bottom Jz (nfgs_solver.py:1498-1567 legacy helper
+ * grid when helper dims exceed the calculation grid; :4519-4546 otherwise).
+ * bx, by: calculation-grid planes.  The helper grid is (hny, hnx) with the
+ * calculation grid placed at (bfy, bfx).  legacy selects the helper-grid
+ * edge zeroing.  ijzbcor in {0..4}. */
+int nfgs_setup_jz(const nfgs_plan* plan, const void* bx, const void* by, void* jz, int hny,
+                  int hnx, int bfy, int bfx, int ijzbcor, int legacy, void* stream);
+
+/* Endpoint-Dirichlet 2-D Poisson of scale*src on one plane: interior solve,
+ * zero edges (nfgs_solver.py:1132-1144).  work: 2*(ny-2)*(nx-2) elements. */
+int nfgs_setup_poisson2d_dirichlet(const nfgs_plan* plan, const void* src, double scale,
+                                   void* out, void* work, void* stream);
+
+/* out = scale * lap_xy(in) on one plane, edges zeroed when zero_edges != 0
+ * (nfgs_solver.py:3039-3046). */
+int nfgs_setup_plane_lap(const nfgs_plan* plan, const void* in, double scale, int zero_edges,
+                         void* out, void* stream);
+
+/* Workspace bytes needed by nfgs_setup_gaussian. */
+long nfgs_gaussian_workspace_bytes(const nfgs_plan* plan);
+
+/* Gaussian separation with the spectral horizontal operator, uniform-open
+ * mean Bz, frozen linear harmonic closure (nfgs_solver.py:4671-4919,
+ * :4990-5056, :5069-5096).  Writes B_bg (3*N), bz_above, phi_bottom
+ * (Phi_above_bottom), dphi_target, d0, phi_above_hat (planes), h_target
+ * (2 values, the constant bottom harmonic residual) and diag
+ * (NFGS_GDIAG_COUNT device doubles: DCT constant modes, sums and sums of
+ * squares from which the reference's diagnostics follow). */
+enum {
+  NFGS_GD_BX_MEAN = 0,   /* DCT2(Bx)[0,0] / (4(nx-1)(ny-1))                      */
+  NFGS_GD_BY_MEAN,       /* same for By                                          */
+  NFGS_GD_BZ_MEAN,       /* same for Bz = Bz_open_mean                            */
+  NFGS_GD_D0_MEAN,       /* constant mode of the spectral divergence             */
+  NFGS_GD_D0_SQ,         /* sum D0^2                                             */
+  NFGS_GD_RECON_SQ,      /* sum (B_bg,z(0) + Bz_above - Bz_obs)^2                 */
+  NFGS_GD_RECON_MAX,     /* max |B_bg,z(0) + Bz_above - Bz_obs|                   */
+  NFGS_GD_BG_INV_SQ,     /* sum (lap_xy Phi_below + open - B_bg,z)^2              */
+  NFGS_GD_HRES_SQ,       /* sum of the squared bottom horizontal residual         */
+  NFGS_GD_HOBS_SQ,       /* sum Bx_obs^2 + By_obs^2                               */
+  NFGS_GD_HCONST_X,      /* constant residual mode, x (= H_x)                     */
+  NFGS_GD_HCONST_Y,      /* constant residual mode, y (= H_y)                     */
+  NFGS_GD_HNONCONST_SQ,  /* sum of squared non-constant residual                  */
+  NFGS_GD_HDIV_SQ,       /* sum of squared spectral divergence of the residual    */
+  NFGS_GD_HCURL_SQ,      /* sum of squared FD curl of the residual                */
+  NFGS_GD_BZ_SUM,        /* sum Bz_obs                                            */
+  NFGS_GD_BZ_ABS_SUM,    /* sum |Bz_obs|                                          */
+  NFGS_GD_BX_SUM,        /* sum Bx_obs                                            */
+  NFGS_GD_BY_SUM,        /* sum By_obs                                            */
+  NFGS_GDIAG_COUNT
+};
+int nfgs_setup_gaussian(const nfgs_plan* plan, const void* bx, const void* by, const void* bz,
+                        const void* az_bottom, void* b_bg, void* bz_above, void* phi_bottom,
+                        void* dphi_target, void* d0, void* phi_above_hat, void* h_target,
+                        double* diag, void* workspace, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NFGS_HIP_H */

@@ -1,0 +1,123 @@
+"""ctypes binding of libnfgs_hip.so (declarations in include/nfgs_hip.h).
+
+The binding never falls back to anything else: if the library is missing or
+fails to load, every solver call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libnfgs_hip.so"
+
+NFGS_OK, NFGS_ERR_ARG, NFGS_ERR_HIP, NFGS_ERR_STATE, NFGS_ERR_ALLOC = 0, -1, -2, -3, -4
+NFGS_F32, NFGS_F64 = 0, 1
+NFGS_DCT1, NFGS_DST1 = 0, 1
+NFGS_FORWARD, NFGS_INVERSE = 0, 1
+(Q_SIGMA_NUM, Q_SIGMA_DEN, Q_DIV_ABS, Q_BMAG, Q_B2, Q_SRC2, Q_ZERO_MODE, Q_FACE_J2,
+ Q_CELL_J2) = range(9)
+NQ = 9
+GD = dict(BX_MEAN=0, BY_MEAN=1, BZ_MEAN=2, D0_MEAN=3, D0_SQ=4, RECON_SQ=5, RECON_MAX=6,
+          BG_INV_SQ=7, HRES_SQ=8, HOBS_SQ=9, HCONST_X=10, HCONST_Y=11, HNONCONST_SQ=12,
+          HDIV_SQ=13, HCURL_SQ=14, BZ_SUM=15, BZ_ABS_SUM=16, BX_SUM=17, BY_SUM=18)
+GDIAG_COUNT = 19
+SD_COUNT = 10
+
+vp = C.c_void_p
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("dtype", C.c_int), ("rho", C.c_double), ("alpha_max", C.c_double),
+                ("weak_field_floor_b2", C.c_double),
+                ("win_z0", C.c_int), ("win_z1", C.c_int), ("win_y0", C.c_int),
+                ("win_y1", C.c_int), ("win_x0", C.c_int), ("win_x1", C.c_int)]
+
+
+class Boundary(C.Structure):
+    _fields_ = [(n, vp) for n in ("b_bg", "bz_obs", "jz_t", "phi_bottom", "az_bottom",
+                                  "phi_bottom_hat", "q_hat", "az_bottom_hat", "h_profile")]
+
+
+class Workspace(C.Structure):
+    _fields_ = [("jp", vp), ("s_bcz", vp), ("s_jz", vp), ("hat0", vp), ("hat1", vp),
+                ("cscr", vp), ("partials", vp), ("metrics", vp)]
+
+
+_lib = None
+_load_error = None
+
+
+def _declare(lib):
+    P = C.POINTER
+    sigs = {
+        "nfgs_last_error": (C.c_char_p, []),
+        "nfgs_plan_create": (C.c_int, [P(GridDesc), P(vp)]),
+        "nfgs_plan_destroy": (C.c_int, [vp]),
+        "nfgs_partials_count": (C.c_long, [vp]),
+        "nfgs_plan_table": (vp, [vp, C.c_int]),
+        "nfgs_transform": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_long, C.c_long, C.c_long, vp,
+                                     C.c_long, C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, vp]),
+        "nfgs_robin_solve": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp]),
+        "nfgs_dirichlet_solve": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp]),
+        "nfgs_reconstruct_b": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp]),
+        "nfgs_face_pass": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp]),
+        "nfgs_sources": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+        "nfgs_reduce": (C.c_int, [vp, vp, vp, vp]),
+        "nfgs_advance": (C.c_int, [vp, P(Boundary), P(Workspace), vp, vp, vp, C.c_int, vp]),
+        "nfgs_setup_jz": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, vp]),
+        "nfgs_setup_poisson2d_dirichlet": (C.c_int, [vp, vp, C.c_double, vp, vp, vp]),
+        "nfgs_setup_plane_lap": (C.c_int, [vp, vp, C.c_double, C.c_int, vp, vp]),
+        "nfgs_gaussian_workspace_bytes": (C.c_long, [vp]),
+        "nfgs_setup_gaussian": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                          vp]),
+        "nfgs_sq_diff": (C.c_int, [vp, vp, vp, C.c_long, vp, vp]),
+        "nfgs_source_diagnostics_workspace_bytes": (C.c_long, [vp]),
+        "nfgs_source_diagnostics": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def load():
+    """Load the HIP library once; raise (never fall back) when unavailable."""
+    global _lib, _load_error
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing; build it with `python -m nfgs_amd.build`")
+    try:
+        _lib = _declare(C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | getattr(os, "RTLD_GLOBAL", 0)))
+    except OSError as exc:  # pragma: no cover - depends on the machine
+        _load_error = exc
+        raise RuntimeError(f"cannot load {LIB_PATH}: {exc}") from exc
+    return _lib
+
+
+class NFGSError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str = ""):
+    """Map a C status code to the reference's exception classes."""
+    if rc == NFGS_OK:
+        return
+    msg = load().nfgs_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if rc == NFGS_ERR_ARG:
+        raise ValueError(text)
+    if rc == NFGS_ERR_STATE:
+        raise RuntimeError(text)
+    raise NFGSError(f"{text} (status {rc})")
+
+
+def ptr(t) -> int:
+    """Device address of a torch tensor (None -> NULL)."""
+    return None if t is None else t.data_ptr()

@@ -1,0 +1,106 @@
+// Shared definitions for the NFGS gfx950 kernels and their C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/nfgs_hip.h"
+
+namespace nfgs {
+
+// Thread-local last-error text returned by nfgs_last_error().
+void set_error(const std::string& msg);
+
+#define NFGS_HIP_CHECK(expr)                                                              \
+  do {                                                                                    \
+    hipError_t _e = (expr);                                                               \
+    if (_e != hipSuccess) {                                                               \
+      ::nfgs::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));              \
+      return NFGS_ERR_HIP;                                                                \
+    }                                                                                     \
+  } while (0)
+
+#define NFGS_REQUIRE(cond, code, msg)                                                     \
+  do {                                                                                    \
+    if (!(cond)) {                                                                        \
+      ::nfgs::set_error(msg);                                                             \
+      return code;                                                                        \
+    }                                                                                     \
+  } while (0)
+
+// Launch-then-check helper: kernel launch errors surface as NFGS_ERR_HIP.
+#define NFGS_LAUNCH_CHECK()                                                               \
+  do {                                                                                    \
+    hipError_t _e = hipGetLastError();                                                    \
+    if (_e != hipSuccess) {                                                               \
+      ::nfgs::set_error(std::string("kernel launch: ") + hipGetErrorString(_e));         \
+      return NFGS_ERR_HIP;                                                                \
+    }                                                                                     \
+  } while (0)
+
+// Second-order centred first derivative with one-sided ends (needs n >= 3),
+// evaluated with the operand order of the reference stencil.
+template <typename T>
+__device__ __forceinline__ T fd1(const T* f, long s, int i, int n, T h) {
+  if (i > 0 && i < n - 1) return (f[s] - f[-s]) / (T(2) * h);
+  if (i == 0) return (T(-3) * f[0] + T(4) * f[s] - f[2 * s]) / (T(2) * h);
+  return (T(3) * f[0] - T(4) * f[-s] + f[-2 * s]) / (T(2) * h);
+}
+
+// Same derivative on a value accessor: g(o) returns the sample at offset o.
+template <typename T, typename G>
+__device__ __forceinline__ T fd1g(G g, int i, int n, T h) {
+  if (i > 0 && i < n - 1) return (g(1) - g(-1)) / (T(2) * h);
+  if (i == 0) return (T(-3) * g(0) + T(4) * g(1) - g(2)) / (T(2) * h);
+  return (T(3) * g(0) - T(4) * g(-1) + g(-2)) / (T(2) * h);
+}
+
+// Mirror-Neumann 5-point lateral Laplacian at (y, x) of a (ny, nx) plane with
+// row stride ld, accumulated in the reference's term order.
+template <typename T>
+__device__ __forceinline__ T lap_xy_at(const T* p, int y, int x, int ny, int nx, long ld,
+                                       T idx2, T idy2) {
+  const T* c = p + (long)y * ld + x;
+  T acc = T(0);
+  if (x > 0 && x < nx - 1) acc += (c[1] - T(2) * c[0] + c[-1]) * idx2;
+  if (y > 0 && y < ny - 1) acc += (c[ld] - T(2) * c[0] + c[-ld]) * idy2;
+  if (y == 0) acc += (T(2) * c[ld] - T(2) * c[0]) * idy2;
+  if (y == ny - 1) acc += (T(2) * c[-ld] - T(2) * c[0]) * idy2;
+  if (x == 0) acc += (T(2) * c[1] - T(2) * c[0]) * idx2;
+  if (x == nx - 1) acc += (T(2) * c[-1] - T(2) * c[0]) * idx2;
+  return acc;
+}
+
+// Block-wide sum of NQ doubles; result valid in thread 0.  blockDim.x must be
+// a multiple of 64 and at most 1024.
+template <int NQ>
+__device__ __forceinline__ void block_sum(double (&v)[NQ], double* lds /* [NQ*16] */) {
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double x = v[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    v[q] = x;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) lds[q * 16 + wid] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < nw; ++w) s += lds[q * 16 + w];
+      v[q] = s;
+    }
+  }
+}
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace nfgs

@@ -1,0 +1,42 @@
+// Internal plan layout shared by the NFGS kernels' launchers.
+#pragma once
+
+#include "nfgs_common.h"
+
+constexpr int kRedScratch = 4096;
+
+// Half-size operands of a type-I transform matrix T (n x n).  T has the
+// pairing symmetry T[n-1-k][j] = (-1)^j T[k][j], so with E[k][j'] = T[k][2j']
+// and O[k][j'] = T[k][2j'+1] the outputs are out[k] = (E x_even + O x_odd)[k]
+// and out[n-1-k] = (E x_even - O x_odd)[k] for k < h = ceil(n/2).  E and O are
+// stored transposed (row j', column k) and zero-padded to rows_pad x cols_pad.
+struct PairedMatrix {
+  int n = 0, h = 0, rows = 0, rows_pad = 0, cols_pad = 0;
+  void* et[2] = {nullptr, nullptr};  // [direction] E^T
+  void* ot[2] = {nullptr, nullptr};  // [direction] O^T
+};
+
+struct nfgs_plan {
+  nfgs_grid_desc d;
+  long N = 0;        // nz*ny*nx
+  long plane = 0;    // ny*nx
+  long iplane = 0;   // (ny-2)*(nx-2)
+  int esize = 4;     // element bytes
+  PairedMatrix dct_x, dct_y, dst_x, dst_y;
+  void* lam = nullptr;   // ny*nx endpoint-Neumann eigenvalues
+  void* kk = nullptr;    // ny*nx sqrt(-lambda)
+  void* lamd = nullptr;  // (ny-2)*(nx-2) endpoint-Dirichlet eigenvalues
+  void* ax = nullptr;    // nx   2 sin(pi k / (2(nx-1))) / dx
+  void* ay = nullptr;    // ny   2 sin(pi k / (2(ny-1))) / dy
+  int pblocks = 0;       // blocks of the reducing kernels (partials per slot)
+  double* red = nullptr; // kRedScratch doubles of reduction scratch (not re-entrant)
+};
+
+namespace nfgs {
+const PairedMatrix* find_matrix(const nfgs_plan* p, int kind, int n);
+int fill_zero(const nfgs_plan* p, void* ptr, long n, hipStream_t s);
+int fill_zero_f64(double* ptr, long n, hipStream_t s);
+int launch_transform(const nfgs_plan* p, const PairedMatrix* m, int direction, const void* in,
+                     long ies, long ivs, long ips, void* out, long oes, long ovs, long ops,
+                     int nvec, int nplanes, hipStream_t s);
+}  // namespace nfgs

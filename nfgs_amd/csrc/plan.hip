@@ -1,0 +1,211 @@
+// Plan lifecycle: grid constants, paired transform matrices, modal tables.
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "plan.h"
+
+namespace nfgs {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
+// Exact-argument type-I matrices: the integer phase is reduced before the
+// trigonometric call so every entry is correctly rounded in double.
+double type1_entry(int kind, int n, int k, int j) {
+  const double pi = 3.14159265358979323846;
+  if (kind == NFGS_DCT1) {
+    const long period = 2L * (n - 1);
+    const long ph = ((long)j * k) % period;
+    const double w = (j == 0 || j == n - 1) ? 1.0 : 2.0;
+    return w * std::cos(pi * (double)ph / (double)(n - 1));
+  }
+  const long period = 2L * (n + 1);
+  const long ph = ((long)(j + 1) * (k + 1)) % period;
+  return 2.0 * std::sin(pi * (double)ph / (double)(n + 1));
+}
+
+template <typename T>
+int upload(const std::vector<double>& host, void** dev) {
+  std::vector<T> cast(host.size());
+  for (size_t i = 0; i < host.size(); ++i) cast[i] = static_cast<T>(host[i]);
+  NFGS_HIP_CHECK(hipMalloc(dev, cast.size() * sizeof(T)));
+  NFGS_HIP_CHECK(hipMemcpy(*dev, cast.data(), cast.size() * sizeof(T), hipMemcpyHostToDevice));
+  return NFGS_OK;
+}
+
+int upload_dtype(int dtype, const std::vector<double>& host, void** dev) {
+  return dtype == NFGS_F32 ? upload<float>(host, dev) : upload<double>(host, dev);
+}
+
+int build_paired(int dtype, int kind, int n, PairedMatrix* m) {
+  m->n = n;
+  m->h = (n + 1) / 2;
+  m->rows = (n + 1) / 2;
+  m->rows_pad = ((m->rows + 15) / 16) * 16;
+  m->cols_pad = ((m->h + 15) / 16) * 16;
+  const double inv_scale = kind == NFGS_DCT1 ? 1.0 / (2.0 * (n - 1)) : 1.0 / (2.0 * (n + 1));
+  for (int dir = 0; dir < 2; ++dir) {
+    const double sc = dir == NFGS_FORWARD ? 1.0 : inv_scale;
+    std::vector<double> et((size_t)m->rows_pad * m->cols_pad, 0.0);
+    std::vector<double> ot((size_t)m->rows_pad * m->cols_pad, 0.0);
+    for (int jp = 0; jp < m->rows; ++jp) {
+      for (int k = 0; k < m->h; ++k) {
+        const int je = 2 * jp, jo = 2 * jp + 1;
+        if (je < n) et[(size_t)jp * m->cols_pad + k] = sc * type1_entry(kind, n, k, je);
+        if (jo < n) ot[(size_t)jp * m->cols_pad + k] = sc * type1_entry(kind, n, k, jo);
+      }
+    }
+    int rc = upload_dtype(dtype, et, &m->et[dir]);
+    if (rc) return rc;
+    rc = upload_dtype(dtype, ot, &m->ot[dir]);
+    if (rc) return rc;
+  }
+  return NFGS_OK;
+}
+
+void free_paired(PairedMatrix* m) {
+  for (int d = 0; d < 2; ++d) {
+    if (m->et[d]) (void)hipFree(m->et[d]);
+    if (m->ot[d]) (void)hipFree(m->ot[d]);
+    m->et[d] = m->ot[d] = nullptr;
+  }
+}
+
+}  // namespace
+
+const PairedMatrix* find_matrix(const nfgs_plan* p, int kind, int n) {
+  if (kind == NFGS_DCT1) {
+    if (n == p->dct_x.n) return &p->dct_x;
+    if (n == p->dct_y.n) return &p->dct_y;
+  } else if (kind == NFGS_DST1) {
+    if (n == p->dst_x.n) return &p->dst_x;
+    if (n == p->dst_y.n) return &p->dst_y;
+  }
+  return nullptr;
+}
+
+}  // namespace nfgs
+
+extern "C" const char* nfgs_last_error(void) { return nfgs::g_last_error.c_str(); }
+
+extern "C" int nfgs_plan_destroy(nfgs_plan* p) {
+  if (!p) return NFGS_OK;
+  nfgs::free_paired(&p->dct_x);
+  nfgs::free_paired(&p->dct_y);
+  nfgs::free_paired(&p->dst_x);
+  nfgs::free_paired(&p->dst_y);
+  if (p->lam) (void)hipFree(p->lam);
+  if (p->kk) (void)hipFree(p->kk);
+  if (p->lamd) (void)hipFree(p->lamd);
+  if (p->ax) (void)hipFree(p->ax);
+  if (p->ay) (void)hipFree(p->ay);
+  if (p->red) (void)hipFree(p->red);
+  delete p;
+  return NFGS_OK;
+}
+
+extern "C" int nfgs_plan_create(const nfgs_grid_desc* d, nfgs_plan** out) {
+  using namespace nfgs;
+  NFGS_REQUIRE(d != nullptr && out != nullptr, NFGS_ERR_ARG, "plan_create: null argument");
+  *out = nullptr;
+  NFGS_REQUIRE(d->nx >= 3 && d->ny >= 3 && d->nz >= 3, NFGS_ERR_ARG,
+               "plan_create: every calculation-grid dimension must be >= 3");
+  NFGS_REQUIRE(d->dtype == NFGS_F32 || d->dtype == NFGS_F64, NFGS_ERR_ARG,
+               "plan_create: dtype must be NFGS_F32 or NFGS_F64");
+  NFGS_REQUIRE(d->dx > 0 && d->dy > 0 && d->dz > 0, NFGS_ERR_ARG,
+               "plan_create: grid spacings must be positive");
+  NFGS_REQUIRE(d->rho >= 0.0 && d->rho <= 1.0, NFGS_ERR_ARG,
+               "bottom_divergence_robin_weight must be between 0 and 1");
+  NFGS_REQUIRE(d->win_z0 >= 0 && d->win_z1 <= d->nz - 1 && d->win_y0 >= 0 &&
+                   d->win_y1 <= d->ny && d->win_x0 >= 0 && d->win_x1 <= d->nx,
+               NFGS_ERR_ARG, "plan_create: sigma window outside the face grid");
+  NFGS_REQUIRE((long)d->nx * d->ny * d->nz < (1L << 40), NFGS_ERR_ARG, "plan_create: grid too large");
+
+  nfgs_plan* p = new nfgs_plan();
+  p->d = *d;
+  p->plane = (long)d->ny * d->nx;
+  p->iplane = (long)(d->ny - 2) * (d->nx - 2);
+  p->N = p->plane * d->nz;
+  p->esize = d->dtype == NFGS_F32 ? 4 : 8;
+  p->pblocks = 1024;
+
+  int rc = build_paired(d->dtype, NFGS_DCT1, d->nx, &p->dct_x);
+  if (!rc) rc = build_paired(d->dtype, NFGS_DCT1, d->ny, &p->dct_y);
+  if (!rc) rc = build_paired(d->dtype, NFGS_DST1, d->nx - 2, &p->dst_x);
+  if (!rc) rc = build_paired(d->dtype, NFGS_DST1, d->ny - 2, &p->dst_y);
+
+  if (!rc) {
+    // Endpoint-Neumann eigenvalues (nfgs_solver.py:1098-1109) and k = |lambda|^(1/2).
+    const double pi = 3.14159265358979323846;
+    std::vector<double> lam((size_t)p->plane), kk((size_t)p->plane);
+    std::vector<double> lx(d->nx), ly(d->ny);
+    for (int i = 0; i < d->nx; ++i) {
+      const double s = std::sin(pi * i / (2.0 * (d->nx - 1)));
+      lx[i] = -4.0 * s * s / (d->dx * d->dx);
+    }
+    for (int j = 0; j < d->ny; ++j) {
+      const double s = std::sin(pi * j / (2.0 * (d->ny - 1)));
+      ly[j] = -4.0 * s * s / (d->dy * d->dy);
+    }
+    for (int j = 0; j < d->ny; ++j)
+      for (int i = 0; i < d->nx; ++i) {
+        const double v = lx[i] + ly[j];
+        lam[(size_t)j * d->nx + i] = v;
+        kk[(size_t)j * d->nx + i] = std::sqrt(v < 0 ? -v : 0.0);
+      }
+    // Endpoint-Dirichlet interior eigenvalues (nfgs_solver.py:1083-1096).
+    std::vector<double> lamd((size_t)p->iplane);
+    const int nxi = d->nx - 2, nyi = d->ny - 2;
+    std::vector<double> dx_(nxi), dy_(nyi);
+    for (int i = 0; i < nxi; ++i) {
+      const double s = std::sin(pi * (i + 1) / (2.0 * (nxi + 1)));
+      dx_[i] = -4.0 * s * s / (d->dx * d->dx);
+    }
+    for (int j = 0; j < nyi; ++j) {
+      const double s = std::sin(pi * (j + 1) / (2.0 * (nyi + 1)));
+      dy_[j] = -4.0 * s * s / (d->dy * d->dy);
+    }
+    for (int j = 0; j < nyi; ++j)
+      for (int i = 0; i < nxi; ++i) lamd[(size_t)j * nxi + i] = dx_[i] + dy_[j];
+    // |d/dx| factors of the spectral horizontal operator (nfgs_solver.py:4357-4383).
+    std::vector<double> ax(d->nx), ay(d->ny);
+    for (int i = 0; i < d->nx; ++i) ax[i] = 2.0 * std::sin(pi * i / (2.0 * (d->nx - 1))) / d->dx;
+    for (int j = 0; j < d->ny; ++j) ay[j] = 2.0 * std::sin(pi * j / (2.0 * (d->ny - 1))) / d->dy;
+    rc = upload_dtype(d->dtype, lam, &p->lam);
+    if (!rc) rc = upload_dtype(d->dtype, ax, &p->ax);
+    if (!rc) rc = upload_dtype(d->dtype, ay, &p->ay);
+    if (!rc) rc = upload_dtype(d->dtype, kk, &p->kk);
+    if (!rc) rc = upload_dtype(d->dtype, lamd, &p->lamd);
+    if (!rc && hipMalloc(reinterpret_cast<void**>(&p->red), kRedScratch * sizeof(double)) !=
+                   hipSuccess) {
+      set_error("plan_create: hipMalloc of the reduction scratch failed");
+      rc = NFGS_ERR_HIP;
+    }
+  }
+  if (rc) {
+    nfgs_plan_destroy(p);
+    return rc == NFGS_ERR_HIP ? NFGS_ERR_ALLOC : rc;
+  }
+  *out = p;
+  return NFGS_OK;
+}
+
+extern "C" long nfgs_partials_count(const nfgs_plan* p) {
+  return p ? (long)p->pblocks * NFGS_NQ : 0;
+}
+
+extern "C" const void* nfgs_plan_table(const nfgs_plan* p, int which) {
+  if (!p) return nullptr;
+  switch (which) {
+    case 0: return p->lam;
+    case 1: return p->kk;
+    case 2: return p->lamd;
+    case 3: return p->ax;
+    case 4: return p->ay;
+    default: return nullptr;
+  }
+}

@@ -1,0 +1,72 @@
+// One plain NFGS iteration at mu = 1, enqueued as a fixed kernel sequence.
+
+#include "plan.h"
+
+namespace {
+
+inline void* at(const nfgs_plan* p, void* base, long elems) {
+  return static_cast<char*>(base) + elems * p->esize;
+}
+inline const void* at(const nfgs_plan* p, const void* base, long elems) {
+  return static_cast<const char*>(base) + elems * p->esize;
+}
+
+#define NFGS_TRY(expr)      \
+  do {                      \
+    int _rc = (expr);       \
+    if (_rc) return _rc;    \
+  } while (0)
+
+}  // namespace
+
+extern "C" int nfgs_advance(const nfgs_plan* p, const nfgs_boundary* bnd,
+                            const nfgs_workspace* ws, void* phi_out, void* az_out, void* b_out,
+                            int zero_source, void* stream) {
+  using namespace nfgs;
+  NFGS_REQUIRE(p && bnd && ws && phi_out && az_out && b_out, NFGS_ERR_ARG,
+               "advance: null argument");
+  NFGS_REQUIRE(ws->jp && ws->s_bcz && ws->s_jz && ws->hat0 && ws->hat1 && ws->cscr &&
+                   ws->partials && ws->metrics,
+               NFGS_ERR_ARG, "advance: workspace buffer missing");
+  hipStream_t s = as_stream(stream);
+  const int nx = p->d.nx, ny = p->d.ny, nz = p->d.nz;
+  const int nxi = nx - 2, nyi = ny - 2;
+  const long pl = p->plane, ip = p->iplane;
+
+  // Sources of both Poisson problems from the parallel current.
+  if (zero_source) {
+    NFGS_TRY(fill_zero(p, ws->s_bcz, p->N, s));
+    NFGS_TRY(fill_zero(p, ws->s_jz, (long)nz * ip, s));
+    NFGS_TRY(fill_zero_f64(ws->partials + (long)NFGS_Q_SRC2 * p->pblocks, p->pblocks, s));
+  } else {
+    NFGS_TRY(nfgs_sources(p, ws->jp, ws->s_bcz, ws->s_jz, ws->partials, stream));
+  }
+
+  // Phi_c: DCT-I x, DCT-I y, modal Robin sweep, inverse y... on planes 1..nz-1.
+  NFGS_TRY(launch_transform(p, &p->dct_x, NFGS_FORWARD, ws->s_bcz, 1, nx, pl, ws->hat0, 1, nx, pl,
+                            ny, nz, s));
+  NFGS_TRY(launch_transform(p, &p->dct_y, NFGS_FORWARD, ws->hat0, nx, 1, pl, ws->hat1, nx, 1, pl,
+                            nx, nz, s));
+  NFGS_TRY(nfgs_robin_solve(p, bnd, ws->hat1, ws->hat0, ws->cscr, ws->partials, stream));
+  NFGS_TRY(launch_transform(p, &p->dct_x, NFGS_INVERSE, at(p, ws->hat0, pl), 1, nx, pl,
+                            at(p, ws->hat1, pl), 1, nx, pl, ny, nz - 1, s));
+  NFGS_TRY(launch_transform(p, &p->dct_y, NFGS_INVERSE, at(p, ws->hat1, pl), nx, 1, pl,
+                            at(p, phi_out, pl), nx, 1, pl, nx, nz - 1, s));
+
+  // Az: DST-I on the interior of planes 1..nz-2, Dirichlet sweep, inverse.
+  NFGS_TRY(launch_transform(p, &p->dst_x, NFGS_FORWARD, at(p, ws->s_jz, ip), 1, nxi, ip,
+                            at(p, ws->hat0, ip), 1, nxi, ip, nyi, nz - 2, s));
+  NFGS_TRY(launch_transform(p, &p->dst_y, NFGS_FORWARD, at(p, ws->hat0, ip), nxi, 1, ip,
+                            at(p, ws->hat1, ip), nxi, 1, ip, nxi, nz - 2, s));
+  NFGS_TRY(nfgs_dirichlet_solve(p, bnd, ws->hat1, ws->hat0, ws->cscr, stream));
+  NFGS_TRY(launch_transform(p, &p->dst_x, NFGS_INVERSE, at(p, ws->hat0, ip), 1, nxi, ip,
+                            at(p, ws->hat1, ip), 1, nxi, ip, nyi, nz - 2, s));
+  NFGS_TRY(launch_transform(p, &p->dst_y, NFGS_INVERSE, at(p, ws->hat1, ip), nxi, 1, ip,
+                            at(p, az_out, pl + nx + 1), nx, 1, pl, nxi, nz - 2, s));
+
+  // New state, its face pass (next J_par and the monitors) and the reduction.
+  NFGS_TRY(nfgs_reconstruct_b(p, bnd, phi_out, az_out, b_out, stream));
+  NFGS_TRY(nfgs_face_pass(p, bnd, b_out, az_out, ws->jp, nullptr, ws->partials, stream));
+  NFGS_TRY(nfgs_reduce(p, ws->partials, ws->metrics, stream));
+  return NFGS_OK;
+}

@@ -1,0 +1,945 @@
+"""MI355X drop-in for the reference ``NFGSSolverGPU`` plain iterate.
+
+Same constructor, ``set_bottom_vector_boundary``, ``iterate`` and
+``fields_to_numpy`` as nfgs_solver.py:3905-7196, computed by the gfx950 HIP
+library (``libnfgs_hip.so``) on device tensors.  PyTorch provides device
+memory, the stream and host copies; every arithmetic step of the path runs
+in the HIP kernels.  There is no CPU path: without a ROCm device or without
+the built library the class raises.
+
+Supported profile (the reference defaults and production settings):
+``source_partition_mode='gaussian_separated'``, ``mean_bz_policy`` in
+{'uniform_open', 'require_balanced'}, ``horizontal_zero_mode_policy`` in
+{'frozen_linear_harmonic', 'omit'}, ``gaussian_horizontal_operator=
+'spectral'``, ``vertical_current_operator='staggered_face'``,
+``poloidal_update_mode='direct'``, projection weight 0, any
+``bottom_divergence_robin_weight`` in [0, 1], ``init_mode='gs_cfcf'``,
+``mu=1``, ``polarity_mask='both'``, ``stop_mode='sigma_best'``.  Other
+values the reference accepts raise ``NotImplementedError``; values it
+rejects raise its ``ValueError``.
+"""
+from __future__ import annotations
+
+import math
+import warnings
+from typing import Callable, Optional
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .grid import calculation_grid, centred_embedding, fortran_helper, sigma_window
+
+__all__ = ["NFGSSolverGPU"]
+
+
+def _norm(v) -> str:
+    return str(v).lower().replace("-", "_")
+
+
+def _unsupported(what: str):
+    raise NotImplementedError(f"{what} is outside the plain NFGS path implemented by nfgs_amd")
+
+
+class _Options:
+    """NFGS keyword parsing and validation (nfgs_solver.py:3925-4235)."""
+
+    def __init__(self, kw: dict):
+        pop = kw.pop
+        self.source_partition_mode = _norm(pop("source_partition_mode", "gaussian_separated"))
+        self.weak_field_floor_b2 = float(pop("weak_field_floor_b2", 1.0e-20))
+        alpha_clip_scale = pop("alpha_clip_scale", None)
+        if alpha_clip_scale is not None:
+            alpha_clip_scale = float(alpha_clip_scale)
+            if "alpha_limit_factor" in kw and float(kw["alpha_limit_factor"]) != alpha_clip_scale:
+                raise ValueError("alpha_clip_scale and alpha_limit_factor specify different values")
+            kw["alpha_limit_factor"] = alpha_clip_scale
+        self.mean_bz_policy = str(pop("mean_bz_policy", "uniform_open")).lower()
+        self.horizontal_zero_mode_policy = str(
+            pop("horizontal_zero_mode_policy", "frozen_linear_harmonic")).lower()
+        self.d0_zero_mode_policy = str(pop("d0_zero_mode_policy", "error")).lower()
+        self.harmonic_residual_policy = str(pop("harmonic_residual_policy", "diagnose")).lower()
+        self.source_zero_mode_policy = str(pop("source_zero_mode_policy", "error")).lower()
+        self.handoff_projection_policy = str(pop("handoff_projection_policy", "project")).lower()
+        self.mean_mode_tolerance = float(pop("mean_mode_tolerance", 1.0e-6))
+        self.source_zero_mode_tolerance = float(pop("source_zero_mode_tolerance", 1.0e-2))
+        self.k0_ode_smoothness = float(pop("k0_ode_smoothness", 1.0e-3))
+        self.coupled_k0_discretization = _norm(pop(
+            "coupled_k0_discretization", "staggered_face_D1_true_three_point_curvature"))
+        self.k0_update_mu = pop("k0_update_mu", None)
+        self.k0_spline_basis_size = int(pop("k0_spline_basis_size", 8))
+        k0_place = pop("k0_current_target_placement", None)
+        self.k0_current_target_placement = "cell" if k0_place is None else _norm(k0_place)
+        self.bottom_parallel_current_mode = _norm(pop("bottom_parallel_current_mode",
+                                                      "observed_full"))
+        self.handoff_projection_tolerance = float(pop("handoff_projection_tolerance", 1.0e-6))
+        self.projection_weight = float(pop("bottom_divergence_projection_weight", 0.0))
+        self.rho = float(pop("bottom_divergence_robin_weight", 0.0))
+        self.poloidal_update_mode = _norm(pop("poloidal_update_mode", "direct"))
+        self.record_iteration_diagnostics = bool(pop("record_iteration_diagnostics", False))
+        self.soft_cfcf_epsilon = pop("soft_cfcf_epsilon", None)
+        self.soft_cfcf_mode_batch_size = int(pop("soft_cfcf_mode_batch_size", 1024))
+        self.soft_cfcf_normal_ridge = float(pop("soft_cfcf_normal_ridge", 0.0))
+        self.user_supplied_open_bz = pop("user_supplied_open_bz", None)
+        self.user_supplied_horizontal_field = pop("user_supplied_horizontal_field", None)
+        self.external_potential_field = pop("external_potential_field", None)
+        poisson_z_order = int(pop("poisson_z_order", 2))
+        poisson_z_backend = str(pop("poisson_z_backend", "thomas")).lower()
+        ccu_pure_current = bool(pop("ccu_pure_current", True))
+        t_source = str(pop("t_source", "jz")).lower()
+        toroidal_boundary = bool(pop("toroidal_boundary", True))
+        poisson_mudpack_like = bool(pop("poisson_mudpack_like", True))
+        poisson_xy_order = int(pop("poisson_xy_order", 2))
+        fortran_bjt00 = bool(pop("fortran_bjt00", False))
+        fortran_phi_total_solve = bool(pop("fortran_phi_total_solve", False))
+        self.gaussian_horizontal_operator = str(pop("gaussian_horizontal_operator",
+                                                    "spectral")).lower()
+        self.vertical_current_operator = _norm(pop("vertical_current_operator", "staggered_face"))
+        # Stable-layer knobs (nfgs_solver.py:2911-2965)
+        alf = pop("alpha_limit_factor", 1.0)
+        self.alpha_limit_factor = None if alf is None else float(alf)
+        self.bottom_jperp_zero = bool(pop("bottom_jperp_zero", True))
+        self.relax_scalars = bool(pop("relax_scalars", True))
+        self.t_filter_scale = float(pop("t_filter_scale", 0.0))
+        self.bottom_imprint_robin_weight = float(pop("bottom_imprint_robin_weight", 0.0))
+        self._validate(poisson_z_order, poisson_z_backend, ccu_pure_current, t_source,
+                       toroidal_boundary, poisson_mudpack_like, poisson_xy_order, fortran_bjt00,
+                       fortran_phi_total_solve)
+
+    def _validate(self, pzo, pzb, ccu, tsrc, tor, mud, pxo, bjt, phit):
+        # NFGS v1 profile restrictions (nfgs_solver.py:4052-4076)
+        if pzo != 2:
+            raise ValueError("NFGS v1 supports only poisson_z_order=2 for the top Robin solve")
+        if pzb != "thomas":
+            raise ValueError("NFGS v1 supports only poisson_z_backend='thomas'")
+        if pxo != 2:
+            raise ValueError("NFGS v1 supports only poisson_xy_order=2")
+        if not mud:
+            raise ValueError("NFGS requires poisson_mudpack_like=True so separation and "
+                             "2-D Poisson use the same endpoint-Neumann spectrum")
+        if not ccu:
+            raise ValueError("NFGS requires ccu_pure_current=True")
+        if tsrc != "jz":
+            raise ValueError("NFGS v1 supports only t_source='jz'")
+        if not tor:
+            raise ValueError("NFGS v1 requires toroidal_boundary=True")
+        if bjt:
+            raise ValueError("NFGS requires fortran_bjt00=False to preserve explicit mean modes")
+        if phit:
+            raise ValueError("NFGS requires fortran_phi_total_solve=False")
+        # enumerations (nfgs_solver.py:4078-4235)
+        valid_mean = {"uniform_open", "require_balanced", "external_potential", "user_supplied"}
+        valid_h = {"omit", "frozen_linear_harmonic", "constrained_spline_harmonic", "coupled_k0",
+                   "external_potential", "user_supplied"}
+        valid_proj = {"error", "warn_project", "project"}
+        if self.source_partition_mode not in {"gaussian_separated", "legacy_single_field"}:
+            raise ValueError("source_partition_mode must be 'gaussian_separated' or "
+                             "'legacy_single_field'")
+        if self.weak_field_floor_b2 <= 0.0 or not math.isfinite(self.weak_field_floor_b2):
+            raise ValueError("weak_field_floor_b2 must be finite and positive")
+        if self.mean_bz_policy not in valid_mean:
+            raise ValueError(f"mean_bz_policy must be one of {sorted(valid_mean)}")
+        if self.horizontal_zero_mode_policy not in valid_h:
+            raise ValueError(f"horizontal_zero_mode_policy must be one of {sorted(valid_h)}")
+        if self.d0_zero_mode_policy not in valid_proj:
+            raise ValueError(f"d0_zero_mode_policy must be one of {sorted(valid_proj)}")
+        if self.source_zero_mode_policy not in valid_proj:
+            raise ValueError(f"source_zero_mode_policy must be one of {sorted(valid_proj)}")
+        if self.harmonic_residual_policy not in {"diagnose", "warn", "error"}:
+            raise ValueError("harmonic_residual_policy must be 'diagnose', 'warn', or 'error'")
+        if self.handoff_projection_policy not in {"project", "error"}:
+            raise ValueError("handoff_projection_policy must be 'project' or 'error'")
+        if self.mean_mode_tolerance <= 0.0 or self.source_zero_mode_tolerance <= 0.0:
+            raise ValueError("zero-mode tolerances must be positive")
+        if self.k0_ode_smoothness < 0.0:
+            raise ValueError("k0_ode_smoothness must be non-negative")
+        if self.coupled_k0_discretization not in {"staggered_face_d1_true_three_point_curvature",
+                                                  "legacy_plane_centred"}:
+            raise ValueError("coupled_k0_discretization must be one of "
+                             "['legacy_plane_centred', "
+                             "'staggered_face_d1_true_three_point_curvature']")
+        if self.k0_update_mu is not None and not 0.0 < float(self.k0_update_mu) <= 1.0:
+            raise ValueError("k0_update_mu must be in (0, 1]")
+        if self.k0_spline_basis_size < 4:
+            raise ValueError("k0_spline_basis_size must be >= 4")
+        if self.k0_current_target_placement not in {"face", "cell"}:
+            raise ValueError("k0_current_target_placement must be 'face' or 'cell'")
+        if self.bottom_parallel_current_mode not in {"observed_full", "face_parallel_exact_jz"}:
+            raise ValueError("bottom_parallel_current_mode must be one of "
+                             "['face_parallel_exact_jz', 'observed_full']")
+        if self.handoff_projection_tolerance <= 0.0:
+            raise ValueError("handoff_projection_tolerance must be positive")
+        if not 0.0 <= self.projection_weight <= 1.0:
+            raise ValueError("bottom_divergence_projection_weight must be between 0 and 1")
+        if not 0.0 <= self.rho <= 1.0:
+            raise ValueError("bottom_divergence_robin_weight must be between 0 and 1")
+        if self.projection_weight > 0.0 and self.rho > 0.0:
+            raise ValueError("Bottom-divergence projection and mixed Robin controls are "
+                             "mutually exclusive")
+        if self.poloidal_update_mode not in {"direct", "two_stage"}:
+            raise ValueError("poloidal_update_mode must be 'direct' or 'two_stage'")
+        if self.soft_cfcf_epsilon is not None and float(self.soft_cfcf_epsilon) < 0.0:
+            raise ValueError("soft_cfcf_epsilon must be non-negative")
+        if self.soft_cfcf_mode_batch_size <= 0:
+            raise ValueError("soft_cfcf_mode_batch_size must be positive")
+        if self.soft_cfcf_normal_ridge < 0.0:
+            raise ValueError("soft_cfcf_normal_ridge must be non-negative")
+        if self.gaussian_horizontal_operator not in {"fd", "spectral"}:
+            raise ValueError("gaussian_horizontal_operator must be one of ['fd', 'spectral']")
+        if self.vertical_current_operator not in {"staggered_face", "collocated_centered"}:
+            raise ValueError("vertical_current_operator must be one of "
+                             "['collocated_centered', 'staggered_face']")
+        if not 0.0 <= self.bottom_imprint_robin_weight <= 1.0:
+            raise ValueError("bottom_imprint_robin_weight must be between 0 and 1")
+        # supported subset of the valid configurations
+        if self.source_partition_mode != "gaussian_separated":
+            _unsupported("source_partition_mode='legacy_single_field'")
+        if self.mean_bz_policy not in {"uniform_open", "require_balanced"}:
+            _unsupported(f"mean_bz_policy={self.mean_bz_policy!r}")
+        if self.horizontal_zero_mode_policy not in {"frozen_linear_harmonic", "omit"}:
+            _unsupported(f"horizontal_zero_mode_policy={self.horizontal_zero_mode_policy!r}")
+        if self.external_potential_field is not None:
+            raise ValueError("external_potential_field was supplied but neither mean-mode policy "
+                             "selects 'external_potential'")
+        if self.gaussian_horizontal_operator != "spectral":
+            _unsupported("gaussian_horizontal_operator='fd'")
+        if self.vertical_current_operator != "staggered_face":
+            _unsupported("vertical_current_operator='collocated_centered'")
+        if self.poloidal_update_mode != "direct":
+            _unsupported("poloidal_update_mode='two_stage'")
+        if self.projection_weight != 0.0:
+            _unsupported("bottom_divergence_projection_weight > 0")
+        if self.bottom_parallel_current_mode != "observed_full":
+            _unsupported("bottom_parallel_current_mode='face_parallel_exact_jz'")
+        if self.record_iteration_diagnostics:
+            _unsupported("record_iteration_diagnostics=True")
+        if self.t_filter_scale != 0.0 or self.bottom_imprint_robin_weight != 0.0:
+            _unsupported("Stable-layer T filtering or bottom imprint")
+        if not self.bottom_jperp_zero:
+            _unsupported("bottom_jperp_zero=False")
+
+
+class NFGSSolverGPU:
+    """Gaussian-separated NFPT solver on MI355X (nfgs_solver.py:3905).
+
+    Arrays follow the reference: ``(component, z, y, x)``; ``dtype`` defaults
+    to float64 as in the reference base class (nfgs_solver.py:768).
+    """
+
+    def __init__(self, Nx, Ny, Nz, dx, dy, dz, device="cuda", *, buffer_thickness=None,
+                 pin_potential_boundaries: bool = False, lateral_wall="paper_bc1",
+                 padded_buffer_zone: bool = True, pad_calculation_domain: Optional[bool] = True,
+                 calculation_domain_mode="mudpack", dtype: torch.dtype = torch.float64, **kwargs):
+        opts = _Options(kwargs)
+        for key in ("buffer_xy_only", "rhs_lap_xy", "rhs_lap_xy_spectral_ext", "rhs_jz_div_chain",
+                    "lap_phi_z_boundary_mean", "finite_difference_order", "poisson_2d_interior_rhs_only",
+                    "poisson_2d_endpoint_dirichlet", "poisson_3d_endpoint_neumann_xy",
+                    "poisson_3d_top_neumann_one_sided", "fortran_iphiws3_potential"):
+            if key in kwargs:
+                val = kwargs.pop(key)
+                defaults = {"buffer_xy_only": False, "rhs_lap_xy": "fd", "rhs_jz_div_chain": False,
+                            "rhs_lap_xy_spectral_ext": "neumann", "finite_difference_order": 2,
+                            "lap_phi_z_boundary_mean": False}
+                if key in defaults and val != defaults[key]:
+                    _unsupported(f"{key}={val!r}")
+        if kwargs:
+            raise TypeError(f"unexpected keyword arguments: {sorted(kwargs)}")
+        if buffer_thickness not in (None, 0):
+            _unsupported("buffer_thickness > 0")
+        lw = _norm(lateral_wall)
+        if lw in ("bc1", "paper_bc1", "paper"):
+            lateral = "BC1"
+        elif lw in ("potential_pin", "legacy_potential_pin", "potentialpin", "pin", "legacy"):
+            lateral = "potential_pin"
+        else:
+            raise ValueError("lateral_wall must be one of: 'potential_pin', 'legacy_potential_pin', "
+                             f"'BC1', 'paper_bc1', 'paper'; got {lateral_wall!r}")
+        if opts.rho > 0.0 and lateral != "BC1":
+            raise ValueError("Stable-Robin currently requires lateral_wall='BC1'")
+        if pin_potential_boundaries:
+            _unsupported("pin_potential_boundaries=True")
+        if dtype not in (torch.float32, torch.float64):
+            raise ValueError("dtype must be torch.float32 or torch.float64")
+        self.opts = opts
+        self.source_partition_mode = opts.source_partition_mode
+        self.bottom_divergence_robin_weight = opts.rho
+        self.horizontal_zero_mode_policy = opts.horizontal_zero_mode_policy
+        self.vertical_current_operator = opts.vertical_current_operator
+        self.source_zero_mode_policy = opts.source_zero_mode_policy
+        self.source_zero_mode_tolerance = opts.source_zero_mode_tolerance
+        self.weak_field_floor_b2 = opts.weak_field_floor_b2
+        self.alpha_clip_scale = opts.alpha_limit_factor
+        self.record_iteration_diagnostics = False
+
+        self._input_shape_zyx = (int(Nz), int(Ny), int(Nx))
+        use_pad = bool(padded_buffer_zone) if pad_calculation_domain is None \
+            else bool(pad_calculation_domain)
+        cx, cy, cz, pad_xy = calculation_grid(int(Nx), int(Ny), int(Nz), use_pad,
+                                              calculation_domain_mode)
+        self._calculation_pad_xy = pad_xy
+        self._calculation_domain_mode = _norm(calculation_domain_mode)
+        self._padded_buffer_zone = bool(padded_buffer_zone)
+        self.Nx, self.Ny, self.Nz = cx, cy, cz
+        if self.Nz < 3:
+            raise ValueError("NFGS requires Nz >= 3")
+        if self.Nx < 3 or self.Ny < 3:
+            raise ValueError("the calculation grid needs at least 3 nodes per lateral axis")
+        self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+        self.dtype = dtype
+        if not torch.cuda.is_available():
+            raise RuntimeError("NFGSSolverGPU (nfgs_amd) needs a ROCm GPU: the HIP library has no "
+                               "CPU path")
+        dev = torch.device(device)
+        if dev.type != "cuda":
+            raise ValueError(f"nfgs_amd computes on the GPU only; got device={device!r}")
+        self.device = dev if dev.index is not None else torch.device("cuda", torch.cuda.current_device())
+        self._lib = L.load()
+        self._plan = None
+        self._reset_state()
+
+    # ------------------------------------------------------------------
+    def _reset_state(self):
+        self.B_background = None
+        self.Bz_obs = self.Bx_obs = self.By_obs = self.Jz_obs = None
+        self._B = self._J = self._Phi_c = self._Az = None
+        self.fov_slice_yx = None
+        self._observed_shape_zyx = None
+        self._source_zero_mode_history = []
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        if plan is not None and getattr(self, "_lib", None) is not None:
+            try:
+                torch.cuda.synchronize(self.device)
+            except Exception:  # pragma: no cover - interpreter teardown
+                pass
+            self._lib.nfgs_plan_destroy(plan)
+            self._plan = None
+
+    def alpha_max(self) -> Optional[float]:
+        """nfgs_solver.py:2967-2975."""
+        if self.opts.alpha_limit_factor is None:
+            return None
+        return self.opts.alpha_limit_factor / min(self.dx, self.dy, self.dz)
+
+    @property
+    def _stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _empty(self, *shape):
+        return torch.empty(shape, device=self.device, dtype=self.dtype)
+
+    def _zeros(self, *shape):
+        return torch.zeros(shape, device=self.device, dtype=self.dtype)
+
+    # ------------------------------------------------------------------
+    # Boundary setup (nfgs_solver.py:4661-4669, :4548-4605, :1999-2110)
+    # ------------------------------------------------------------------
+    def _plane_input(self, a, name):
+        t = torch.as_tensor(np.asarray(a) if not torch.is_tensor(a) else a)
+        t = t.to(device=self.device, dtype=self.dtype)
+        if t.ndim == 3:
+            if int(t.shape[0]) != self.Nz:
+                raise ValueError(f"{name} 3D input must be in (Nz, Ny, Nx) order with Nz={self.Nz}; "
+                                 f"got {tuple(t.shape)}.")
+            t = t[0]
+        elif t.ndim != 2:
+            raise ValueError(f"{name} must be 2D (Ny, Nx) or 3D (Nz, Ny, Nx); got {tuple(t.shape)}.")
+        return t.contiguous()
+
+    def set_bottom_vector_boundary(self, Bx_obs, By_obs, Bz_obs, **kwargs):
+        """Set NFGS from vector magnetogram maps in ``(Ny, Nx)`` order."""
+        return self.set_bottom_boundary(Bz_obs=Bz_obs, Jz_obs=None, Bx_obs=Bx_obs, By_obs=By_obs,
+                                        **kwargs)
+
+    def set_bottom_boundary(self, Bz_obs, Jz_obs=None, zero_net_current=None, *, Bx_obs=None,
+                            By_obs=None, smooth_passes=None, padded_buffer_zone=None, ijzbcor=3):
+        if Bx_obs is None or By_obs is None:
+            raise ValueError("NFGS requires Bx_obs and By_obs; use set_bottom_vector_boundary() "
+                             "or pass both tangential components to set_bottom_boundary().")
+        if Jz_obs is not None:
+            _unsupported("an explicit Jz_obs")
+        if zero_net_current:
+            _unsupported("zero_net_current=True")
+        if smooth_passes not in (None, 0):
+            _unsupported("smooth_passes > 0")
+        self._reset_state()
+        bz = self._plane_input(Bz_obs, "Bz_obs")
+        bx = self._plane_input(Bx_obs, "Bx_obs")
+        by = self._plane_input(By_obs, "By_obs")
+        if bx.shape != bz.shape or by.shape != bz.shape:
+            raise ValueError("Bx_obs/By_obs must have the same observed-FOV shape as Bz_obs. "
+                             f"Got Bx={tuple(bx.shape)}, By={tuple(by.shape)}, Bz={tuple(bz.shape)}.")
+        use_pad = self._padded_buffer_zone if padded_buffer_zone is None else bool(padded_buffer_zone)
+        Ny, Nx, Nz = self.Ny, self.Nx, self.Nz
+        if tuple(bz.shape) != (Ny, Nx):
+            if not use_pad:
+                raise ValueError(f"Bz_obs/Jz_obs must have shape ({Ny}, {Nx}); got {tuple(bz.shape)}.")
+            obs_ny, obs_nx = int(bz.shape[0]), int(bz.shape[1])
+            (sy0, sy1, sx0, sx1), (dy0, dy1, dx0, dx1) = centred_embedding(obs_ny, obs_nx, Ny, Nx)
+            planes = []
+            for t in (bx, by, bz):
+                e = self._zeros(Ny, Nx)
+                e[dy0:dy1, dx0:dx1] = t[sy0:sy1, sx0:sx1]
+                planes.append(e)
+            bx, by, bz = planes
+            self.fov_slice_yx = (slice(dy0, dy1), slice(dx0, dx1))
+            self._observed_shape_zyx = (min(Nz, self._input_shape_zyx[0]), obs_ny, obs_nx)
+        else:
+            self.fov_slice_yx = (slice(0, Ny), slice(0, Nx))
+            self._observed_shape_zyx = (Nz, Ny, Nx)
+        self.Bx_obs, self.By_obs, self.Bz_obs = bx, by, bz
+        self._make_plan()
+        s = self._stream
+        lib = self._lib
+
+        # Bottom Jz: legacy helper grid when the domain is padded (:4489-4546)
+        calc = (Nz, Ny, Nx)
+        padded = any(c > o for c, o in zip(calc, self._input_shape_zyx)) or \
+            any(c > o for c, o in zip(calc, self._observed_shape_zyx)) or \
+            any(p > 0 for p in self._calculation_pad_xy)
+        self.Jz_obs = self._empty(Ny, Nx)
+        if padded:
+            self._bottom_current_grid_mode = "legacy_helper"
+            h = fortran_helper(Nx, Ny, max(2, Nz))
+            L.check(lib.nfgs_setup_jz(self._plan, L.ptr(bx), L.ptr(by), L.ptr(self.Jz_obs), h.ny,
+                                      h.nx, h.bfy, h.bfx, int(ijzbcor), 1, s), "bottom Jz")
+        else:
+            self._bottom_current_grid_mode = "calculation_grid"
+            L.check(lib.nfgs_setup_jz(self._plan, L.ptr(bx), L.ptr(by), L.ptr(self.Jz_obs), Ny, Nx,
+                                      0, 0, int(ijzbcor), 0, s), "bottom Jz")
+
+        # Az_bottom (2-D Dirichlet Poisson of -Jz) and the toroidal bottom current
+        self.Az_bottom = self._empty(Ny, Nx)
+        work = self._empty(2 * (Ny - 2) * (Nx - 2))
+        L.check(lib.nfgs_setup_poisson2d_dirichlet(self._plan, L.ptr(self.Jz_obs), -1.0,
+                                                   L.ptr(self.Az_bottom), L.ptr(work), s), "Az_bottom")
+        self.Jz_toroidal_boundary = self._empty(Ny, Nx)
+        L.check(lib.nfgs_setup_plane_lap(self._plan, L.ptr(self.Az_bottom), -1.0, 1,
+                                         L.ptr(self.Jz_toroidal_boundary), s), "Jz_T")
+
+        # Gaussian separation (:4671-4919)
+        N = Nz * Ny * Nx
+        self.B_background = self._empty(3, Nz, Ny, Nx)
+        self.Bz_above = self._empty(Ny, Nx)
+        self.Phi_above_bottom = self._empty(Ny, Nx)
+        self._bottom_dphi_c_target = self._empty(Ny, Nx)
+        self.D0 = self._empty(Ny, Nx)
+        self._phi_above_hat = self._empty(Ny, Nx)
+        h_target = self._empty(2)
+        gdiag = torch.zeros(L.GDIAG_COUNT, device=self.device, dtype=torch.float64)
+        gws = torch.empty(int(lib.nfgs_gaussian_workspace_bytes(self._plan)), device=self.device,
+                          dtype=torch.uint8)
+        L.check(lib.nfgs_setup_gaussian(
+            self._plan, L.ptr(bx), L.ptr(by), L.ptr(bz), L.ptr(self.Az_bottom),
+            L.ptr(self.B_background), L.ptr(self.Bz_above), L.ptr(self.Phi_above_bottom),
+            L.ptr(self._bottom_dphi_c_target), L.ptr(self.D0), L.ptr(self._phi_above_hat),
+            L.ptr(h_target), L.ptr(gdiag), L.ptr(gws), s), "Gaussian separation")
+        del gws
+        # toroidal diagnostics (:3047-3055): sums from the device
+        sd = torch.zeros(4, device=self.device, dtype=torch.float64)
+        L.check(lib.nfgs_sq_diff(self._plan, L.ptr(self.Jz_toroidal_boundary), L.ptr(self.Jz_obs),
+                                 Ny * Nx, L.ptr(sd), s), "toroidal diagnostics")
+        L.check(lib.nfgs_sq_diff(self._plan, L.ptr(self.Jz_obs), L.ptr(self.Jz_toroidal_boundary),
+                                 Ny * Nx, L.ptr(sd) + 16, s), "toroidal diagnostics")
+        g = gdiag.cpu().numpy()
+        sdh = sd.cpu().numpy()
+        n2 = float(Ny * Nx)
+        self.toroidal_boundary_diagnostics = {
+            "t_source": "jz", "t_filter_scale": 0.0,
+            "raw_jz_rms": math.sqrt(sdh[1] / n2),
+            "t_implied_jz_rms": math.sqrt(sdh[3] / n2),
+            "raw_vs_t_implied_jz_rms": math.sqrt(sdh[0] / n2),
+        }
+        self._configure_gaussian_diagnostics(g, h_target)
+        self._bind_boundary()
+        return None
+
+    def _make_plan(self):
+        if self._plan is not None:
+            torch.cuda.synchronize(self.device)
+            self._lib.nfgs_plan_destroy(self._plan)
+            self._plan = None
+        win = sigma_window(self.Nz - 1, self.Ny, self.Nx, self._observed_shape_zyx,
+                           (self.fov_slice_yx[0].start, self.fov_slice_yx[1].start))
+        am = self.alpha_max()
+        desc = L.GridDesc(self.Nx, self.Ny, self.Nz, self.dx, self.dy, self.dz,
+                          L.NFGS_F32 if self.dtype == torch.float32 else L.NFGS_F64,
+                          self.opts.rho, am if am is not None else float("inf"),
+                          self.opts.weak_field_floor_b2, *win)
+        plan = L.C.c_void_p()
+        L.check(self._lib.nfgs_plan_create(L.C.byref(desc), L.C.byref(plan)), "plan")
+        self._plan = plan
+        self._window = win
+        self._partials = torch.zeros(int(self._lib.nfgs_partials_count(plan)), device=self.device,
+                                     dtype=torch.float64)
+        self._metrics = torch.zeros(L.NQ, device=self.device, dtype=torch.float64)
+        self._metrics_host = torch.zeros(L.NQ, dtype=torch.float64).pin_memory()
+
+    def _configure_gaussian_diagnostics(self, g, h_target):
+        o = self.opts
+        Ny, Nx = self.Ny, self.Nx
+        n2 = float(Ny * Nx)
+        bz_mean = g[L.GD["BZ_MEAN"]]
+        flux_imb = abs(g[L.GD["BZ_SUM"]]) / max(g[L.GD["BZ_ABS_SUM"]], 1e-30)
+        if o.mean_bz_policy == "require_balanced" and flux_imb > o.mean_mode_tolerance:
+            raise ValueError("The residual bottom Bz is not flux balanced: "
+                             f"fraction={flux_imb:.6e}, tolerance={o.mean_mode_tolerance:.6e}.")
+        d0_mean = g[L.GD["D0_MEAN"]]
+        d0_rms = math.sqrt(g[L.GD["D0_SQ"]] / n2)
+        d0_ratio = abs(d0_mean) / max(d0_rms, 1e-30)
+        self._handle_projected_zero_mode(
+            d0_ratio, o.mean_mode_tolerance, o.d0_zero_mode_policy,
+            "The bottom vector field is incompatible with the zero-mean endpoint-Neumann GS model")
+        res_rms = math.sqrt(g[L.GD["HRES_SQ"]] / (2 * n2))
+        obs_rms = math.sqrt(g[L.GD["HOBS_SQ"]] / (2 * n2))
+        ratio = res_rms / max(obs_rms, 1e-30)
+        if ratio > o.mean_mode_tolerance:
+            msg = ("Bottom horizontal reconstruction leaves a finite-domain harmonic "
+                   f"residual ratio={ratio:.6e}, tolerance={o.mean_mode_tolerance:.6e}.")
+            if o.harmonic_residual_policy == "error":
+                raise ValueError(msg)
+            if o.harmonic_residual_policy == "warn":
+                warnings.warn(msg, RuntimeWarning, stacklevel=3)
+        hc = [g[L.GD["HCONST_X"]], g[L.GD["HCONST_Y"]]]
+        self.Bz_open_mean = bz_mean
+        self.B_mean = [g[L.GD["BX_MEAN"]], g[L.GD["BY_MEAN"]], bz_mean]
+        self.gaussian_separation_diagnostics = {
+            "bottom_reconstruction_rms": math.sqrt(g[L.GD["RECON_SQ"]] / n2),
+            "bottom_reconstruction_max": g[L.GD["RECON_MAX"]],
+            "mean_field": list(self.B_mean),
+            "dct_zero_mode_Bx": self.B_mean[0],
+            "dct_zero_mode_By": self.B_mean[1],
+            "dct_zero_mode_Bz": self.B_mean[2],
+            "arithmetic_mean_B": [g[L.GD["BX_SUM"]] / n2, g[L.GD["BY_SUM"]] / n2,
+                                  g[L.GD["BZ_SUM"]] / n2],
+            "mean_bz_policy": o.mean_bz_policy,
+            "horizontal_zero_mode_policy": o.horizontal_zero_mode_policy,
+            "gaussian_horizontal_operator": o.gaussian_horizontal_operator,
+            "Bz_open_mean": bz_mean,
+            "Bh_mean_bottom": self.B_mean[:2],
+            "flux_imbalance_fraction": flux_imb,
+            "D0_zero_mode": d0_mean,
+            "D0_zero_mode_ratio": d0_ratio,
+            "d0_removed_constant_mode": d0_mean,
+            "background_scalar_plus_explicit_rms":
+                math.sqrt(g[L.GD["BG_INV_SQ"]] / (3.0 * self.Nz * n2)),
+            "bottom_horizontal_reconstruction_rms": res_rms,
+            "bottom_horizontal_reconstruction_ratio": ratio,
+            "harmonic_residual_constant": hc,
+            "harmonic_residual_constant_rms": math.sqrt((hc[0] ** 2 + hc[1] ** 2) / 2.0),
+            "harmonic_residual_nonconstant_rms": math.sqrt(g[L.GD["HNONCONST_SQ"]] / (2 * n2)),
+            "harmonic_residual_divergence_rms": math.sqrt(g[L.GD["HDIV_SQ"]] / n2),
+            "harmonic_residual_curl_rms": math.sqrt(g[L.GD["HCURL_SQ"]] / n2),
+        }
+        if o.horizontal_zero_mode_policy == "frozen_linear_harmonic":
+            self._H_k0 = h_target.view(2, 1).expand(2, self.Nz).contiguous()
+            self._k0_diagnostics = {
+                "enabled": True, "closure": "frozen_linear_harmonic",
+                "representation": "analytic_Az_linear_lift", "bottom_target": hc,
+                "current_free": True, "open_side_current": False, "top_derivative": [0.0, 0.0],
+                "profile_max_abs": max(abs(hc[0]), abs(hc[1])),
+            }
+        else:
+            self._H_k0 = None
+            self._k0_diagnostics = {}
+        self._H_k0_bottom_target = h_target
+
+    def _handle_projected_zero_mode(self, ratio, tolerance, policy, message):
+        """nfgs_solver.py:4477-4487."""
+        if ratio <= tolerance:
+            return
+        detail = f"{message} ratio={ratio:.6e} exceeds tolerance={tolerance:.6e}."
+        if policy == "error":
+            raise ValueError(detail)
+        if policy == "warn_project":
+            warnings.warn(detail + " The incompatible zero mode will be projected out.",
+                          RuntimeWarning, stacklevel=3)
+
+    def _bind_boundary(self):
+        """Modal boundary tables and the fixed planes of the state slots."""
+        lib, s, plan = self._lib, self._stream, self._plan
+        Nz, Ny, Nx = self.Nz, self.Ny, self.Nx
+        N = Nz * Ny * Nx
+        tmp = self._empty(Ny, Nx)
+        self._phi_bottom_hat = self._empty(Ny, Nx)
+        self._dct2_plane(self.Phi_above_bottom, self._phi_bottom_hat, tmp)
+        self._phi_bottom_hat[0, 0] = 0.0
+        self._q_hat = None
+        if self.opts.rho > 0.0:
+            self._q_hat = self._empty(Ny, Nx)
+            self._dct2_plane(self._bottom_dphi_c_target, self._q_hat, tmp)
+            self._q_hat[0, 0] = 0.0
+        nxi, nyi = Nx - 2, Ny - 2
+        t2 = self._empty(nyi * nxi)
+        self._az_bottom_hat = self._empty(nyi, nxi)
+        L.check(lib.nfgs_transform(plan, L.NFGS_DST1, L.NFGS_FORWARD,
+                                   L.ptr(self.Az_bottom) + (Nx + 1) * self.Az_bottom.element_size(),
+                                   1, Nx, 0, L.ptr(t2), 1, nxi, 0, nxi, nyi, 1, s), "DST x")
+        L.check(lib.nfgs_transform(plan, L.NFGS_DST1, L.NFGS_FORWARD, L.ptr(t2), nxi, 1, 0,
+                                   L.ptr(self._az_bottom_hat), nxi, 1, 0, nyi, nxi, 1, s), "DST y")
+        self._h_profile = self._H_k0 if self._H_k0 is not None else self._zeros(2, Nz)
+        self._bnd = L.Boundary(
+            L.ptr(self.B_background), L.ptr(self.Bz_obs), L.ptr(self.Jz_toroidal_boundary),
+            L.ptr(self.Phi_above_bottom), L.ptr(self.Az_bottom), L.ptr(self._phi_bottom_hat),
+            L.ptr(self._q_hat) if self._q_hat is not None else None, L.ptr(self._az_bottom_hat),
+            L.ptr(self._h_profile))
+        # work buffers and two (Phi_c, Az) slots whose fixed planes never change
+        ip = nyi * nxi
+        self._ws_t = dict(jp=self._empty(3, Nz, Ny, Nx), s_bcz=self._empty(N),
+                          s_jz=self._empty(Nz * ip), hat0=self._empty(N), hat1=self._empty(N),
+                          cscr=self._empty(N))
+        self._ws = L.Workspace(*(L.ptr(self._ws_t[k]) for k in
+                                 ("jp", "s_bcz", "s_jz", "hat0", "hat1", "cscr")),
+                               L.ptr(self._partials), L.ptr(self._metrics))
+        self._slots = []
+        for _ in range(2):
+            phi = self._zeros(Nz, Ny, Nx)
+            az = self._zeros(Nz, Ny, Nx)
+            phi[0].copy_(self.Phi_above_bottom)
+            az[0].copy_(self.Az_bottom)
+            self._slots.append((phi, az))
+        self._Bwork = self._empty(3, Nz, Ny, Nx)
+
+    def _dct2_plane(self, src, dst, tmp):
+        lib, s, plan = self._lib, self._stream, self._plan
+        Ny, Nx = self.Ny, self.Nx
+        L.check(lib.nfgs_transform(plan, L.NFGS_DCT1, L.NFGS_FORWARD, L.ptr(src), 1, Nx, 0,
+                                   L.ptr(tmp), 1, Nx, 0, Nx, Ny, 1, s), "DCT x")
+        L.check(lib.nfgs_transform(plan, L.NFGS_DCT1, L.NFGS_FORWARD, L.ptr(tmp), Nx, 1, 0,
+                                   L.ptr(dst), Nx, 1, 0, Ny, Nx, 1, s), "DCT y")
+
+    # ------------------------------------------------------------------
+    # One step and its monitors
+    # ------------------------------------------------------------------
+    def _require_boundary(self):
+        if self.B_background is None:
+            raise RuntimeError("NFGS boundary is not set. Call set_bottom_vector_boundary() first.")
+
+    def _advance(self, slot: int, zero_source: bool):
+        phi, az = self._slots[slot]
+        L.check(self._lib.nfgs_advance(self._plan, L.C.byref(self._bnd), L.C.byref(self._ws),
+                                       L.ptr(phi), L.ptr(az), L.ptr(self._Bwork), int(zero_source),
+                                       self._stream), "iteration")
+
+    def _read_metrics(self):
+        self._metrics_host.copy_(self._metrics, non_blocking=True)
+        torch.cuda.current_stream(self.device).synchronize()
+        m = self._metrics_host.numpy()
+        Nz, Ny, Nx = self.Nz, self.Ny, self.Nx
+        N = float(Nz * Ny * Nx)
+        sigma = m[L.Q_SIGMA_NUM] / max(m[L.Q_SIGMA_DEN], 1.0e-30)
+        delta = min(self.dx, self.dy, self.dz)
+        divb = (m[L.Q_DIV_ABS] / N) / (6.0 * (m[L.Q_BMAG] / N) / delta + 1e-12)
+        energy = m[L.Q_B2] * self.dx * self.dy * self.dz / 2.0
+        norm_rms = math.sqrt(m[L.Q_ZERO_MODE] / Nz)
+        src_rms = math.sqrt(m[L.Q_SRC2] / N)
+        ratio = norm_rms / max(src_rms, 1.0e-30)
+        return float(sigma), float(divb), float(energy), float(ratio)
+
+    def _check_zero_mode(self, ratio: float):
+        self._source_zero_mode_history.append(ratio)
+        self._handle_projected_zero_mode(ratio, self.source_zero_mode_tolerance,
+                                         self.source_zero_mode_policy,
+                                         "The Phi_c nonlinear source has an incompatible zero mode")
+
+    def _state_fields(self, slot: int, B_out=None, J_out=None, face_diag: bool = False):
+        """Rebuild B and J of a slot (identical to the iteration's own)."""
+        phi, az = self._slots[slot]
+        B = self._empty(3, self.Nz, self.Ny, self.Nx) if B_out is None else B_out
+        J = self._empty(3, self.Nz, self.Ny, self.Nx) if J_out is None else J_out
+        L.check(self._lib.nfgs_reconstruct_b(self._plan, L.C.byref(self._bnd), L.ptr(phi),
+                                             L.ptr(az), L.ptr(B), self._stream), "reconstruct")
+        L.check(self._lib.nfgs_face_pass(self._plan, L.C.byref(self._bnd), L.ptr(B), L.ptr(az),
+                                         None, L.ptr(J), L.ptr(self._partials) if face_diag else None,
+                                         self._stream), "face pass")
+        if face_diag:
+            L.check(self._lib.nfgs_reduce(self._plan, L.ptr(self._partials), L.ptr(self._metrics),
+                                          self._stream), "reduce")
+        return B, J
+
+    def compute_initial_conditions(self, *, init_mode="gs_cfcf", **_unused):
+        """gs_cfcf start (nfgs_solver.py:6613-6645); returns (B, J, Phi_c, Az, None)."""
+        mode = _norm(init_mode)
+        if mode == "yi_soft":
+            mode = "soft_cfcf"
+        if mode not in ("gs_cfcf", "cfcf", "curl_free_current", "soft_cfcf"):
+            raise ValueError("NFGS supports init_mode='gs_cfcf', 'cfcf', or 'soft_cfcf'")
+        self._require_boundary()
+        if mode != "gs_cfcf":
+            _unsupported(f"init_mode={init_mode!r}")
+        self._advance(0, zero_source=True)
+        sigma, _, _, ratio = self._read_metrics()
+        self._check_zero_mode(ratio)
+        B, J = self._state_fields(0)
+        phi, az = self._slots[0]
+        return B, J, phi.clone(), az.clone(), None
+
+    def iteration_step(self, B, J, Phi_c, Az, B_c=None, mu=1.0, polarity_mask: str = "both"):
+        """One map of the state (nfgs_solver.py:6647-6935) at mu = 1.
+
+        The state must come from this solver (its J is the current of its B
+        and Az); B_c is not used at mu = 1 and is returned as None.
+        """
+        self._require_boundary()
+        if float(mu) != 1.0:
+            _unsupported("mu != 1")
+        if polarity_mask != "both":
+            if polarity_mask in ("positive", "negative"):
+                _unsupported(f"polarity_mask={polarity_mask!r}")
+            raise ValueError("polarity_mask must be 'positive', 'negative', or 'both'")
+        Bc = B.to(device=self.device, dtype=self.dtype).contiguous()
+        azc = Az.to(device=self.device, dtype=self.dtype).contiguous()
+        L.check(self._lib.nfgs_face_pass(self._plan, L.C.byref(self._bnd), L.ptr(Bc), L.ptr(azc),
+                                         L.ptr(self._ws_t["jp"]), None, None, self._stream),
+                "face pass")
+        self._advance(1, zero_source=False)
+        _, _, _, ratio = self._read_metrics()
+        self._check_zero_mode(ratio)
+        Bn, Jn = self._state_fields(1)
+        phi, az = self._slots[1]
+        return Bn, Jn, phi.clone(), az.clone(), None
+
+    # ------------------------------------------------------------------
+    # Driver (nfgs_solver.py:3522-3850, :6937-7009)
+    # ------------------------------------------------------------------
+    def iterate(self, max_iter=200, tol=1e-5, mu=1.0, patience=1, verbose=True,
+                polarity_mask: str = "both", init_mode="gs_cfcf", auto_rbsl_field_mode="direct",
+                auto_rbsl_include_residual_current=True, max_mfr=5,
+                sigma_improvement_tol: Optional[float] = 1e-5, auto_rbsl_improvement_grace=20,
+                mfr_override=None, axis_n_pts=64, height_frac=1.0, min_sep_px=None,
+                radius_frac=0.25, chunk_size=None, work_dtype=None, axis_mode="jhook_pil",
+                use_mirror_path=True, jhook_quantile=0.80, jhook_search_radius_frac=0.35,
+                jhook_search_radius_px=None, jhook_min_component_px=4,
+                pil_corridor_width_frac=0.40, pil_corridor_width_px=None, pil_n_ctrl=9,
+                pil_smooth_passes=2, jhook_arc_fit_samples=100,
+                per_step_callback: Optional[Callable] = None, *, adaptive_mu: bool = False,
+                mu_min=0.02, mu_max=1.0, mu_backoff=0.5, mu_growth=1.1, growth_streak=5,
+                divergence_factor=2.0, max_rollbacks=12, stop_mode: str = "sigma_best",
+                fixed_point_update_tol=None, fixed_point_pde_tol=1.0e-8,
+                fixed_point_observable_tol=None, fixed_point_patience=10):
+        mode = _norm(init_mode)
+        if mode == "yi_soft":
+            mode = "soft_cfcf"
+        if mode not in ("gs_cfcf", "cfcf", "curl_free_current", "soft_cfcf"):
+            raise ValueError("NFGS supports init_mode='gs_cfcf', 'cfcf', or 'soft_cfcf'")
+        stop_mode = _norm(stop_mode)
+        if stop_mode == "fixed_point":
+            raise ValueError("stop_mode='fixed_point' requires record_iteration_diagnostics=True")
+        if stop_mode not in {"sigma_best", "fixed_point"}:
+            raise ValueError("stop_mode must be 'sigma_best' or 'fixed_point'")
+        update_tol = float(tol if fixed_point_update_tol is None else fixed_point_update_tol)
+        observable_tol = float(tol if fixed_point_observable_tol is None
+                               else fixed_point_observable_tol)
+        if update_tol <= 0.0 or float(fixed_point_pde_tol) <= 0.0 or observable_tol <= 0.0:
+            raise ValueError("fixed-point tolerances must be positive")
+        if int(fixed_point_patience) < 1:
+            raise ValueError("fixed_point_patience must be >= 1")
+        if polarity_mask not in ("positive", "negative", "both"):
+            raise ValueError("polarity_mask must be 'positive', 'negative', or 'both'")
+        if polarity_mask != "both":
+            _unsupported(f"polarity_mask={polarity_mask!r}")
+        if float(mu) != 1.0 or adaptive_mu:
+            _unsupported("mu != 1 or adaptive_mu")
+        self._require_boundary()
+        if mode != "gs_cfcf":
+            _unsupported(f"init_mode={init_mode!r}")
+
+        # initial state in slot 0 (gs_cfcf) and its monitors
+        self._advance(0, zero_source=True)
+        sigma0, divB0, E0, ratio0 = self._read_metrics()
+        self._check_zero_mode(ratio0)
+        if verbose:
+            am = self.alpha_max()
+            am_text = f"{am:.3e}" if am is not None else "off"
+            print(f"Initial  sigma={sigma0:.4e}  divB={divB0:.4e}  E={E0:.4e} "
+                  f"(mu0={mu}, adaptive={adaptive_mu}, alpha_max={am_text}, toroidal_bc=True)")
+        history = {k: [] for k in ("sigma", "divB", "energy", "mu", "rollback",
+                                   "sigma_relative_change", "energy_relative_change",
+                                   "fixed_point_criteria_met")}
+        best_sigma, best_iter, best_slot = sigma0, 0, 0
+        final_slot = 0
+        stall = flat = 0
+        need = max(int(patience), 1)
+        prev = sigma0
+        use_imp = sigma_improvement_tol is not None and float(sigma_improvement_tol) > 0.0
+        eps = float(sigma_improvement_tol) if use_imp else 0.0
+        last_imp = None
+        reason = "max_iter"
+        final_iter = 0
+        J_cb = None
+        for n in range(1, int(max_iter) + 1):
+            slot = 1 - best_slot
+            self._advance(slot, zero_source=False)
+            sigma, divB, energy, ratio = self._read_metrics()
+            self._check_zero_mode(ratio)
+            final_slot = slot
+            pe = history["energy"][-1] if history["energy"] else E0
+            history["sigma"].append(sigma)
+            history["divB"].append(divB)
+            history["energy"].append(energy)
+            history["mu"].append(1.0)
+            history["rollback"].append(0)
+            history["sigma_relative_change"].append(abs(sigma - prev) / max(abs(prev), 1.0e-30))
+            history["energy_relative_change"].append(abs(energy - pe) / max(abs(pe), 1.0e-30))
+            history["fixed_point_criteria_met"].append(0)
+            final_iter = n
+            if per_step_callback is not None:
+                if J_cb is None:
+                    J_cb = self._empty(3, self.Nz, self.Ny, self.Nx)
+                L.check(self._lib.nfgs_face_pass(self._plan, L.C.byref(self._bnd),
+                                                 L.ptr(self._Bwork), L.ptr(self._slots[slot][1]),
+                                                 None, L.ptr(J_cb), None, self._stream), "J")
+                per_step_callback(n, sigma, self._Bwork, J_cb)
+            imp = prev - sigma
+            last_imp = imp
+            prev = sigma
+            flat = flat + 1 if imp < eps else 0
+            if sigma < best_sigma:
+                best_sigma, best_iter, best_slot = sigma, n, slot
+                stall = 0
+            else:
+                stall += 1
+            if verbose and (n == 1 or n % 10 == 0):
+                print(f"Iter {n:4d} | sigma={sigma:.4e} | divB={divB:.4e} | E={energy:.4e} | "
+                      f"mu={1.0:.3f}")
+            if sigma < tol:
+                reason = "tol"
+                if verbose:
+                    print(f"Converged at iteration {n}")
+                break
+            if use_imp and flat >= need:
+                reason = "sigma_improvement_tol"
+                if verbose:
+                    print(f"Early stop at iter {n}: sigma improvement flat for {flat} steps "
+                          f"(last {imp:.4e} < {eps:.4e}; best sigma={best_sigma:.4e} at iter "
+                          f"{best_iter})")
+                break
+            if patience > 0 and stall >= patience:
+                reason = "sigma_stall"
+                if verbose:
+                    print(f"Early stop at iter {n}: sigma not improving (best sigma="
+                          f"{best_sigma:.4e} at iter {best_iter})")
+                break
+
+        # selected (best) state, final-vs-best and the staggered diagnostic
+        final_sigma = history["sigma"][-1] if history["sigma"] else sigma0
+        B_best, J_best = self._state_fields(best_slot, face_diag=True)
+        m = self._metrics.cpu().numpy()
+        N = float(self.Nz * self.Ny * self.Nx)
+        self._last_staggered_face_diagnostics = {
+            "operator": "vertical_face_curl_then_conservative_cell_projection",
+            "face_current_rms": math.sqrt(m[L.Q_FACE_J2] / (3.0 * (self.Nz - 1) * self.Ny * self.Nx)),
+            "projected_cell_current_rms": math.sqrt(m[L.Q_CELL_J2] / (3.0 * N)),
+            "face_planes": self.Nz - 1,
+        }
+        if final_slot != best_slot:
+            B_fin, _ = self._state_fields(final_slot)
+            d = torch.zeros(2, device=self.device, dtype=torch.float64)
+            L.check(self._lib.nfgs_sq_diff(self._plan, L.ptr(B_fin), L.ptr(B_best), 3 * int(N),
+                                           L.ptr(d), self._stream), "final vs best")
+            dh = d.cpu().numpy()
+            fvb = math.sqrt(dh[0] / (3 * N)) / max(math.sqrt(dh[1] / (3 * N)), 1.0e-30)
+            del B_fin
+        else:
+            fvb = 0.0
+        self._B, self._J = B_best, J_best
+        self._Phi_c, self._Az = (t.clone() for t in self._slots[best_slot])
+        history.update(
+            best_sigma=best_sigma, best_iter=best_iter, rollbacks=0, stop_reason=reason,
+            stop_mode="sigma_best", selected_state="best_sigma", final_iter=final_iter,
+            final_sigma=float(final_sigma), final_vs_best_B_relative=float(fvb),
+            fixed_point_converged=False, fixed_point_streak=0,
+            fixed_point_tolerances={"update": update_tol, "pde": float(fixed_point_pde_tol),
+                                    "observable": observable_tol,
+                                    "patience": int(fixed_point_patience)},
+            sigma_improvement_tol=eps if use_imp else None, auto_rbsl_improvement_grace=0,
+            last_sigma_improvement=last_imp,
+            init_diagnostics={"init_mode": "gs_cfcf", "mfr_count": 0,
+                              "gaussian_separation": self.gaussian_separation_diagnostics,
+                              "coupled_k0": self._k0_diagnostics},
+            source_partition_mode=self.source_partition_mode,
+            bottom_current_grid_mode=self._bottom_current_grid_mode,
+            vertical_current_operator=self.vertical_current_operator,
+            sigma_placement="vertical_faces",
+        )
+        history["source_diagnostics"] = self.compute_source_diagnostics()
+        return history
+
+    # ------------------------------------------------------------------
+    def compute_source_diagnostics(self, B=None) -> dict:
+        """Source-attribution diagnostics of the selected state (nfgs_solver.py:7014-7196)."""
+        self._require_boundary()
+        if B is None:
+            B = self._B
+        if B is None:
+            raise RuntimeError("No solution stored; call iterate() first.")
+        B = B.to(device=self.device, dtype=self.dtype).contiguous()
+        ws = torch.empty(int(self._lib.nfgs_source_diagnostics_workspace_bytes(self._plan)),
+                         device=self.device, dtype=torch.uint8)
+        out = torch.zeros(L.SD_COUNT, device=self.device, dtype=torch.float64)
+        L.check(self._lib.nfgs_source_diagnostics(
+            self._plan, L.C.byref(self._bnd), L.ptr(B), L.ptr(self.D0),
+            L.ptr(self._bottom_dphi_c_target), L.ptr(self.Bz_above), L.ptr(ws), L.ptr(out),
+            self._stream), "source diagnostics")
+        o = out.cpu().numpy()
+        n2 = float(self.Ny * self.Nx)
+        rms = lambda v: math.sqrt(v / n2)
+        hist = self._source_zero_mode_history
+        o_ = self.opts
+        diag = {
+            "source_partition_mode": self.source_partition_mode,
+            "weak_field_floor_B2": self.weak_field_floor_b2,
+            "alpha_clip_scale": self.alpha_clip_scale,
+            "bottom_reconstruction_rms": rms(o[0]),
+            "bottom_reconstruction_max": float(o[1]),
+            "correction_bottom_rms": rms(o[2]),
+            "top_robin_rms": rms(o[3]),
+            "top_phi_robin_rms": rms(o[4]),
+            "top_upward_source_rms": rms(o[5]),
+            "top_upward_source_ratio": rms(o[5]) / max(rms(o[6]), 1e-30),
+            "source_zero_mode_ratio_last": hist[-1] if hist else 0.0,
+            "source_zero_mode_ratio_max": max(hist) if hist else 0.0,
+            "source_zero_mode_tolerance": o_.source_zero_mode_tolerance,
+            "source_zero_mode_policy": o_.source_zero_mode_policy,
+            "bottom_divergence_projection_weight": o_.projection_weight,
+            "bottom_divergence_robin_weight": o_.rho,
+            "poloidal_update_mode": o_.poloidal_update_mode,
+            "vertical_current_operator": self.vertical_current_operator,
+            "bottom_current_grid_mode": self._bottom_current_grid_mode,
+            "vertical_staggered_face": dict(getattr(self, "_last_staggered_face_diagnostics", {})),
+            "bottom_divergence_control_mode": "mixed_robin" if o_.rho > 0.0 else "none",
+            "bottom_divergence_rms": rms(o[7]),
+            "bottom_divergence_relative": math.sqrt(o[7] / n2) / max(math.sqrt(o[8] / n2), 1e-30),
+            "bottom_dphi_residual_rms": rms(o[9]),
+            "bottom_divergence_projection_last": None,
+        }
+        tor = self.toroidal_boundary_diagnostics or {}
+        raw = float(tor.get("raw_jz_rms", 0.0) or 0.0)
+        proj = float(tor.get("raw_vs_t_implied_jz_rms", 0.0) or 0.0)
+        diag.update({
+            "Jz_projected_out_rms": proj,
+            "Jz_projected_out_over_Jz_rms": proj / max(raw, 1.0e-30),
+            "B_harm_over_Bobs_rms": float(self.gaussian_separation_diagnostics.get(
+                "bottom_horizontal_reconstruction_ratio", 0.0)),
+        })
+        diag.update(self.gaussian_separation_diagnostics)
+        if o_.horizontal_zero_mode_policy == "frozen_linear_harmonic":
+            prof = self._H_k0.double().cpu().numpy()
+            diag["frozen_linear_harmonic"] = dict(self._k0_diagnostics)
+            diag["frozen_linear_harmonic"]["profile_by_height"] = [list(map(float, r)) for r in prof]
+        return diag
+
+    def fields_to_numpy(self, *, crop_to_observed_fov: bool = True):
+        """nfgs_solver.py:2881-2903."""
+        if self._B is None or self._J is None:
+            raise RuntimeError("No solution stored; call iterate() first.")
+        B, J = self._B, self._J
+        if crop_to_observed_fov and self.fov_slice_yx is not None:
+            if self._observed_shape_zyx is not None:
+                nzo = self._observed_shape_zyx[0]
+                B, J = B[:, :nzo], J[:, :nzo]
+            sy, sx = self.fov_slice_yx
+            B, J = B[..., sy, sx], J[..., sy, sx]
+        return {"Bx": B[0].cpu().numpy(), "By": B[1].cpu().numpy(), "Bz": B[2].cpu().numpy(),
+                "Jx": J[0].cpu().numpy(), "Jy": J[1].cpu().numpy(), "Jz": J[2].cpu().numpy()}

@@ -1,0 +1,192 @@
+"""HIP path against the oracle and the reference's golden vectors (GPU)."""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import nfgs_oracle as O
+from tests.golden_cases import case_files, load, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+CASES = case_files()
+
+
+def _announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no ROCm device")
+    from nfgs_amd import _lib as L
+    return L.load()
+
+
+def _plan(nx, ny, nz, dtype):
+    from nfgs_amd import _lib as L
+    desc = L.GridDesc(nx, ny, nz, 1.0, 1.0, 1.0, L.NFGS_F32 if dtype == torch.float32 else
+                      L.NFGS_F64, 0.0, 1.0, 1e-20, 1, 1, 0, 0, 0, 0)
+    plan = C.c_void_p()
+    L.check(L.load().nfgs_plan_create(C.byref(desc), C.byref(plan)))
+    return plan
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("shape", [(9, 14, 14), (5, 13, 17), (3, 66, 66), (3, 258, 258),
+                                   (2, 130, 98)])
+def test_transforms_match_oracle(dtype, shape):
+    from nfgs_amd import _lib as L
+    nz, ny, nx = shape
+    _announce(f"transform {shape} {dtype}")
+    lib = L.load()
+    plan = _plan(nx, ny, max(nz, 3), dtype)
+    try:
+        rng = np.random.default_rng(nx * 31 + ny)
+        x = rng.standard_normal(shape)
+        xt = torch.tensor(x, dtype=dtype, device="cuda")
+        out = torch.zeros_like(xt)
+        s = torch.cuda.current_stream().cuda_stream
+        tol = 1e-12 if dtype == torch.float64 else 3e-6
+        pl = ny * nx
+        for kind, fwd, inv in ((L.NFGS_DCT1, O.dct1, O.idct1), (L.NFGS_DST1, O.dst1, O.idst1)):
+            for direction, ref in ((L.NFGS_FORWARD, fwd), (L.NFGS_INVERSE, inv)):
+                if kind == L.NFGS_DCT1:
+                    # along x (elements contiguous) and along y (vectors contiguous)
+                    out.zero_()
+                    L.check(lib.nfgs_transform(plan, kind, direction, xt.data_ptr(), 1, nx, pl,
+                                               out.data_ptr(), 1, nx, pl, nx, ny, nz, s))
+                    assert rel_l2(out.cpu().numpy(), ref(x, -1)) < tol
+                    out.zero_()
+                    L.check(lib.nfgs_transform(plan, kind, direction, xt.data_ptr(), nx, 1, pl,
+                                               out.data_ptr(), nx, 1, pl, ny, nx, nz, s))
+                    assert rel_l2(out.cpu().numpy(), ref(x, -2)) < tol
+                else:
+                    # interior vectors, as the Dirichlet solves use them
+                    es = xt.element_size()
+                    out.zero_()
+                    L.check(lib.nfgs_transform(plan, kind, direction, xt.data_ptr() + (nx + 1) * es,
+                                               1, nx, pl, out.data_ptr() + (nx + 1) * es, 1, nx, pl,
+                                               nx - 2, ny - 2, nz, s))
+                    got = out.cpu().numpy()[:, 1:-1, 1:-1]
+                    assert rel_l2(got, ref(x[:, 1:-1, 1:-1], -1)) < tol
+                    out.zero_()
+                    L.check(lib.nfgs_transform(plan, kind, direction, xt.data_ptr() + (nx + 1) * es,
+                                               nx, 1, pl, out.data_ptr() + (nx + 1) * es, nx, 1, pl,
+                                               ny - 2, nx - 2, nz, s))
+                    got = out.cpu().numpy()[:, 1:-1, 1:-1]
+                    assert rel_l2(got, ref(x[:, 1:-1, 1:-1], -2)) < tol
+        torch.cuda.synchronize()
+    finally:
+        lib.nfgs_plan_destroy(plan)
+
+
+def _solver_for(meta):
+    from nfgs_amd import NFGSSolverGPU
+    c = meta["case"]
+    nxo, nyo, nz = c["observed"]
+    dtype = torch.float64 if c["dtype"] == "float64" else torch.float32
+    return NFGSSolverGPU(nxo, nyo, nz, 1.0, 1.0, 1.0, device="cuda", dtype=dtype,
+                         lateral_wall="paper_bc1", buffer_thickness=0, padded_buffer_zone=True,
+                         pad_calculation_domain=c["pad"], calculation_domain_mode=c["mode"],
+                         bottom_divergence_robin_weight=c["rho"], mean_bz_policy="uniform_open",
+                         horizontal_zero_mode_policy="frozen_linear_harmonic",
+                         d0_zero_mode_policy="project", source_zero_mode_policy="project",
+                         harmonic_residual_policy="diagnose", poloidal_update_mode="direct",
+                         vertical_current_operator="staggered_face")
+
+
+def _t(x):
+    return x.detach().double().cpu().numpy()
+
+
+# Tolerances: FP64 is the 1e-10 relative-L2 bar; FP32 fields are compared with
+# the reference's own FP32 run, whose roundoff differs from ours (dense-GEMM
+# versus FFT transforms), so the per-field bar is 2e-5 for B and 2e-4 for the
+# derived currents and scalars (derivatives amplify roundoff by ~1/h).
+def _tols(c):
+    if c["dtype"] == "float64":
+        return dict(B=1e-10, J=1e-10, S=1e-10, plane=1e-10)
+    return dict(B=2e-5, J=2e-4, S=2e-4, plane=2e-5)
+
+
+@pytest.mark.parametrize("path", CASES, ids=[os.path.basename(p)[:-4] for p in CASES])
+def test_setup_init_and_steps_match_reference(path):
+    g, meta = load(path)
+    c = meta["case"]
+    _announce(f"golden setup/steps {c['name']}")
+    tol = _tols(c)
+    s = _solver_for(meta)
+    s.set_bottom_vector_boundary(g["in_bx"], g["in_by"], g["in_bz"], zero_net_current=False,
+                                 smooth_passes=0, ijzbcor=3)
+    assert (s.Nz, s.Ny, s.Nx) == tuple(g["calc_shape_zyx"])
+    assert (s.fov_slice_yx[0].start, s.fov_slice_yx[0].stop, s.fov_slice_yx[1].start,
+            s.fov_slice_yx[1].stop) == tuple(g["fov_yx"])
+    for key, val in [("Jz_obs", s.Jz_obs), ("Az_bottom", s.Az_bottom), ("Jz_T", s.Jz_toroidal_boundary),
+                     ("B_background", s.B_background), ("Phi_above_bottom", s.Phi_above_bottom),
+                     ("Bz_above", s.Bz_above), ("D0", s.D0), ("dphi_target", s._bottom_dphi_c_target)]:
+        assert rel_l2(_t(val), g[key]) < tol["plane"], key
+    assert np.max(np.abs(_t(s._H_k0) - g["H_k0"])) < tol["plane"] * np.max(np.abs(g["Bz_obs"]))
+    st = s.compute_initial_conditions()
+    for k, v, key in zip(("B", "J", "Phi_c", "Az"), st[:4], ("B", "J", "S", "S")):
+        assert rel_l2(_t(v), g["init_" + k]) < tol[key], "init_" + k
+    for step in (1, 2):
+        st = s.iteration_step(*st)
+        for k, v, key in zip(("B", "J", "Phi_c", "Az"), st[:4], ("B", "J", "S", "S")):
+            assert rel_l2(_t(v), g[f"step{step}_{k}"]) < tol[key], f"step{step}_{k}"
+
+
+@pytest.mark.parametrize("path", CASES, ids=[os.path.basename(p)[:-4] for p in CASES])
+def test_iterate_matches_reference_trajectory(path):
+    g, meta = load(path)
+    c = meta["case"]
+    _announce(f"golden iterate {c['name']}")
+    tol = _tols(c)
+    s = _solver_for(meta)
+    s.set_bottom_vector_boundary(g["in_bx"], g["in_by"], g["in_bz"], zero_net_current=False,
+                                 smooth_passes=0, ijzbcor=3)
+    it = c["iterate"]
+    h = s.iterate(init_mode="gs_cfcf", max_iter=it["max_iter"], tol=it["tol"], mu=1,
+                  patience=it["patience"], sigma_improvement_tol=it["sigma_improvement_tol"],
+                  stop_mode="sigma_best", verbose=False)
+    ref_sigma = g["hist_sigma"]
+    n = min(len(h["sigma"]), len(ref_sigma))
+    assert rel_l2(h["sigma"][:n], ref_sigma[:n]) < tol["S"]
+    assert rel_l2(h["energy"][:n], g["hist_energy"][:n]) < tol["B"] * 10
+    assert rel_l2(h["divB"][:n], g["hist_divB"][:n]) < tol["S"] * 10
+    if c["dtype"] == "float64":
+        assert h["stop_reason"] == meta["stop_reason"]
+        if h["best_iter"] != meta["best_iter"]:
+            assert abs(ref_sigma[h["best_iter"] - 1] - ref_sigma[meta["best_iter"] - 1]) < 1e-13
+        fields = s.fields_to_numpy()
+        for k, v in fields.items():
+            assert v.shape == g["out_" + k].shape
+            assert rel_l2(v, g["out_" + k]) < (tol["B"] if k.startswith("B") else tol["J"]), k
+        sd = h["source_diagnostics"]
+        for k, v in meta["source_diagnostics"].items():
+            assert abs(sd[k] - v) <= 1e-7 * max(1.0, abs(v)), k
+    else:
+        fields = s.fields_to_numpy()
+        if h["best_iter"] == meta["best_iter"]:
+            for k in ("Bx", "By", "Bz"):
+                assert rel_l2(fields[k], g["out_" + k]) < tol["B"] * 5, k
+
+
+def test_unsupported_options_raise():
+    from nfgs_amd import NFGSSolverGPU
+    with pytest.raises(NotImplementedError):
+        NFGSSolverGPU(12, 10, 9, 1, 1, 1, poloidal_update_mode="two_stage")
+    with pytest.raises(ValueError):
+        NFGSSolverGPU(12, 10, 9, 1, 1, 1, bottom_divergence_robin_weight=1.5)
+    with pytest.raises(ValueError):
+        NFGSSolverGPU(12, 10, 9, 1, 1, 1, poisson_z_backend="pcr")
+    s = NFGSSolverGPU(12, 10, 9, 1, 1, 1)
+    with pytest.raises(RuntimeError):
+        s.fields_to_numpy()
+    with pytest.raises(RuntimeError):
+        s.iterate(max_iter=1, verbose=False)
