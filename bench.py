@@ -1,0 +1,332 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain NFGS iterate on one MI355X.
+
+Workload: 256x256 observed multipole boundary, Nz=256 (calculation grid
+258x258x257), FP32, rho in {0, 0.5, 1}; the headline value is the wall time
+of set_bottom_vector_boundary + iterate() to the sigma_best stop at rho=0.5
+(max_iter 500, tol 1e-5, patience 40, sigma_improvement_tol 1e-5) with the
+boundary maps already in HBM.  A step is one plain iteration (sources, both
+spectral Poisson solves, reconstruction, face pass and the monitor
+readback); --steps K timed steps give ms_per_step.
+
+Also reported: per-kernel times from HIP events on the launch stream with
+the dominant kernel's roofline, the per-field error of the FP32 solve
+against an FP64 solve of the same boundary, and the CPU oracle timed on a
+bounded sample of the same workload.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "sec-to-converge at 256^3 (warm/batched vs plain), stencil HBM GB/s, err vs FP64"
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--n", type=int, default=256, help="observed lateral size and Nz")
+    p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--max-iter", type=int, default=500)
+    p.add_argument("--kernel-reps", type=int, default=20)
+    p.add_argument("--cpu-iters", type=int, default=2)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    p.add_argument("--no-fp64", action="store_true", help="skip the FP64 error reference")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="run a 256 MiB device copy for counter calibration")
+    return p.parse_args()
+
+
+def solver_kwargs(rho):
+    return dict(lateral_wall="paper_bc1", pin_potential_boundaries=False, buffer_thickness=0,
+                padded_buffer_zone=True, pad_calculation_domain=True,
+                calculation_domain_mode="mudpack", bottom_divergence_robin_weight=rho,
+                mean_bz_policy="uniform_open", horizontal_zero_mode_policy="frozen_linear_harmonic",
+                d0_zero_mode_policy="project", source_zero_mode_policy="project",
+                harmonic_residual_policy="diagnose", poloidal_update_mode="direct",
+                vertical_current_operator="staggered_face")
+
+
+ITER_KW = dict(init_mode="gs_cfcf", tol=1e-5, mu=1, patience=40, sigma_improvement_tol=1e-5,
+               stop_mode="sigma_best", verbose=False)
+
+
+def solve(torch, NFGSSolverGPU, planes, n, rho, dtype, max_iter):
+    """One timed solve with the boundary already resident on the device."""
+    bx, by, bz = (p.to(dtype) for p in planes)
+    s = NFGSSolverGPU(n, n, n, 1.0, 1.0, 1.0, device="cuda", dtype=dtype, **solver_kwargs(rho))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.set_bottom_vector_boundary(bx, by, bz, zero_net_current=False, smooth_passes=0, ijzbcor=3)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    h = s.iterate(max_iter=max_iter, **ITER_KW)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return s, h, t1 - t0, t2 - t0
+
+
+def time_kernels(torch, s, L, reps):
+    """Average device time per launch of each kernel class (HIP events on the
+    stream the kernels are enqueued on)."""
+    lib, plan, bnd, ws = s._lib, s._plan, L.C.byref(s._bnd), s._ws
+    st = s._stream
+    wt = s._ws_t
+    phi, az = s._slots[0]
+    Nz, Ny, Nx = s.Nz, s.Ny, s.Nx
+    pl, nxi, nyi = Ny * Nx, Nx - 2, Ny - 2
+    ip = nxi * nyi
+    es = phi.element_size()
+    part = L.ptr(s._partials)
+    calls = {
+        "face_pass": lambda: lib.nfgs_face_pass(plan, bnd, L.ptr(s._Bwork), L.ptr(az),
+                                                L.ptr(wt["jp"]), None, part, st),
+        "sources": lambda: lib.nfgs_sources(plan, L.ptr(wt["jp"]), L.ptr(wt["s_bcz"]),
+                                            L.ptr(wt["s_jz"]), part, st),
+        "reconstruct_b": lambda: lib.nfgs_reconstruct_b(plan, bnd, L.ptr(phi), L.ptr(az),
+                                                        L.ptr(s._Bwork), st),
+        "robin_sweep": lambda: lib.nfgs_robin_solve(plan, bnd, L.ptr(wt["hat1"]), L.ptr(wt["hat0"]),
+                                                    L.ptr(wt["cscr"]), part, st),
+        "dirichlet_sweep": lambda: lib.nfgs_dirichlet_solve(plan, bnd, L.ptr(wt["hat1"]),
+                                                            L.ptr(wt["hat0"]), L.ptr(wt["cscr"]),
+                                                            st),
+        "dct_x": lambda: lib.nfgs_transform(plan, L.NFGS_DCT1, L.NFGS_FORWARD, L.ptr(wt["s_bcz"]),
+                                            1, Nx, pl, L.ptr(wt["hat0"]), 1, Nx, pl, Nx, Ny, Nz, st),
+        "dct_y": lambda: lib.nfgs_transform(plan, L.NFGS_DCT1, L.NFGS_FORWARD, L.ptr(wt["hat0"]),
+                                            Nx, 1, pl, L.ptr(wt["hat1"]), Nx, 1, pl, Ny, Nx, Nz, st),
+        "dst_x": lambda: lib.nfgs_transform(plan, L.NFGS_DST1, L.NFGS_FORWARD,
+                                            L.ptr(wt["s_jz"]) + ip * es, 1, nxi, ip,
+                                            L.ptr(wt["hat0"]) + ip * es, 1, nxi, ip, nxi, nyi,
+                                            Nz - 2, st),
+        "dst_y": lambda: lib.nfgs_transform(plan, L.NFGS_DST1, L.NFGS_FORWARD,
+                                            L.ptr(wt["hat0"]) + ip * es, nxi, 1, ip,
+                                            L.ptr(wt["hat1"]) + ip * es, nxi, 1, ip, nyi, nxi,
+                                            Nz - 2, st),
+    }
+    out = {}
+    for name, fn in calls.items():
+        L.check(fn(), name)  # warm
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            L.check(fn(), name)
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = e0.elapsed_time(e1) / reps * 1e-3  # seconds per launch
+    return out
+
+
+def kernel_model(s, esize):
+    """Algorithmic bytes (or flops) per launch.  Units: one node of the
+    258x258x257 grid; words per node follow the fused-plan table of the
+    hot-path scope (FP words: 4 B at FP32)."""
+    Nz, Ny, Nx = s.Nz, s.Ny, s.Nx
+    N = Nz * Ny * Nx
+    NI = (Nz - 2) * (Ny - 2) * (Nx - 2)
+    return {
+        # words per node: reads + writes
+        "face_pass": dict(bytes=(3 + 3 + 1 + 3) * N * esize, words=10),
+        "sources": dict(bytes=(3 + 1) * N * esize + NI * esize, words=5),
+        "reconstruct_b": dict(bytes=(1 + 1 + 3 + 3) * N * esize, words=8),
+        "robin_sweep": dict(bytes=2 * N * esize, words=2),
+        "dirichlet_sweep": dict(bytes=2 * NI * esize, words=2),
+        "dct_x": dict(bytes=2 * N * esize, flops=2.0 * Nx * N),
+        "dct_y": dict(bytes=2 * N * esize, flops=2.0 * Ny * N),
+        "dst_x": dict(bytes=2 * NI * esize, flops=2.0 * (Nx - 2) * NI),
+        "dst_y": dict(bytes=2 * NI * esize, flops=2.0 * (Ny - 2) * NI),
+    }
+
+
+# launches of each kernel class in one iteration (forward and inverse passes
+# of each transform run the same kernel on the same shapes)
+PER_ITER = {"face_pass": 1, "sources": 1, "reconstruct_b": 1, "robin_sweep": 1,
+            "dirichlet_sweep": 1, "dct_x": 2, "dct_y": 2, "dst_x": 2, "dst_y": 2}
+
+
+def cpu_baseline(n, rho, planes_np, iters, gpu_iters):
+    """CPU oracle (NumPy, one thread of the host) on a bounded sample: setup,
+    gs_cfcf start and `iters` iterations at the full grid, extrapolated to
+    the GPU solve's iteration count."""
+    import numpy as np
+    from oracle.nfgs_oracle import OracleNFGS
+
+    bx, by, bz = planes_np
+    t0 = time.perf_counter()
+    o = OracleNFGS(n, n, n, 1.0, 1.0, 1.0, dtype=np.float32, bottom_divergence_robin_weight=rho,
+                   source_zero_mode_policy="project", d0_zero_mode_policy="project")
+    o.set_bottom_vector_boundary(bx, by, bz)
+    st = o.initial_state()
+    o.sigma(st[0], st[1])
+    t1 = time.perf_counter()
+    for _ in range(iters):
+        st = o.step(*st)
+        o.sigma(st[0], st[1]), o.divB(st[0]), o.energy(st[0])
+    t2 = time.perf_counter()
+    per_it = (t2 - t1) / max(iters, 1)
+    return dict(setup_init_s=t1 - t0, s_per_iter=per_it,
+                extrapolated_s=(t1 - t0) + per_it * gpu_iters)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus "
+              f"{a.gpus}", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from nfgs_amd import NFGSSolverGPU
+    from nfgs_amd import _lib as L
+    from nfgs_amd.boundary import multipole_boundary
+
+    n = a.n
+    bx, by, bz, alpha = multipole_boundary(n, seed=a.seed)
+    dev = torch.device("cuda")
+    planes = tuple(torch.tensor(p, dtype=torch.float64, device=dev) for p in (bx, by, bz))
+    torch.cuda.synchronize()
+    if a.calibrate_copy:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 256 MiB read + 256 MiB written")
+        del src, dst
+
+    # headline: plain method at rho 0, 0.5 and 1 (value = rho 0.5)
+    per_rho = {}
+    keep = None
+    for rho in (0.5, 0.0, 1.0):
+        log(f"solve fp32 rho={rho}")
+        s, h, setup_s, total_s = solve(torch, NFGSSolverGPU, planes, n, rho, torch.float32,
+                                       a.max_iter)
+        per_rho[str(rho)] = dict(sec_to_converge=total_s, setup_s=setup_s,
+                                 iterations=h["final_iter"], best_iter=h["best_iter"],
+                                 best_sigma=h["best_sigma"], stop_reason=h["stop_reason"])
+        log(f"  {total_s:.3f} s, {h['final_iter']} it, best {h['best_sigma']:.5e} "
+            f"({h['stop_reason']})")
+        if rho == 0.5:
+            keep = (s, h)
+        else:
+            del s
+    s, h = keep
+    head = per_rho["0.5"]
+
+    # K timed steps of the plain iteration (the loop body incl. monitor readback)
+    for _ in range(a.warmup):
+        s._advance(1, zero_source=False)
+        s._read_metrics()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        s._advance(1, zero_source=False)
+        s._read_metrics()
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) / max(a.steps, 1) * 1e3
+    log(f"step: {ms_per_step:.4f} ms")
+
+    # per-kernel device times and the dominant kernel's roofline
+    esize = 4
+    ktimes = time_kernels(torch, s, L, a.kernel_reps)
+    model = kernel_model(s, esize)
+    share = {k: ktimes[k] * PER_ITER[k] for k in ktimes}
+    kernels = {}
+    for k, t in ktimes.items():
+        m = model[k]
+        rec = dict(us=round(t * 1e6, 2), per_iter=PER_ITER[k],
+                   hbm_gbs=round(m["bytes"] / t / 1e9, 1))
+        if "flops" in m:
+            rec["tflops"] = round(m["flops"] / t / 1e12, 2)
+        kernels[k] = rec
+    dom = max(share, key=share.get)
+    stencils = [k for k in ktimes if "flops" not in model[k]]
+    dom_sten = max(stencils, key=lambda k: share[k])
+    def roof(k):
+        m, t = model[k], ktimes[k]
+        if "flops" in m:
+            ach = m["flops"] / t / 1e12
+            return dict(kernel=k, bound="mfma", achieved=round(ach, 2),
+                        peak=MFMA_PEAK_TFLOPS["f32"], unit="TFLOP/s",
+                        frac=round(ach / MFMA_PEAK_TFLOPS["f32"], 4), traffic=None,
+                        avg_launch_us=round(t * 1e6, 2))
+        ach = m["bytes"] / t / 1e9
+        return dict(kernel=k, bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                    avg_launch_us=round(t * 1e6, 2))
+    roofline = roof(dom)
+    stencil_roof = roof(dom_sten)
+    log(f"dominant kernel {dom}: {roofline}")
+
+    # error of the FP32 solve against an FP64 solve of the same boundary
+    err = None
+    if not a.no_fp64:
+        log("solve fp64 rho=0.5 (error reference)")
+        s64, h64, _, t64 = solve(torch, NFGSSolverGPU, planes, n, 0.5, torch.float64, a.max_iter)
+        f32 = s.fields_to_numpy()
+        f64 = s64.fields_to_numpy()
+        err = {k: float(np.linalg.norm(f32[k].astype(np.float64) - f64[k]) /
+                        max(np.linalg.norm(f64[k]), 1e-300)) for k in f32}
+        err["best_sigma_rel"] = abs(h["best_sigma"] - h64["best_sigma"]) / h64["best_sigma"]
+        err["fp64_sec_to_converge"] = t64
+        err["fp64_best_iter"] = h64["best_iter"]
+        del s64
+
+    cpu = None
+    if not a.no_cpu:
+        log(f"CPU oracle sample: setup + init + {a.cpu_iters} iterations")
+        cb = cpu_baseline(n, 0.5, (bx, by, bz), a.cpu_iters, head["iterations"])
+        cpu = dict(value=round(cb["extrapolated_s"], 2), unit="s", cores=1, kind="port",
+                   sample=f"NumPy oracle, FP32, 258x258x257, rho=0.5: setup+gs_cfcf init "
+                          f"{cb['setup_init_s']:.2f} s measured, {a.cpu_iters} iterations at "
+                          f"{cb['s_per_iter']:.2f} s each, extrapolated to the GPU solve's "
+                          f"{head['iterations']} iterations",
+                   s_per_iter=round(cb["s_per_iter"], 3))
+
+    line = {
+        "metric": METRIC,
+        "value": round(head["sec_to_converge"], 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain NFGS iterate, 256x256 multipole boundary, Nz=256 "
+                               "(calculation grid 258x258x257), rho=0.5",
+                   "grid_zyx": [s.Nz, s.Ny, s.Nx], "seed": a.seed, "alpha": round(alpha, 5),
+                   "iterations": head["iterations"], "best_iter": head["best_iter"],
+                   "best_sigma": head["best_sigma"], "stop_reason": head["stop_reason"],
+                   "setup_s": round(head["setup_s"], 4), "per_rho": per_rho,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "stencil_roofline": stencil_roof,
+        "kernels": kernels,
+        "err_vs_fp64": err,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

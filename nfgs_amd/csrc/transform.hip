@@ -24,12 +24,16 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct Mfma;
+// Both take A[i][k] from lane i + 16k and B[k][j] from lane j + 16k; the
+// output row held in accumulator register r of lane l differs:
+// f32 gives D[4*(l/16) + r][l%16], f64 gives D[l/16 + 4r][l%16].
 template <>
 struct Mfma<float> {
   using acc = f32x4;
   static __device__ __forceinline__ acc mma(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int l4, int r) { return 4 * l4 + r; }
 };
 template <>
 struct Mfma<double> {
@@ -37,6 +41,7 @@ struct Mfma<double> {
   static __device__ __forceinline__ acc mma(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int l4, int r) { return l4 + 4 * r; }
 };
 
 constexpr int kVec = 64;    // vectors per workgroup
@@ -168,10 +173,10 @@ __global__ __launch_bounds__(256) void paired_transform_kernel(
       for (int r = 0; r < 4; ++r) {
         int k, v;
         if (ELEM_CONTIG) {
-          v = wave * 16 + 4 * l4 + r;
+          v = wave * 16 + Mfma<T>::row(l4, r);
           k = (b0 + b) * 16 + l16;
         } else {
-          k = (b0 + b) * 16 + 4 * l4 + r;
+          k = (b0 + b) * 16 + Mfma<T>::row(l4, r);
           v = wave * 16 + l16;
         }
         const long g = g0 + v;
