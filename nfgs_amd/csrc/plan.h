@@ -35,6 +35,7 @@ struct nfgs_plan {
 namespace nfgs {
 const PairedMatrix* find_matrix(const nfgs_plan* p, int kind, int n);
 int fill_zero(const nfgs_plan* p, void* ptr, long n, hipStream_t s);
+long tiled_blocks(const nfgs_plan* p);
 int fill_zero_f64(double* ptr, long n, hipStream_t s);
 int launch_transform(const nfgs_plan* p, const PairedMatrix* m, int direction, const void* in,
                      long ies, long ivs, long ips, void* out, long oes, long ovs, long ops,

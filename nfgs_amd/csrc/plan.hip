@@ -131,7 +131,7 @@ extern "C" int nfgs_plan_create(const nfgs_grid_desc* d, nfgs_plan** out) {
   p->iplane = (long)(d->ny - 2) * (d->nx - 2);
   p->N = p->plane * d->nz;
   p->esize = d->dtype == NFGS_F32 ? 4 : 8;
-  p->pblocks = 1024;
+  p->pblocks = (int)tiled_blocks(p);
 
   int rc = build_paired(d->dtype, NFGS_DCT1, d->nx, &p->dct_x);
   if (!rc) rc = build_paired(d->dtype, NFGS_DCT1, d->ny, &p->dct_y);

@@ -35,106 +35,188 @@ Grid<T> make_grid(const nfgs_plan* p) {
 }
 
 // ---------------------------------------------------------------------------
-// Reconstruction of B from the scalars (nfgs_solver.py:1711-1715, :3311-3341,
-// :5901-5916, :6069-6078).
+// z-marching tiles.  A block owns a TX x TY column tile and a chunk of z
+// planes; the planes a stencil needs are staged in LDS with a 2-node halo
+// (enough for the one-sided end stencils), and the block walks up in z so
+// every array is read once per plane.  Points outside the domain hold 0 and
+// are never used by a valid stencil.
 // ---------------------------------------------------------------------------
+constexpr int TX = 32, TY = 8, HH = 2;
+constexpr int PW = TX + 2 * HH, PH = TY + 2 * HH, PS = PW * PH;
+constexpr int KCHUNK = 32;
+
+struct Tile {
+  int x0, y0, tx, ty, x, y;
+  bool valid;
+  int c;  // LDS index of this thread's point
+};
+
 template <typename T>
-__device__ __forceinline__ T dz_at(const T* f, const Grid<T>& g, int k, long off) {
-  return fd1(f + (long)k * g.plane + off, g.plane, k, g.nz, g.dz);
+__device__ __forceinline__ Tile make_tile(const Grid<T>& g) {
+  Tile t;
+  t.x0 = blockIdx.x * TX;
+  t.y0 = blockIdx.y * TY;
+  t.tx = threadIdx.x % TX;
+  t.ty = threadIdx.x / TX;
+  t.x = t.x0 + t.tx;
+  t.y = t.y0 + t.ty;
+  t.valid = t.x < g.nx && t.y < g.ny;
+  t.c = (t.ty + HH) * PW + t.tx + HH;
+  return t;
 }
 
+// Two-step staging of a plane region (tile + halo): fetch() issues the
+// global loads into registers, commit() writes them to an LDS slot later, so
+// the loads of plane p+1 are in flight while plane p is being computed.
+constexpr int PPT = (PS + 255) / 256;  // staged elements per thread (256 threads)
+
+template <typename T>
+struct Fetch {
+  T v[PPT];
+};
+
+template <typename T>
+__device__ __forceinline__ void fetch(Fetch<T>& r, const T* plane, const Tile& t,
+                                      const Grid<T>& g) {
+#pragma unroll
+  for (int i = 0; i < PPT; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    T v = T(0);
+    if (e < PS) {
+      const int ly = e / PW, lx = e - ly * PW;
+      const int y = t.y0 + ly - HH, x = t.x0 + lx - HH;
+      if (y >= 0 && y < g.ny && x >= 0 && x < g.nx) v = plane[(long)y * g.nx + x];
+    }
+    r.v[i] = v;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void commit(T* slot, const Fetch<T>& r) {
+#pragma unroll
+  for (int i = 0; i < PPT; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    if (e < PS) slot[e] = r.v[i];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void commit_diff(T* slot, const Fetch<T>& a, const Fetch<T>& b) {
+#pragma unroll
+  for (int i = 0; i < PPT; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    if (e < PS) slot[e] = a.v[i] - b.v[i];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T lds_dx(const T* s, const Tile& t, const Grid<T>& g) {
+  return fd1(s + t.c, 1, t.x, g.nx, g.dx);
+}
+template <typename T>
+__device__ __forceinline__ T lds_dy(const T* s, const Tile& t, const Grid<T>& g) {
+  return fd1(s + t.c, (long)PW, t.y, g.ny, g.dy);
+}
+template <typename T>
+__device__ __forceinline__ T lds_lap(const T* s, const Tile& t, const Grid<T>& g) {
+  return lap_xy_at(s + t.c - ((long)t.y * PW + t.x), t.y, t.x, g.ny, g.nx, (long)PW, g.idx2,
+                   g.idy2);
+}
+
+// Reconstruction of B from the scalars (nfgs_solver.py:1711-1715, :3311-3341,
+// :5901-5916, :6069-6078).
 template <typename T>
 __global__ __launch_bounds__(256) void reconstruct_b_kernel(
     Grid<T> g, const T* __restrict__ phi, const T* __restrict__ az, const T* __restrict__ bg,
     const T* __restrict__ bz_obs, const T* __restrict__ hprof, T* __restrict__ b) {
-  const long stride = (long)gridDim.x * blockDim.x;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < g.N; i += stride) {
-    const int k = (int)(i / g.plane);
-    const long off = i - (long)k * g.plane;
-    const int y = (int)(off / g.nx), x = (int)(off - (long)y * g.nx);
-    const T* azk = az + (long)k * g.plane;
-    // d/dx and d/dy of dPhi/dz, each built from the z-derivative at neighbours
-    auto dzx = [&](int o) { return dz_at(phi, g, k, off + o); };
-    auto dzy = [&](int o) { return dz_at(phi, g, k, off + (long)o * g.nx); };
-    const T ddx = fd1g<T>(dzx, x, g.nx, g.dx);
-    const T ddy = fd1g<T>(dzy, y, g.ny, g.dy);
-    const T azx = fd1(azk + off, 1, x, g.nx, g.dx);
-    const T azy = fd1(azk + off, g.nx, y, g.ny, g.dy);
-    const T bcx = -ddx + azy;
-    const T bcy = -ddy - azx;
-    const T bcz = lap_xy_at(phi + (long)k * g.plane, y, x, g.ny, g.nx, (long)g.nx, g.idx2, g.idy2);
-    T bx = bg[i] + bcx;
-    T by = bg[g.N + i] + bcy;
-    T bz = bg[2 * g.N + i] + bcz;
-    if (k == 0) bz = bz_obs[off];
-    bx = bx + hprof[k];
-    by = by + hprof[g.nz + k];
-    b[i] = bx;
-    b[g.N + i] = by;
-    b[2 * g.N + i] = bz;
+  __shared__ T sphi[3][PS];
+  __shared__ T saz[PS];
+  __shared__ T sdz[PS];
+  const Tile t = make_tile(g);
+  const int k0 = blockIdx.z * KCHUNK, k1 = min(k0 + KCHUNK, g.nz);
+  // Phi planes lo..lo+2 are resident while node k is built
+  auto lo_of = [&](int k) { return max(0, min(k - 1, g.nz - 3)); };
+  const long own = t.valid ? (long)t.y * g.nx + t.x : 0;
+  int lo = lo_of(k0);
+  {
+    Fetch<T> f0, f1, f2;
+    fetch(f0, phi + (long)lo * g.plane, t, g);
+    fetch(f1, phi + (long)(lo + 1) * g.plane, t, g);
+    fetch(f2, phi + (long)(lo + 2) * g.plane, t, g);
+    commit(sphi[lo % 3], f0);
+    commit(sphi[(lo + 1) % 3], f1);
+    commit(sphi[(lo + 2) % 3], f2);
+  }
+  Fetch<T> fphi, faz;
+  fetch(faz, az + (long)k0 * g.plane, t, g);
+  bool have_phi = false;
+  T gx = 0, gy = 0, gz = 0;
+  if (t.valid) {
+    const long i = (long)k0 * g.plane + own;
+    gx = bg[i];
+    gy = bg[g.N + i];
+    gz = bg[2 * g.N + i];
+  }
+  for (int k = k0; k < k1; ++k) {
+    if (have_phi) {
+      commit(sphi[(lo + 2) % 3], fphi);
+      have_phi = false;
+    }
+    commit(saz, faz);
+    const T cgx = gx, cgy = gy, cgz = gz;
+    __syncthreads();
+    // prefetch what node k+1 needs
+    if (k + 1 < k1) {
+      const int nlo = lo_of(k + 1);
+      if (nlo > lo) {
+        fetch(fphi, phi + (long)(nlo + 2) * g.plane, t, g);
+        have_phi = true;
+        lo = nlo;
+      }
+      fetch(faz, az + (long)(k + 1) * g.plane, t, g);
+      if (t.valid) {
+        const long i = (long)(k + 1) * g.plane + own;
+        gx = bg[i];
+        gy = bg[g.N + i];
+        gz = bg[2 * g.N + i];
+      }
+    }
+    // dPhi/dz on the tile with halo
+    for (int e = threadIdx.x; e < PS; e += blockDim.x) {
+      T d;
+      if (k == 0)
+        d = (T(-3) * sphi[0][e] + T(4) * sphi[1][e] - sphi[2][e]) / (T(2) * g.dz);
+      else if (k == g.nz - 1)
+        d = (T(3) * sphi[k % 3][e] - T(4) * sphi[(k - 1) % 3][e] + sphi[(k - 2) % 3][e]) /
+            (T(2) * g.dz);
+      else
+        d = (sphi[(k + 1) % 3][e] - sphi[(k - 1) % 3][e]) / (T(2) * g.dz);
+      sdz[e] = d;
+    }
+    __syncthreads();
+    if (t.valid) {
+      const long i = (long)k * g.plane + own;
+      const T bcx = -lds_dx(sdz, t, g) + lds_dy(saz, t, g);
+      const T bcy = -lds_dy(sdz, t, g) - lds_dx(saz, t, g);
+      const T bcz = lds_lap(sphi[k % 3], t, g);
+      T bx = cgx + bcx;
+      T by = cgy + bcy;
+      T bz = cgz + bcz;
+      if (k == 0) bz = bz_obs[own];
+      b[i] = bx + hprof[k];
+      b[g.N + i] = by + hprof[g.nz + k];
+      b[2 * g.N + i] = bz;
+    }
+    __syncthreads();
   }
 }
 
-// ---------------------------------------------------------------------------
-// Vertical-face state (nfgs_solver.py:5738-5778) and everything built on it.
-// ---------------------------------------------------------------------------
+// Vertical-face pass (nfgs_solver.py:5738-5839, :5882-5895, :2689-2697,
+// :5917-5941, :6702-6710).
 template <typename T>
-__device__ __forceinline__ T jz_at(const T* az, const T* jzt, const Grid<T>& g, int k, int y,
-                                   int x) {
-  if (k == 0) return jzt[(long)y * g.nx + x];
-  if (k == g.nz - 1 || y == 0 || y == g.ny - 1 || x == 0 || x == g.nx - 1) return T(0);
-  return -lap_xy_at(az + (long)k * g.plane, y, x, g.ny, g.nx, (long)g.nx, g.idx2, g.idy2);
-}
-
-template <typename T>
-struct Face {
-  T b[3];
-  T j[3];
+struct FaceVal {
+  T b[3], j[3], p[3];  // face B, face current, alpha-projected current
 };
-
-template <typename T>
-__device__ __forceinline__ Face<T> face_at(const T* B, const T* bg, const T* az, const T* jzt,
-                                           const Grid<T>& g, int f, int y, int x) {
-  Face<T> F;
-  const long off = (long)y * g.nx + x;
-  const long i0 = (long)f * g.plane + off, i1 = i0 + g.plane;
-  const T bx0 = B[i0], bx1 = B[i1];
-  const T by0 = B[g.N + i0], by1 = B[g.N + i1];
-  const T bz0 = B[2 * g.N + i0], bz1 = B[2 * g.N + i1];
-  F.b[0] = T(0.5) * (bx1 + bx0);
-  F.b[1] = T(0.5) * (by1 + by0);
-  F.b[2] = T(0.5) * (bz1 + bz0);
-  const T cx0 = bx0 - bg[i0], cx1 = bx1 - bg[i1];
-  const T cy0 = by0 - bg[g.N + i0], cy1 = by1 - bg[g.N + i1];
-  const T dbx = (cx1 - cx0) / g.dz;
-  const T dby = (cy1 - cy0) / g.dz;
-  const T* Bz = B + 2 * g.N;
-  const T* Gz = bg + 2 * g.N;
-  // carrier z on the face at lateral offset o (elements)
-  auto cfz = [&](long o) {
-    const T c1 = Bz[i1 + o] - Gz[i1 + o];
-    const T c0 = Bz[i0 + o] - Gz[i0 + o];
-    return T(0.5) * (c1 + c0);
-  };
-  const T dyc = fd1g<T>([&](int o) { return cfz((long)o * g.nx); }, y, g.ny, g.dy);
-  const T dxc = fd1g<T>([&](int o) { return cfz((long)o); }, x, g.nx, g.dx);
-  F.j[0] = dyc - dby;
-  F.j[1] = dbx - dxc;
-  F.j[2] = T(0.5) * (jz_at(az, jzt, g, f + 1, y, x) + jz_at(az, jzt, g, f, y, x));
-  return F;
-}
-
-template <typename T>
-__device__ __forceinline__ void par_current(const Face<T>& F, T floor_b2, T amax, T (&jp)[3]) {
-  T b2 = F.b[0] * F.b[0] + F.b[1] * F.b[1] + F.b[2] * F.b[2];
-  b2 = b2 < floor_b2 ? floor_b2 : b2;
-  const T jb = F.j[0] * F.b[0] + F.j[1] * F.b[1] + F.j[2] * F.b[2];
-  T a = jb / b2;
-  a = a < -amax ? -amax : (a > amax ? amax : a);
-  jp[0] = a * F.b[0];
-  jp[1] = a * F.b[1];
-  jp[2] = a * F.b[2];
-}
 
 struct Window {
   int z0, z1, y0, y1, x0, x1;
@@ -145,91 +227,166 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
     Grid<T> g, const T* __restrict__ B, const T* __restrict__ bg, const T* __restrict__ az,
     const T* __restrict__ jzt, T* __restrict__ jp, T* __restrict__ jout, double* partials,
     long pstride, T floor_b2, T amax, Window w) {
+  __shared__ T scz[2][PS];
+  __shared__ T sbx[2][PS];
+  __shared__ T sby[2][PS];
+  __shared__ T saz[PS];
   __shared__ double red[NFGS_NQ * 16];
+  const Tile t = make_tile(g);
+  const int k0 = blockIdx.z * KCHUNK, k1 = min(k0 + KCHUNK, g.nz);
+  const int ps = max(k0 - 1, 0), pe = min(k1, g.nz - 1);
+  const long off = t.valid ? (long)t.y * g.nx + t.x : 0;
+  const bool edge = t.y == 0 || t.y == g.ny - 1 || t.x == 0 || t.x == g.nx - 1;
+  const bool in_win = t.y >= w.y0 && t.y < w.y1 && t.x >= w.x0 && t.x < w.x1;
+  const T* Bx = B;
+  const T* By = B + g.N;
+  const T* Bz = B + 2 * g.N;
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  const bool want_j = jout != nullptr;
-  const long stride = (long)gridDim.x * blockDim.x;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < g.N; i += stride) {
-    const int k = (int)(i / g.plane);
-    const long off = i - (long)k * g.plane;
-    const int y = (int)(off / g.nx), x = (int)(off - (long)y * g.nx);
-    const bool edge = (y == 0 || y == g.ny - 1 || x == 0 || x == g.nx - 1);
-    Face<T> lo, hi;
-    const bool has_lo = k >= 1, has_hi = k <= g.nz - 2;
-    if (has_lo) lo = face_at(B, bg, az, jzt, g, k - 1, y, x);
-    if (has_hi) hi = face_at(B, bg, az, jzt, g, k, y, x);
-    if (jp) {
-      T out[3];
-      if (k == 0) {
-        // bottom row of J_par is the state current J[:, 0] (nfgs_solver.py:6702-6710)
-        out[0] = edge ? T(0) : hi.j[0];
-        out[1] = edge ? T(0) : hi.j[1];
-        out[2] = jzt[off];
-      } else {
-        T pl[3];
-        par_current(lo, floor_b2, amax, pl);
-        if (has_hi) {
-          T ph[3];
-          par_current(hi, floor_b2, amax, ph);
-          out[0] = T(0.5) * (pl[0] + ph[0]);
-          out[1] = T(0.5) * (pl[1] + ph[1]);
-          out[2] = T(0.5) * (pl[2] + ph[2]);
-        } else {
-          out[0] = pl[0];
-          out[1] = pl[1];
-          out[2] = pl[2];
+  T pb[3] = {0, 0, 0}, pg[2] = {0, 0}, pjz = 0;  // previous plane, own point
+  FaceVal<T> fprev;
+  Fetch<T> fbz, fgz, fbx, fby, faz;
+  T obz = 0, ogx = 0, ogy = 0;  // own Bz, bg_x, bg_y of the fetched plane
+  auto fetch_plane = [&](int q) {
+    const long qo = (long)q * g.plane;
+    fetch(fbz, Bz + qo, t, g);
+    fetch(fgz, bg + 2 * g.N + qo, t, g);
+    fetch(fbx, Bx + qo, t, g);
+    fetch(fby, By + qo, t, g);
+    fetch(faz, az + qo, t, g);
+    if (t.valid) {
+      obz = Bz[qo + off];
+      ogx = bg[qo + off];
+      ogy = bg[g.N + qo + off];
+    }
+  };
+  fetch_plane(ps);
+  for (int p = ps; p <= pe; ++p) {
+    commit_diff(scz[p & 1], fbz, fgz);
+    commit(sbx[p & 1], fbx);
+    commit(sby[p & 1], fby);
+    commit(saz, faz);
+    const T cbz = obz, cgx = ogx, cgy = ogy;
+    __syncthreads();
+    if (p + 1 <= pe) fetch_plane(p + 1);
+    T cb[3] = {0, 0, 0}, cg[2] = {0, 0}, cjz = 0;
+    if (t.valid) {
+      cb[0] = sbx[p & 1][t.c];
+      cb[1] = sby[p & 1][t.c];
+      cb[2] = cbz;
+      cg[0] = cgx;
+      cg[1] = cgy;
+      if (p == 0)
+        cjz = jzt[off];
+      else if (p == g.nz - 1 || edge)
+        cjz = T(0);
+      else
+        cjz = -lds_lap(saz, t, g);
+    }
+    if (p > ps && t.valid) {
+      const int f = p - 1;  // face between planes f and p
+      FaceVal<T> F;
+      F.b[0] = T(0.5) * (cb[0] + pb[0]);
+      F.b[1] = T(0.5) * (cb[1] + pb[1]);
+      F.b[2] = T(0.5) * (cb[2] + pb[2]);
+      const T dbx = ((cb[0] - cg[0]) - (pb[0] - pg[0])) / g.dz;
+      const T dby = ((cb[1] - cg[1]) - (pb[1] - pg[1])) / g.dz;
+      const T* s1 = scz[p & 1];
+      const T* s0 = scz[f & 1];
+      auto cfz = [&](int o) { return T(0.5) * (s1[t.c + o] + s0[t.c + o]); };
+      const T dyc = fd1g<T>([&](int o) { return cfz(o * PW); }, t.y, g.ny, g.dy);
+      const T dxc = fd1g<T>([&](int o) { return cfz(o); }, t.x, g.nx, g.dx);
+      F.j[0] = dyc - dby;
+      F.j[1] = dbx - dxc;
+      F.j[2] = T(0.5) * (cjz + pjz);
+      {
+        T b2 = F.b[0] * F.b[0] + F.b[1] * F.b[1] + F.b[2] * F.b[2];
+        b2 = b2 < floor_b2 ? floor_b2 : b2;
+        const T jb = F.j[0] * F.b[0] + F.j[1] * F.b[1] + F.j[2] * F.b[2];
+        T a = jb / b2;
+        a = a < -amax ? -amax : (a > amax ? amax : a);
+        F.p[0] = a * F.b[0];
+        F.p[1] = a * F.b[1];
+        F.p[2] = a * F.b[2];
+      }
+      const int k = f;  // node finalised now
+      if (k >= k0 && k < k1) {
+        const long i = (long)k * g.plane + off;
+        if (jp) {
+          if (k == 0) {
+            jp[i] = edge ? T(0) : F.j[0];
+            jp[g.N + i] = edge ? T(0) : F.j[1];
+            jp[2 * g.N + i] = jzt[off];
+          } else {
+            jp[i] = T(0.5) * (fprev.p[0] + F.p[0]);
+            jp[g.N + i] = T(0.5) * (fprev.p[1] + F.p[1]);
+            jp[2 * g.N + i] = T(0.5) * (fprev.p[2] + F.p[2]);
+          }
+        }
+        if (jout) {
+          T c[3];
+          if (k == 0) {
+            c[0] = F.j[0]; c[1] = F.j[1]; c[2] = F.j[2];
+          } else {
+            c[0] = T(0.5) * (fprev.j[0] + F.j[0]);
+            c[1] = T(0.5) * (fprev.j[1] + F.j[1]);
+            c[2] = T(0.5) * (fprev.j[2] + F.j[2]);
+          }
+          acc[6] += (double)c[0] * c[0] + (double)c[1] * c[1] + (double)c[2] * c[2];
+          acc[5] += (double)F.j[0] * F.j[0] + (double)F.j[1] * F.j[1] + (double)F.j[2] * F.j[2];
+          jout[i] = edge ? T(0) : c[0];
+          jout[g.N + i] = edge ? T(0) : c[1];
+          jout[2 * g.N + i] = pjz;
+        }
+        if (partials) {
+          if (k >= w.z0 && k < w.z1 && in_win) {
+            T bn = sqrt(F.b[0] * F.b[0] + F.b[1] * F.b[1] + F.b[2] * F.b[2]);
+            bn = bn < T(1e-10) ? T(1e-10) : bn;
+            const T jn = sqrt(F.j[0] * F.j[0] + F.j[1] * F.j[1] + F.j[2] * F.j[2]);
+            const T c0 = F.j[1] * F.b[2] - F.j[2] * F.b[1];
+            const T c1 = F.j[2] * F.b[0] - F.j[0] * F.b[2];
+            const T c2 = F.j[0] * F.b[1] - F.j[1] * F.b[0];
+            acc[0] += (double)(sqrt(c0 * c0 + c1 * c1 + c2 * c2) / bn);
+            acc[1] += (double)jn;
+          }
+          const T dvx = lds_dx(sbx[k & 1], t, g);
+          const T dvy = lds_dy(sby[k & 1], t, g);
+          const T dvz = fd1(Bz + i, g.plane, k, g.nz, g.dz);
+          const T div = dvx + dvy + dvz;
+          const T b2 = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
+          acc[2] += (double)(div < T(0) ? -div : div);
+          acc[3] += (double)sqrt(b2);
+          acc[4] += (double)b2;
         }
       }
-      jp[i] = out[0];
-      jp[g.N + i] = out[1];
-      jp[2 * g.N + i] = out[2];
+      fprev = F;
     }
-    if (want_j) {
-      // staggered centre current with the NFGS boundary values (:5917-5941)
-      T c[3];
-      if (k == 0) {
-        c[0] = hi.j[0]; c[1] = hi.j[1]; c[2] = hi.j[2];
-      } else if (!has_hi) {
-        c[0] = lo.j[0]; c[1] = lo.j[1]; c[2] = lo.j[2];
-      } else {
-        c[0] = T(0.5) * (lo.j[0] + hi.j[0]);
-        c[1] = T(0.5) * (lo.j[1] + hi.j[1]);
-        c[2] = T(0.5) * (lo.j[2] + hi.j[2]);
-      }
-      acc[6] += (double)c[0] * c[0] + (double)c[1] * c[1] + (double)c[2] * c[2];
-      if (has_hi)
-        acc[5] += (double)hi.j[0] * hi.j[0] + (double)hi.j[1] * hi.j[1] +
-                  (double)hi.j[2] * hi.j[2];
-      const bool zero_xy = edge || k == g.nz - 1;
-      jout[i] = zero_xy ? T(0) : c[0];
-      jout[g.N + i] = zero_xy ? T(0) : c[1];
-      jout[2 * g.N + i] = jz_at(az, jzt, g, k, y, x);
+    pb[0] = cb[0]; pb[1] = cb[1]; pb[2] = cb[2];
+    pg[0] = cg[0]; pg[1] = cg[1];
+    pjz = cjz;
+    __syncthreads();
+  }
+  // top node: no face above it
+  const int kt = g.nz - 1;
+  if (k1 == g.nz && t.valid) {
+    const long i = (long)kt * g.plane + off;
+    if (jp) {
+      jp[i] = fprev.p[0];
+      jp[g.N + i] = fprev.p[1];
+      jp[2 * g.N + i] = fprev.p[2];
+    }
+    if (jout) {
+      acc[6] += (double)fprev.j[0] * fprev.j[0] + (double)fprev.j[1] * fprev.j[1] +
+                (double)fprev.j[2] * fprev.j[2];
+      jout[i] = T(0);
+      jout[g.N + i] = T(0);
+      jout[2 * g.N + i] = T(0);
     }
     if (partials) {
-      // sigma on the face window (:5882-5895)
-      if (has_hi && k >= w.z0 && k < w.z1 && y >= w.y0 && y < w.y1 && x >= w.x0 && x < w.x1) {
-        const T* jb = hi.j;
-        const T* bb = hi.b;
-        T bn = sqrt(bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2]);
-        bn = bn < T(1e-10) ? T(1e-10) : bn;
-        const T jn = sqrt(jb[0] * jb[0] + jb[1] * jb[1] + jb[2] * jb[2]);
-        const T c0 = jb[1] * bb[2] - jb[2] * bb[1];
-        const T c1 = jb[2] * bb[0] - jb[0] * bb[2];
-        const T c2 = jb[0] * bb[1] - jb[1] * bb[0];
-        const T cn = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
-        acc[0] += (double)(cn / bn);
-        acc[1] += (double)jn;
-      }
-      // divB, |B| and |B|^2 on every node (:2689-2697)
-      const T* Bx = B;
-      const T* By = B + g.N;
-      const T* Bz = B + 2 * g.N;
-      const T dvx = fd1(Bx + i, 1, x, g.nx, g.dx);
-      const T dvy = fd1(By + i, (long)g.nx, y, g.ny, g.dy);
-      const T dvz = fd1(Bz + i, g.plane, k, g.nz, g.dz);
+      const T dvx = lds_dx(sbx[kt & 1], t, g);
+      const T dvy = lds_dy(sby[kt & 1], t, g);
+      const T dvz = fd1(Bz + i, g.plane, kt, g.nz, g.dz);
       const T div = dvx + dvy + dvz;
-      const T bx = Bx[i], by = By[i], bz = Bz[i];
-      const T b2 = bx * bx + by * by + bz * bz;
+      const T b2 = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
       acc[2] += (double)(div < T(0) ? -div : div);
       acc[3] += (double)sqrt(b2);
       acc[4] += (double)b2;
@@ -238,56 +395,89 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
   if (!partials) return;
   block_sum<7>(acc, red);
   if (threadIdx.x == 0) {
-    const long b = blockIdx.x;
-    partials[NFGS_Q_SIGMA_NUM * pstride + b] = acc[0];
-    partials[NFGS_Q_SIGMA_DEN * pstride + b] = acc[1];
-    partials[NFGS_Q_DIV_ABS * pstride + b] = acc[2];
-    partials[NFGS_Q_BMAG * pstride + b] = acc[3];
-    partials[NFGS_Q_B2 * pstride + b] = acc[4];
-    partials[NFGS_Q_FACE_J2 * pstride + b] = acc[5];
-    partials[NFGS_Q_CELL_J2 * pstride + b] = acc[6];
+    const long bid = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    partials[NFGS_Q_SIGMA_NUM * pstride + bid] = acc[0];
+    partials[NFGS_Q_SIGMA_DEN * pstride + bid] = acc[1];
+    partials[NFGS_Q_DIV_ABS * pstride + bid] = acc[2];
+    partials[NFGS_Q_BMAG * pstride + bid] = acc[3];
+    partials[NFGS_Q_B2 * pstride + bid] = acc[4];
+    partials[NFGS_Q_FACE_J2 * pstride + bid] = acc[5];
+    partials[NFGS_Q_CELL_J2 * pstride + bid] = acc[6];
   }
 }
 
-// ---------------------------------------------------------------------------
 // Sources of the two Poisson problems (nfgs_solver.py:6726-6745, :1729-1749).
-// ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void sources_kernel(Grid<T> g, const T* __restrict__ jp,
                                                       T* __restrict__ sb, T* __restrict__ sj,
                                                       double* partials, long pstride, T i4dxdz,
                                                       T i4dydz) {
+  __shared__ T sjx[3][PS];
+  __shared__ T sjy[3][PS];
+  __shared__ T sjz[PS];
   __shared__ double red[NFGS_NQ * 16];
-  double acc[1] = {0.0};
+  const Tile t = make_tile(g);
+  const int k0 = blockIdx.z * KCHUNK, k1 = min(k0 + KCHUNK, g.nz);
   const T* jx = jp;
   const T* jy = jp + g.N;
   const T* jzp = jp + 2 * g.N;
-  const long stride = (long)gridDim.x * blockDim.x;
   const int nxi = g.nx - 2;
   const long ip = (long)(g.ny - 2) * nxi;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < g.N; i += stride) {
-    const int k = (int)(i / g.plane);
-    const long off = i - (long)k * g.plane;
-    const int y = (int)(off / g.nx), x = (int)(off - (long)y * g.nx);
-    const T s = fd1(jx + i, (long)g.nx, y, g.ny, g.dy) - fd1(jy + i, 1, x, g.nx, g.dx);
-    sb[i] = s;
-    acc[0] += (double)s * s;
-    if (y >= 1 && y <= g.ny - 2 && x >= 1 && x <= g.nx - 2) {
-      T v = lap_xy_at(jzp + (long)k * g.plane, y, x, g.ny, g.nx, (long)g.nx, g.idx2, g.idy2);
-      if (k >= 1 && k <= g.nz - 2) {
-        const long up = g.plane, nx = g.nx;
-        const T* fx = jx + i;
-        const T dxz = (fx[up + 1] - fx[-up + 1] - fx[up - 1] + fx[-up - 1]) * i4dxdz;
-        const T* fy = jy + i;
-        const T dyz = (fy[up + nx] - fy[-up + nx] - fy[up - nx] + fy[-up - nx]) * i4dydz;
-        v = v - dxz - dyz;
-      }
-      sj[(long)k * ip + (long)(y - 1) * nxi + (x - 1)] = v;
+  const bool interior = t.y >= 1 && t.y <= g.ny - 2 && t.x >= 1 && t.x <= g.nx - 2;
+  double acc[1] = {0.0};
+  Fetch<T> fx, fy, fz;
+  for (int q = max(k0 - 1, 0); q <= k0; ++q) {
+    fetch(fx, jx + (long)q * g.plane, t, g);
+    fetch(fy, jy + (long)q * g.plane, t, g);
+    commit(sjx[q % 3], fx);
+    commit(sjy[q % 3], fy);
+  }
+  if (k0 + 1 <= g.nz - 1) {
+    fetch(fx, jx + (long)(k0 + 1) * g.plane, t, g);
+    fetch(fy, jy + (long)(k0 + 1) * g.plane, t, g);
+  }
+  fetch(fz, jzp + (long)k0 * g.plane, t, g);
+  for (int k = k0; k < k1; ++k) {
+    if (k + 1 <= g.nz - 1) {
+      commit(sjx[(k + 1) % 3], fx);
+      commit(sjy[(k + 1) % 3], fy);
     }
+    commit(sjz, fz);
+    __syncthreads();
+    if (k + 1 < k1) {
+      if (k + 2 <= g.nz - 1) {
+        fetch(fx, jx + (long)(k + 2) * g.plane, t, g);
+        fetch(fy, jy + (long)(k + 2) * g.plane, t, g);
+      }
+      fetch(fz, jzp + (long)(k + 1) * g.plane, t, g);
+    }
+    if (t.valid) {
+      const long i = (long)k * g.plane + (long)t.y * g.nx + t.x;
+      const T s = lds_dy(sjx[k % 3], t, g) - lds_dx(sjy[k % 3], t, g);
+      sb[i] = s;
+      acc[0] += (double)s * s;
+      if (interior) {
+        T v = lds_lap(sjz, t, g);
+        if (k >= 1 && k <= g.nz - 2) {
+          const T* xu = sjx[(k + 1) % 3] + t.c;
+          const T* xd = sjx[(k - 1) % 3] + t.c;
+          const T dxz = (xu[1] - xd[1] - xu[-1] + xd[-1]) * i4dxdz;
+          const T* yu = sjy[(k + 1) % 3] + t.c;
+          const T* yd = sjy[(k - 1) % 3] + t.c;
+          const T dyz = (yu[PW] - yd[PW] - yu[-PW] + yd[-PW]) * i4dydz;
+          v = v - dxz - dyz;
+        }
+        sj[(long)k * ip + (long)(t.y - 1) * nxi + (t.x - 1)] = v;
+      }
+    }
+    __syncthreads();
   }
   if (!partials) return;
   block_sum<1>(acc, red);
-  if (threadIdx.x == 0) partials[NFGS_Q_SRC2 * pstride + blockIdx.x] = acc[0];
+  if (threadIdx.x == 0) {
+    const long bid = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    partials[NFGS_Q_SRC2 * pstride + bid] = acc[0];
+  }
 }
 
 __global__ void reduce_kernel(const double* __restrict__ partials, long pstride,
@@ -314,13 +504,16 @@ __global__ void fill_kernel(T* __restrict__ p, long n, T v) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+dim3 tiled_grid(const nfgs_plan* p) {
+  return dim3((unsigned)((p->d.nx + TX - 1) / TX), (unsigned)((p->d.ny + TY - 1) / TY),
+              (unsigned)((p->d.nz + KCHUNK - 1) / KCHUNK));
+}
+
 template <typename T>
 int reconstruct_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi, const void* az,
                   void* b, hipStream_t s) {
   const Grid<T> g = make_grid<T>(p);
-  const long nb = (g.N + 255) / 256;
-  hipLaunchKernelGGL((reconstruct_b_kernel<T>), dim3((unsigned)(nb < 65536 ? nb : 65536)),
-                     dim3(256), 0, s, g, static_cast<const T*>(phi), static_cast<const T*>(az),
+  hipLaunchKernelGGL((reconstruct_b_kernel<T>), tiled_grid(p), dim3(TX * TY), 0, s, g, static_cast<const T*>(phi), static_cast<const T*>(az),
                      static_cast<const T*>(bnd->b_bg), static_cast<const T*>(bnd->bz_obs),
                      static_cast<const T*>(bnd->h_profile), static_cast<T*>(b));
   NFGS_LAUNCH_CHECK();
@@ -332,7 +525,7 @@ int face_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* b, const vo
            void* jout, double* partials, hipStream_t s) {
   const Grid<T> g = make_grid<T>(p);
   const Window w{p->d.win_z0, p->d.win_z1, p->d.win_y0, p->d.win_y1, p->d.win_x0, p->d.win_x1};
-  hipLaunchKernelGGL((face_pass_kernel<T>), dim3(p->pblocks), dim3(256), 0, s, g,
+  hipLaunchKernelGGL((face_pass_kernel<T>), tiled_grid(p), dim3(TX * TY), 0, s, g,
                      static_cast<const T*>(b), static_cast<const T*>(bnd->b_bg),
                      static_cast<const T*>(az), static_cast<const T*>(bnd->jz_t),
                      static_cast<T*>(jp), static_cast<T*>(jout), partials, (long)p->pblocks,
@@ -345,7 +538,7 @@ template <typename T>
 int sources_t(const nfgs_plan* p, const void* jp, void* sb, void* sj, double* partials,
               hipStream_t s) {
   const Grid<T> g = make_grid<T>(p);
-  hipLaunchKernelGGL((sources_kernel<T>), dim3(p->pblocks), dim3(256), 0, s, g,
+  hipLaunchKernelGGL((sources_kernel<T>), tiled_grid(p), dim3(TX * TY), 0, s, g,
                      static_cast<const T*>(jp), static_cast<T*>(sb), static_cast<T*>(sj),
                      partials, (long)p->pblocks, T(1.0 / (4.0 * p->d.dx * p->d.dz)),
                      T(1.0 / (4.0 * p->d.dy * p->d.dz)));
@@ -354,6 +547,11 @@ int sources_t(const nfgs_plan* p, const void* jp, void* sb, void* sj, double* pa
 }
 
 }  // namespace
+
+long tiled_blocks(const nfgs_plan* p) {
+  const dim3 d = tiled_grid(p);
+  return (long)d.x * d.y * d.z;
+}
 
 int fill_zero(const nfgs_plan* p, void* ptr, long n, hipStream_t s) {
   if (n <= 0) return NFGS_OK;

@@ -14,6 +14,8 @@
 // C = T * X so that lanes hold consecutive vectors.  Either way stores are
 // 64-byte runs.
 
+#include <algorithm>
+
 #include "plan.h"
 
 namespace nfgs {
@@ -193,6 +195,234 @@ __global__ __launch_bounds__(256) void paired_transform_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// v2: matrix-stationary persistent kernel.  A workgroup has one wavefront per
+// 16-row output block (NKB blocks per wave); each wave keeps its E and O
+// fragments for every contraction step in registers for the whole kernel.
+// Tiles of V vectors stream through LDS: the next tile's global loads are
+// issued before the current tile's MFMAs, so HBM latency hides behind the
+// matrix work.  LDS holds the even and odd halves of the tile de-interleaved.
+// ---------------------------------------------------------------------------
+template <typename T, bool EC, int QMAX, int NKB, int PF, int V>
+__global__ __launch_bounds__(512) void paired_transform_v2(
+    const T* __restrict__ in, long ies, long ivs, long ips, T* __restrict__ out, long oes,
+    long ovs, long ops, const T* __restrict__ et, const T* __restrict__ ot, int n, int h,
+    int ne, int cols_pad, int nvec, long nvec_total, int S, int hb, int rem) {
+  using A = typename Mfma<T>::acc;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int Q = (ne + 3) >> 2;
+  const int rowsL = EC ? V : 4 * Q;
+  T* xe = reinterpret_cast<T*>(smem_raw);
+  T* xo = xe + rowsL * S;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, l4 = lane >> 4;
+  // hb: 16-row blocks done by MFMA waves; rem (<= 2) rows after them are
+  // done on the VALU from the same LDS tile.
+  T* mrem = xo + rowsL * S;  // [2][rem][4Q]: E then O columns of the remainder rows
+
+  for (int e = tid; e < 2 * rowsL * S; e += nthr) xe[e] = T(0);
+  for (int e = tid; e < 2 * rem * 4 * Q; e += nthr) {
+    const int half = e / (rem * 4 * Q), rr = (e / (4 * Q)) % rem, jp = e % (4 * Q);
+    const T* src = half ? ot : et;
+    mrem[e] = jp < ne ? src[(long)jp * cols_pad + hb * 16 + rr] : T(0);
+  }
+
+  T mE[NKB][QMAX], mO[NKB][QMAX];
+#pragma unroll
+  for (int kb = 0; kb < NKB; ++kb) {
+    const int blk = wave * NKB + kb;
+    const int col = blk * 16 + l16;
+#pragma unroll
+    for (int ks = 0; ks < QMAX; ++ks) {
+      const int jp = ks * 4 + l4;
+      const bool ok = blk < hb && ks < Q;
+      mE[kb][ks] = ok ? et[(long)jp * cols_pad + col] : T(0);
+      mO[kb][ks] = ok ? ot[(long)jp * cols_pad + col] : T(0);
+    }
+  }
+
+  const long ntiles = (nvec_total + V - 1) / V;
+  // Loader mapping: each thread owns one vector of the tile and a strided
+  // subset of its elements (EC: nthr/V consecutive threads per vector;
+  // otherwise V consecutive threads cover V consecutive vectors).
+  const int myv = EC ? tid / (nthr / V) : tid % V;
+  const int j0 = EC ? tid % (nthr / V) : tid / V;
+  const int jstep = nthr / V;
+  T pf[PF];
+  auto load_tile = [&](long t) {
+    const long g = t * V + myv;
+    long base = -1;
+    if (g < nvec_total) {
+      const long p = g / nvec, q = g - p * nvec;
+      base = p * ips + q * ivs;
+    }
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int j = j0 + jstep * i;
+      pf[i] = (base >= 0 && j < n) ? in[base + (long)j * ies] : T(0);
+    }
+  };
+  long tile = blockIdx.x;
+  if (tile < ntiles) load_tile(tile);
+  __syncthreads();
+
+  for (; tile < ntiles; tile += gridDim.x) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int j = j0 + jstep * i;
+      if (j < n) {
+        const int jp = j >> 1;
+        const int idx = EC ? myv * S + jp : jp * S + myv;
+        if (j & 1)
+          xo[idx] = pf[i];
+        else
+          xe[idx] = pf[i];
+      }
+    }
+    __syncthreads();
+    const long next = tile + gridDim.x;
+    if (next < ntiles) load_tile(next);
+
+    A accE[V / 16][NKB], accO[V / 16][NKB];
+#pragma unroll
+    for (int g = 0; g < V / 16; ++g)
+#pragma unroll
+      for (int kb = 0; kb < NKB; ++kb) {
+        accE[g][kb] = A{0, 0, 0, 0};
+        accO[g][kb] = A{0, 0, 0, 0};
+      }
+#pragma unroll
+    for (int ks = 0; ks < QMAX; ++ks) {
+      if (ks < Q) {
+        const int jp = ks * 4 + l4;
+#pragma unroll
+        for (int g = 0; g < V / 16; ++g) {
+          if (EC) {
+            const T aE = xe[(g * 16 + l16) * S + jp];
+            const T aO = xo[(g * 16 + l16) * S + jp];
+#pragma unroll
+            for (int kb = 0; kb < NKB; ++kb) {
+              accE[g][kb] = Mfma<T>::mma(aE, mE[kb][ks], accE[g][kb]);
+              accO[g][kb] = Mfma<T>::mma(aO, mO[kb][ks], accO[g][kb]);
+            }
+          } else {
+            const T bE = xe[jp * S + g * 16 + l16];
+            const T bO = xo[jp * S + g * 16 + l16];
+#pragma unroll
+            for (int kb = 0; kb < NKB; ++kb) {
+              accE[g][kb] = Mfma<T>::mma(mE[kb][ks], bE, accE[g][kb]);
+              accO[g][kb] = Mfma<T>::mma(mO[kb][ks], bO, accO[g][kb]);
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < V / 16; ++g)
+#pragma unroll
+      for (int kb = 0; kb < NKB; ++kb) {
+        const int blk = wave * NKB + kb;
+        if (blk >= hb) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int k, v;
+          if (EC) {
+            v = g * 16 + Mfma<T>::row(l4, r);
+            k = blk * 16 + l16;
+          } else {
+            k = blk * 16 + Mfma<T>::row(l4, r);
+            v = g * 16 + l16;
+          }
+          const long gi = tile * V + v;
+          if (k < h && gi < nvec_total) {
+            const long p = gi / nvec, q = gi - p * nvec;
+            T* o = out + p * ops + q * ovs;
+            const T e = accE[g][kb][r], od = accO[g][kb][r];
+            o[(long)k * oes] = e + od;
+            const int k2 = n - 1 - k;
+            if (k2 != k) o[(long)k2 * oes] = e - od;
+          }
+        }
+      }
+    // remainder rows: 16 lanes per (vector, row) pair, shuffle reduction
+    for (int pr = tid >> 4; pr < V * rem; pr += nthr >> 4) {
+      const int v = pr % V, rr = pr / V;
+      const int part = tid & 15;
+      const T* cE = mrem + rr * 4 * Q;
+      const T* cO = mrem + (rem + rr) * 4 * Q;
+      T e = T(0), od = T(0);
+      for (int jp = part; jp < ne; jp += 16) {
+        const int ix = EC ? v * S + jp : jp * S + v;
+        e += cE[jp] * xe[ix];
+        od += cO[jp] * xo[ix];
+      }
+#pragma unroll
+      for (int off = 8; off > 0; off >>= 1) {
+        e += __shfl_xor(e, off, 16);
+        od += __shfl_xor(od, off, 16);
+      }
+      const int k = hb * 16 + rr;
+      const long gi = tile * V + v;
+      if (part == 0 && k < h && gi < nvec_total) {
+        const long p = gi / nvec, q = gi - p * nvec;
+        T* o = out + p * ops + q * ovs;
+        o[(long)k * oes] = e + od;
+        const int k2 = n - 1 - k;
+        if (k2 != k) o[(long)k2 * oes] = e - od;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+int cu_count() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+  }
+  return cus;
+}
+
+// Launch v2 when the shape fits its register plan; returns -1 when it does not.
+template <typename T, bool EC>
+int launch_v2(const PairedMatrix* m, int dir, const T* in, long ies, long ivs, long ips, T* out,
+              long oes, long ovs, long ops, int nvec, long total, hipStream_t s) {
+  constexpr int V = sizeof(T) == 4 ? 32 : 16;
+  constexpr int QMAX = 33, NKB = 1, PF = 18;
+  const int h = m->h, ne = m->rows, Q = (ne + 3) / 4;
+  int hb = (h + 15) / 16, rem = 0;
+  if (h >= 16 && h % 16 <= 2) {
+    hb = h / 16;
+    rem = h % 16;
+  }
+  if (Q > QMAX || hb > 8) return -1;
+  const int nthr = 64 * hb;
+  if (m->n > PF * (nthr / V)) return -1;
+  int S;
+  if (EC) {
+    S = 4 * Q;
+    while (S % 32 != 2) ++S;
+  } else {
+    S = V + (sizeof(T) == 4 ? 16 : 0);
+  }
+  const int rowsL = EC ? V : 4 * Q;
+  const size_t lds = (2ul * rowsL * S + 2ul * rem * 4 * Q) * sizeof(T);
+  if (lds > 64 * 1024) return -1;
+  const long ntiles = (total + V - 1) / V;
+  const long grid = std::min<long>(ntiles, 2L * cu_count());
+  hipLaunchKernelGGL((paired_transform_v2<T, EC, QMAX, NKB, PF, V>), dim3((unsigned)grid),
+                     dim3(nthr), lds, s, in, ies, ivs, ips, out, oes, ovs, ops,
+                     static_cast<const T*>(m->et[dir]), static_cast<const T*>(m->ot[dir]), m->n,
+                     h, ne, m->cols_pad, nvec, total, S, hb, rem);
+  NFGS_LAUNCH_CHECK();
+  return NFGS_OK;
+}
+
 template <typename T>
 int launch_t(const PairedMatrix* m, int dir, const void* in, long ies, long ivs, long ips,
              void* out, long oes, long ovs, long ops, int nvec, int nplanes, hipStream_t s) {
@@ -204,6 +434,13 @@ int launch_t(const PairedMatrix* m, int dir, const void* in, long ies, long ivs,
   T* to = static_cast<T*>(out);
   const T* et = static_cast<const T*>(m->et[dir]);
   const T* ot = static_cast<const T*>(m->ot[dir]);
+  if (ies == 1 && oes == 1) {
+    const int rc = launch_v2<T, true>(m, dir, ti, ies, ivs, ips, to, oes, ovs, ops, nvec, total, s);
+    if (rc >= 0) return rc;
+  } else if (ivs == 1 && ovs == 1) {
+    const int rc = launch_v2<T, false>(m, dir, ti, ies, ivs, ips, to, oes, ovs, ops, nvec, total, s);
+    if (rc >= 0) return rc;
+  }
   if (ies == 1 && oes == 1) {
     hipLaunchKernelGGL((paired_transform_kernel<T, true>), dim3((unsigned)nblk), dim3(256), 0, s,
                        ti, ies, ivs, ips, to, oes, ovs, ops, et, ot, m->n, m->h, m->rows,

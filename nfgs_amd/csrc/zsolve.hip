@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void robin_sweep_kernel(
     out[plane + m] = dp;
   }
   const int nu = nz - 1;
-#pragma unroll 4
+#pragma unroll 16
   for (int r = 1; r < nu - 1; ++r) {
     const T s = src[(long)(r + 1) * plane + m] / l;
     const T den = diag - lo * cp;
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void robin_sweep_kernel(
   }
   T u = dp;
   out[(long)(nz - 1) * plane + m] = u;
-#pragma unroll 4
+#pragma unroll 16
   for (int r = nu - 2; r >= 0; --r) {
     const long o = (long)(r + 1) * plane + m;
     u = out[o] - cs[o] * u;
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void dirichlet_sweep_kernel(
     T dp = (rhs(1) - a * u0) / bd;
     cs[plane + m] = cp;
     out[plane + m] = dp;
-#pragma unroll 4
+#pragma unroll 16
     for (int i = 1; i < n; ++i) {
       const T d = rhs(i + 1);
       const T ce = (i == n - 1) ? T(0) : c;
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void dirichlet_sweep_kernel(
       out[(long)(i + 1) * plane + m] = dp;
     }
     T u = dp;
-#pragma unroll 4
+#pragma unroll 16
     for (int i = n - 2; i >= 0; --i) {
       const long o = (long)(i + 1) * plane + m;
       u = out[o] - cs[o] * u;
