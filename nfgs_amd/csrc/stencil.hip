@@ -38,8 +38,9 @@ Grid<T> make_grid(const nfgs_plan* p) {
 // z-marching tiles.  A block owns a TX x TY column tile and a chunk of z
 // planes; the planes a stencil needs are staged in LDS with a 2-node halo
 // (enough for the one-sided end stencils), and the block walks up in z so
-// every array is read once per plane.  Points outside the domain hold 0 and
-// are never used by a valid stencil.
+// every array is read once per plane.  Halo points outside the domain hold a
+// clamped in-domain value and are never used by a valid stencil (end stencils
+// are one-sided).
 // ---------------------------------------------------------------------------
 constexpr int TX = 32, TY = 8, HH = 2;
 constexpr int PW = TX + 2 * HH, PH = TY + 2 * HH, PS = PW * PH;
@@ -65,6 +66,13 @@ __device__ __forceinline__ Tile make_tile(const Grid<T>& g) {
   return t;
 }
 
+// in-domain offset of a thread's own point (clamped for threads past the edge)
+template <typename T>
+__device__ __forceinline__ long own_offset(const Tile& t, const Grid<T>& g) {
+  const int x = t.x < g.nx ? t.x : g.nx - 1, y = t.y < g.ny ? t.y : g.ny - 1;
+  return (long)y * g.nx + x;
+}
+
 // Two-step staging of a plane region (tile + halo): fetch() issues the
 // global loads into registers, commit() writes them to an LDS slot later, so
 // the loads of plane p+1 are in flight while plane p is being computed.
@@ -78,16 +86,17 @@ struct Fetch {
 template <typename T>
 __device__ __forceinline__ void fetch(Fetch<T>& r, const T* plane, const Tile& t,
                                       const Grid<T>& g) {
+  // Unconditional loads from clamped coordinates: halo points outside the
+  // domain get a neighbour's value, which no valid stencil reads.
 #pragma unroll
   for (int i = 0; i < PPT; ++i) {
-    const int e = threadIdx.x + 256 * i;
-    T v = T(0);
-    if (e < PS) {
-      const int ly = e / PW, lx = e - ly * PW;
-      const int y = t.y0 + ly - HH, x = t.x0 + lx - HH;
-      if (y >= 0 && y < g.ny && x >= 0 && x < g.nx) v = plane[(long)y * g.nx + x];
-    }
-    r.v[i] = v;
+    int e = threadIdx.x + 256 * i;
+    e = e < PS ? e : PS - 1;
+    const int ly = e / PW, lx = e - ly * PW;
+    int y = t.y0 + ly - HH, x = t.x0 + lx - HH;
+    y = y < 0 ? 0 : (y >= g.ny ? g.ny - 1 : y);
+    x = x < 0 ? 0 : (x >= g.nx ? g.nx - 1 : x);
+    r.v[i] = plane[(long)y * g.nx + x];
   }
 }
 
@@ -136,7 +145,7 @@ __global__ __launch_bounds__(256) void reconstruct_b_kernel(
   const int k0 = blockIdx.z * KCHUNK, k1 = min(k0 + KCHUNK, g.nz);
   // Phi planes lo..lo+2 are resident while node k is built
   auto lo_of = [&](int k) { return max(0, min(k - 1, g.nz - 3)); };
-  const long own = t.valid ? (long)t.y * g.nx + t.x : 0;
+  const long own = own_offset(t, g);
   int lo = lo_of(k0);
   {
     Fetch<T> f0, f1, f2;
@@ -150,8 +159,8 @@ __global__ __launch_bounds__(256) void reconstruct_b_kernel(
   Fetch<T> fphi, faz;
   fetch(faz, az + (long)k0 * g.plane, t, g);
   bool have_phi = false;
-  T gx = 0, gy = 0, gz = 0;
-  if (t.valid) {
+  T gx, gy, gz;
+  {
     const long i = (long)k0 * g.plane + own;
     gx = bg[i];
     gy = bg[g.N + i];
@@ -174,12 +183,10 @@ __global__ __launch_bounds__(256) void reconstruct_b_kernel(
         lo = nlo;
       }
       fetch(faz, az + (long)(k + 1) * g.plane, t, g);
-      if (t.valid) {
-        const long i = (long)(k + 1) * g.plane + own;
-        gx = bg[i];
-        gy = bg[g.N + i];
-        gz = bg[2 * g.N + i];
-      }
+      const long i = (long)(k + 1) * g.plane + own;
+      gx = bg[i];
+      gy = bg[g.N + i];
+      gz = bg[2 * g.N + i];
     }
     // dPhi/dz on the tile with halo
     for (int e = threadIdx.x; e < PS; e += blockDim.x) {
@@ -235,7 +242,7 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
   const Tile t = make_tile(g);
   const int k0 = blockIdx.z * KCHUNK, k1 = min(k0 + KCHUNK, g.nz);
   const int ps = max(k0 - 1, 0), pe = min(k1, g.nz - 1);
-  const long off = t.valid ? (long)t.y * g.nx + t.x : 0;
+  const long off = own_offset(t, g);
   const bool edge = t.y == 0 || t.y == g.ny - 1 || t.x == 0 || t.x == g.nx - 1;
   const bool in_win = t.y >= w.y0 && t.y < w.y1 && t.x >= w.x0 && t.x < w.x1;
   const T* Bx = B;
@@ -253,11 +260,9 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
     fetch(fbx, Bx + qo, t, g);
     fetch(fby, By + qo, t, g);
     fetch(faz, az + qo, t, g);
-    if (t.valid) {
-      obz = Bz[qo + off];
-      ogx = bg[qo + off];
-      ogy = bg[g.N + qo + off];
-    }
+    obz = Bz[qo + off];
+    ogx = bg[qo + off];
+    ogy = bg[g.N + qo + off];
   };
   fetch_plane(ps);
   for (int p = ps; p <= pe; ++p) {

@@ -203,29 +203,35 @@ __global__ __launch_bounds__(256) void paired_transform_kernel(
 // issued before the current tile's MFMAs, so HBM latency hides behind the
 // matrix work.  LDS holds the even and odd halves of the tile de-interleaved.
 // ---------------------------------------------------------------------------
-template <typename T, bool EC, int QMAX, int NKB, int PF, int V>
+template <typename T, bool EC, int QMAX, int NKB, int PF, int V, int QC>
 __global__ __launch_bounds__(512) void paired_transform_v2(
     const T* __restrict__ in, long ies, long ivs, long ips, T* __restrict__ out, long oes,
     long ovs, long ops, const T* __restrict__ et, const T* __restrict__ ot, int n, int h,
     int ne, int cols_pad, int nvec, long nvec_total, int S, int hb, int rem) {
   using A = typename Mfma<T>::acc;
+  typedef T vec4 __attribute__((ext_vector_type(4)));
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int Q = (ne + 3) >> 2;
-  const int rowsL = EC ? V : 4 * Q;
+  // Contraction order: lane group l4 takes j' = l4*Q + ks for ks < Q, so a
+  // lane's successive steps are adjacent in LDS and one 4-wide LDS read feeds
+  // four MFMAs.  In LDS the tile row of vector v holds segment l4 at
+  // [l4*Qp, l4*Qp + Q) (Qp = Q rounded up to 4; the gap stays zero).  QC > 0
+  // fixes Q at compile time so the MFMA sequence has no branches.
+  const int Q = QC > 0 ? QC : (ne + 3) >> 2;
+  const int Qp = (Q + 3) & ~3;
   T* xe = reinterpret_cast<T*>(smem_raw);
-  T* xo = xe + rowsL * S;
+  T* xo = xe + V * S;
+  T* mrem = xo + V * S;  // [2][rem][ne]: E then O columns of the remainder rows
+  const int trash = 2 * V * S + 2 * rem * ne;  // word after mrem, never read
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, l4 = lane >> 4;
-  // hb: 16-row blocks done by MFMA waves; rem (<= 2) rows after them are
-  // done on the VALU from the same LDS tile.
-  T* mrem = xo + rowsL * S;  // [2][rem][4Q]: E then O columns of the remainder rows
+  auto pos = [&](int jp) { const int sg = jp / Q; return sg * Qp + (jp - sg * Q); };
 
-  for (int e = tid; e < 2 * rowsL * S; e += nthr) xe[e] = T(0);
-  for (int e = tid; e < 2 * rem * 4 * Q; e += nthr) {
-    const int half = e / (rem * 4 * Q), rr = (e / (4 * Q)) % rem, jp = e % (4 * Q);
+  for (int e = tid; e < 2 * V * S; e += nthr) xe[e] = T(0);
+  for (int e = tid; e < 2 * rem * ne; e += nthr) {
+    const int half = e / (rem * ne), rr = (e / ne) % rem, jp = e % ne;
     const T* src = half ? ot : et;
-    mrem[e] = jp < ne ? src[(long)jp * cols_pad + hb * 16 + rr] : T(0);
+    mrem[e] = src[(long)jp * cols_pad + hb * 16 + rr];
   }
 
   T mE[NKB][QMAX], mO[NKB][QMAX];
@@ -235,7 +241,7 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
     const int col = blk * 16 + l16;
 #pragma unroll
     for (int ks = 0; ks < QMAX; ++ks) {
-      const int jp = ks * 4 + l4;
+      const int jp = l4 * Q + ks;
       const bool ok = blk < hb && ks < Q;
       mE[kb][ks] = ok ? et[(long)jp * cols_pad + col] : T(0);
       mO[kb][ks] = ok ? ot[(long)jp * cols_pad + col] : T(0);
@@ -250,39 +256,74 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
   const int j0 = EC ? tid % (nthr / V) : tid / V;
   const int jstep = nthr / V;
   T pf[PF];
-  auto load_tile = [&](long t) {
-    const long g = t * V + myv;
-    long base = -1;
-    if (g < nvec_total) {
-      const long p = g / nvec, q = g - p * nvec;
-      base = p * ips + q * ivs;
+  bool pvalid = false;
+  // Loads are unconditional: a vector past the end reads vector 0 and
+  // elements past n re-read element n-1.  The raw values go to registers and
+  // are masked only when they are committed to LDS, so nothing waits on the
+  // loads before the MFMA work they overlap.
+  // Offset of vector tp*nvec + tq + v in a (plane, vector) layout.  The
+  // tile base is divided once per tile; when nvec >= V a tile offset wraps
+  // into the next plane at most once.
+  auto vec_off = [&](long tp, long tq, int v, long ps, long vs) -> long {
+    long q = tq + v, p = tp;
+    if (nvec >= V) {
+      const bool w = q >= nvec;
+      q -= w ? nvec : 0;
+      p += w ? 1 : 0;
+    } else {
+      const long d = q / nvec;
+      p += d;
+      q -= d * nvec;
     }
+    return p * ps + q * vs;
+  };
+  // (plane, vector) coordinates of tile t's first vector, advanced by the
+  // grid stride without a divide
+  const long dstep = (long)gridDim.x * V;
+  const long sp = dstep / nvec, sq = dstep - sp * nvec;
+  auto advance = [&](long& tp, long& tq) {
+    tp += sp;
+    tq += sq;
+    if (tq >= nvec) {
+      tq -= nvec;
+      ++tp;
+    }
+  };
+  auto load_tile = [&](long t, long tp, long tq) {
+    pvalid = t * V + myv < nvec_total;
+    const long base = pvalid ? vec_off(tp, tq, myv, ips, ivs) : 0;
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
       const int j = j0 + jstep * i;
-      pf[i] = (base >= 0 && j < n) ? in[base + (long)j * ies] : T(0);
+      const int jc = j < n ? j : n - 1;
+      pf[i] = in[base + (long)jc * ies];
     }
   };
   long tile = blockIdx.x;
-  if (tile < ntiles) load_tile(tile);
+  long ctp = tile * V / nvec, ctq = tile * V - ctp * nvec;
+  if (tile < ntiles) load_tile(tile, ctp, ctq);
   __syncthreads();
 
   for (; tile < ntiles; tile += gridDim.x) {
+    // Every register is stored (elements past n go to a trash word after
+    // the tile), so the waits on the loads sit in straight-line code.
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
       const int j = j0 + jstep * i;
-      if (j < n) {
-        const int jp = j >> 1;
-        const int idx = EC ? myv * S + jp : jp * S + myv;
-        if (j & 1)
-          xo[idx] = pf[i];
-        else
-          xe[idx] = pf[i];
-      }
+      const int idx = j < n ? (j & 1) * V * S + myv * S + pos(j >> 1) : trash;
+      xe[idx] = pvalid ? pf[i] : T(0);
     }
     __syncthreads();
+    // the last tile re-reads itself instead of branching around the loads
     const long next = tile + gridDim.x;
-    if (next < ntiles) load_tile(next);
+    const long otp = ctp, otq = ctq;
+    advance(ctp, ctq);
+    if (next < ntiles)
+      load_tile(next, ctp, ctq);
+    else
+      load_tile(tile, otp, otq);
+    // keep the loads ahead of the MFMA section they overlap
+    __builtin_amdgcn_sched_barrier(0);
 
     A accE[V / 16][NKB], accO[V / 16][NKB];
 #pragma unroll
@@ -292,31 +333,82 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
         accE[g][kb] = A{0, 0, 0, 0};
         accO[g][kb] = A{0, 0, 0, 0};
       }
+    if (QC > 0) {
+      // static schedule: fragments of step group kq+1 are read while the
+      // MFMAs of group kq run
+      constexpr int KQ = (QC + 3) / 4;
+      vec4 ce[V / 16], co[V / 16];
+      const int rb = l16 * S + l4 * Qp;
 #pragma unroll
-    for (int ks = 0; ks < QMAX; ++ks) {
-      if (ks < Q) {
-        const int jp = ks * 4 + l4;
+      for (int g = 0; g < V / 16; ++g) {
+        ce[g] = *reinterpret_cast<const vec4*>(xe + g * 16 * S + rb);
+        co[g] = *reinterpret_cast<const vec4*>(xo + g * 16 * S + rb);
+      }
+#pragma unroll
+      for (int kq = 0; kq < KQ; ++kq) {
+        vec4 ne_[V / 16], no_[V / 16];
+        if (kq + 1 < KQ) {
+#pragma unroll
+          for (int g = 0; g < V / 16; ++g) {
+            ne_[g] = *reinterpret_cast<const vec4*>(xe + g * 16 * S + rb + (kq + 1) * 4);
+            no_[g] = *reinterpret_cast<const vec4*>(xo + g * 16 * S + rb + (kq + 1) * 4);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int ks = kq * 4 + u;
+          if (ks < QC) {
+#pragma unroll
+            for (int g = 0; g < V / 16; ++g) {
+#pragma unroll
+              for (int kb = 0; kb < NKB; ++kb) {
+                if (EC) {
+                  accE[g][kb] = Mfma<T>::mma(ce[g][u], mE[kb][ks], accE[g][kb]);
+                  accO[g][kb] = Mfma<T>::mma(co[g][u], mO[kb][ks], accO[g][kb]);
+                } else {
+                  accE[g][kb] = Mfma<T>::mma(mE[kb][ks], ce[g][u], accE[g][kb]);
+                  accO[g][kb] = Mfma<T>::mma(mO[kb][ks], co[g][u], accO[g][kb]);
+                }
+              }
+            }
+          }
+        }
+        if (kq + 1 < KQ) {
+#pragma unroll
+          for (int g = 0; g < V / 16; ++g) {
+            ce[g] = ne_[g];
+            co[g] = no_[g];
+          }
+        }
+      }
+    } else {
+#pragma unroll
+    for (int kq = 0; kq < (QMAX + 3) / 4; ++kq) {
+      if (kq * 4 < Q) {
 #pragma unroll
         for (int g = 0; g < V / 16; ++g) {
-          if (EC) {
-            const T aE = xe[(g * 16 + l16) * S + jp];
-            const T aO = xo[(g * 16 + l16) * S + jp];
+          const int ix = (g * 16 + l16) * S + l4 * Qp + kq * 4;
+          const vec4 ve = *reinterpret_cast<const vec4*>(xe + ix);
+          const vec4 vo = *reinterpret_cast<const vec4*>(xo + ix);
 #pragma unroll
-            for (int kb = 0; kb < NKB; ++kb) {
-              accE[g][kb] = Mfma<T>::mma(aE, mE[kb][ks], accE[g][kb]);
-              accO[g][kb] = Mfma<T>::mma(aO, mO[kb][ks], accO[g][kb]);
-            }
-          } else {
-            const T bE = xe[jp * S + g * 16 + l16];
-            const T bO = xo[jp * S + g * 16 + l16];
+          for (int u = 0; u < 4; ++u) {
+            const int ks = kq * 4 + u;
+            if (ks < QMAX && ks < Q) {
 #pragma unroll
-            for (int kb = 0; kb < NKB; ++kb) {
-              accE[g][kb] = Mfma<T>::mma(mE[kb][ks], bE, accE[g][kb]);
-              accO[g][kb] = Mfma<T>::mma(mO[kb][ks], bO, accO[g][kb]);
+              for (int kb = 0; kb < NKB; ++kb) {
+                if (EC) {
+                  accE[g][kb] = Mfma<T>::mma(ve[u], mE[kb][ks], accE[g][kb]);
+                  accO[g][kb] = Mfma<T>::mma(vo[u], mO[kb][ks], accO[g][kb]);
+                } else {
+                  accE[g][kb] = Mfma<T>::mma(mE[kb][ks], ve[u], accE[g][kb]);
+                  accO[g][kb] = Mfma<T>::mma(mO[kb][ks], vo[u], accO[g][kb]);
+                }
+              }
             }
           }
         }
       }
+    }
     }
 #pragma unroll
     for (int g = 0; g < V / 16; ++g)
@@ -336,8 +428,7 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
           }
           const long gi = tile * V + v;
           if (k < h && gi < nvec_total) {
-            const long p = gi / nvec, q = gi - p * nvec;
-            T* o = out + p * ops + q * ovs;
+            T* o = out + vec_off(otp, otq, v, ops, ovs);
             const T e = accE[g][kb][r], od = accO[g][kb][r];
             o[(long)k * oes] = e + od;
             const int k2 = n - 1 - k;
@@ -349,11 +440,11 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
     for (int pr = tid >> 4; pr < V * rem; pr += nthr >> 4) {
       const int v = pr % V, rr = pr / V;
       const int part = tid & 15;
-      const T* cE = mrem + rr * 4 * Q;
-      const T* cO = mrem + (rem + rr) * 4 * Q;
+      const T* cE = mrem + rr * ne;
+      const T* cO = mrem + (rem + rr) * ne;
       T e = T(0), od = T(0);
       for (int jp = part; jp < ne; jp += 16) {
-        const int ix = EC ? v * S + jp : jp * S + v;
+        const int ix = v * S + pos(jp);
         e += cE[jp] * xe[ix];
         od += cO[jp] * xo[ix];
       }
@@ -365,8 +456,7 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
       const int k = hb * 16 + rr;
       const long gi = tile * V + v;
       if (part == 0 && k < h && gi < nvec_total) {
-        const long p = gi / nvec, q = gi - p * nvec;
-        T* o = out + p * ops + q * ovs;
+        T* o = out + vec_off(otp, otq, v, ops, ovs);
         o[(long)k * oes] = e + od;
         const int k2 = n - 1 - k;
         if (k2 != k) o[(long)k2 * oes] = e - od;
@@ -403,22 +493,34 @@ int launch_v2(const PairedMatrix* m, int dir, const T* in, long ies, long ivs, l
   if (Q > QMAX || hb > 8) return -1;
   const int nthr = 64 * hb;
   if (m->n > PF * (nthr / V)) return -1;
-  int S;
-  if (EC) {
-    S = 4 * Q;
-    while (S % 32 != 2) ++S;
-  } else {
-    S = V + (sizeof(T) == 4 ? 16 : 0);
-  }
-  const int rowsL = EC ? V : 4 * Q;
-  const size_t lds = (2ul * rowsL * S + 2ul * rem * 4 * Q) * sizeof(T);
+  // row stride >= 4*Qp, congruent to 4 mod 32 words: the 16-byte reads of
+  // eight consecutive rows land on distinct bank groups
+  const int Qp = (Q + 3) & ~3;
+  int S = 4 * Qp;
+  while (S % 32 != 4) ++S;
+  const size_t lds = (2ul * V * S + 2ul * rem * ne + 1) * sizeof(T);
   if (lds > 64 * 1024) return -1;
   const long ntiles = (total + V - 1) / V;
   const long grid = std::min<long>(ntiles, 2L * cu_count());
-  hipLaunchKernelGGL((paired_transform_v2<T, EC, QMAX, NKB, PF, V>), dim3((unsigned)grid),
-                     dim3(nthr), lds, s, in, ies, ivs, ips, out, oes, ovs, ops,
-                     static_cast<const T*>(m->et[dir]), static_cast<const T*>(m->ot[dir]), m->n,
-                     h, ne, m->cols_pad, nvec, total, S, hb, rem);
+#define NFGS_V2(QC_)                                                                      \
+  hipLaunchKernelGGL((paired_transform_v2<T, EC, QMAX, NKB, PF, V, QC_>), dim3((unsigned)grid), \
+                     dim3(nthr), lds, s, in, ies, ivs, ips, out, oes, ovs, ops,                 \
+                     static_cast<const T*>(m->et[dir]), static_cast<const T*>(m->ot[dir]),     \
+                     m->n, h, ne, m->cols_pad, nvec, total, S, hb, rem)
+  switch (Q) {
+    case 33: NFGS_V2(33); break;
+    case 32: NFGS_V2(32); break;
+    case 25: NFGS_V2(25); break;
+    case 24: NFGS_V2(24); break;
+    case 17: NFGS_V2(17); break;
+    case 16: NFGS_V2(16); break;
+    case 13: NFGS_V2(13); break;
+    case 12: NFGS_V2(12); break;
+    case 9: NFGS_V2(9); break;
+    case 8: NFGS_V2(8); break;
+    default: NFGS_V2(0); break;
+  }
+#undef NFGS_V2
   NFGS_LAUNCH_CHECK();
   return NFGS_OK;
 }

@@ -12,6 +12,35 @@
 namespace nfgs {
 namespace {
 
+constexpr int KB = 32;
+
+// u_r = d'_r - c'_r u_{r+1} for r = top .. 0 (row r lives on plane r+1);
+// blocks of KB rows load d' and c' before any store of the block.
+template <typename T>
+__device__ __forceinline__ void back_substitute(T* out, const T* cs, long plane, long m, int top,
+                                                T u) {
+  for (int rb = top; rb >= 0; rb -= KB) {
+    T dv[KB], cv[KB];
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+      const int r = rb - i;
+      if (r >= 0) {
+        const long o = (long)(r + 1) * plane + m;
+        dv[i] = out[o];
+        cv[i] = cs[o];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+      const int r = rb - i;
+      if (r >= 0) {
+        u = dv[i] - cv[i] * u;
+        out[(long)(r + 1) * plane + m] = u;
+      }
+    }
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void robin_sweep_kernel(
     const T* __restrict__ src, T* __restrict__ out, T* __restrict__ cs, const T* __restrict__ lam,
@@ -56,14 +85,27 @@ __global__ __launch_bounds__(256) void robin_sweep_kernel(
     out[plane + m] = dp;
   }
   const int nu = nz - 1;
-#pragma unroll 16
-  for (int r = 1; r < nu - 1; ++r) {
-    const T s = src[(long)(r + 1) * plane + m] / l;
-    const T den = diag - lo * cp;
-    cp = up / den;
-    dp = (s - lo * dp) / den;
-    cs[(long)(r + 1) * plane + m] = cp;
-    out[(long)(r + 1) * plane + m] = dp;
+  // Rows are processed in blocks whose loads are all issued before the
+  // block's stores, so KB row loads per lane are in flight at once.
+  for (int rb = 1; rb < nu - 1; rb += KB) {
+    T sv[KB];
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+      const int r = rb + i;
+      sv[i] = r < nu - 1 ? src[(long)(r + 1) * plane + m] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+      const int r = rb + i;
+      if (r < nu - 1) {
+        const T s = sv[i] / l;
+        const T den = diag - lo * cp;
+        cp = up / den;
+        dp = (s - lo * dp) / den;
+        cs[(long)(r + 1) * plane + m] = cp;
+        out[(long)(r + 1) * plane + m] = dp;
+      }
+    }
   }
   // Top Robin row with u_{N-2} eliminated through the PDE at plane N-1.
   {
@@ -76,12 +118,7 @@ __global__ __launch_bounds__(256) void robin_sweep_kernel(
   }
   T u = dp;
   out[(long)(nz - 1) * plane + m] = u;
-#pragma unroll 16
-  for (int r = nu - 2; r >= 0; --r) {
-    const long o = (long)(r + 1) * plane + m;
-    u = out[o] - cs[o] * u;
-    out[o] = u;
-  }
+  back_substitute(out, cs, plane, m, nu - 2, u);
   out[m] = b;
 }
 
@@ -104,23 +141,28 @@ __global__ __launch_bounds__(256) void dirichlet_sweep_kernel(
     T dp = (rhs(1) - a * u0) / bd;
     cs[plane + m] = cp;
     out[plane + m] = dp;
-#pragma unroll 16
-    for (int i = 1; i < n; ++i) {
-      const T d = rhs(i + 1);
-      const T ce = (i == n - 1) ? T(0) : c;
-      const T den = bd - a * cp;
-      cp = ce / den;
-      dp = (d - a * dp) / den;
-      cs[(long)(i + 1) * plane + m] = cp;
-      out[(long)(i + 1) * plane + m] = dp;
+    for (int ib = 1; ib < n; ib += KB) {
+      T sv[KB];
+#pragma unroll
+      for (int q = 0; q < KB; ++q) {
+        const int i = ib + q;
+        sv[q] = i < n ? src[(long)(i + 1) * plane + m] : T(0);
+      }
+#pragma unroll
+      for (int q = 0; q < KB; ++q) {
+        const int i = ib + q;
+        if (i < n) {
+          const T d = (-sv[q]) / l;
+          const T ce = (i == n - 1) ? T(0) : c;
+          const T den = bd - a * cp;
+          cp = ce / den;
+          dp = (d - a * dp) / den;
+          cs[(long)(i + 1) * plane + m] = cp;
+          out[(long)(i + 1) * plane + m] = dp;
+        }
+      }
     }
-    T u = dp;
-#pragma unroll 16
-    for (int i = n - 2; i >= 0; --i) {
-      const long o = (long)(i + 1) * plane + m;
-      u = out[o] - cs[o] * u;
-      out[o] = u;
-    }
+    back_substitute(out, cs, plane, m, n - 2, dp);
   }
   out[m] = u0;
   out[(long)(nz - 1) * plane + m] = T(0);
