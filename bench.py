@@ -160,6 +160,26 @@ PER_ITER = {"face_pass": 1, "sources": 1, "reconstruct_b": 1, "robin_sweep": 1,
             "dirichlet_sweep": 1, "dct_x": 2, "dct_y": 2, "dst_x": 2, "dst_y": 2}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_258x258x257_f32.json")
+
+
+def measured_traffic(kernel, grid_zyx):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the calibration kernel), or
+    None when no summary matches this kernel and grid."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if list(rec.get("grid_zyx", [])) != list(grid_zyx):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return int(round(k["fetch_bytes"] + k["write_bytes"]))
+
+
 def cpu_baseline(n, rho, planes_np, iters, gpu_iters):
     """CPU oracle (NumPy, one thread of the host) on a bounded sample: setup,
     gs_cfcf start and `iters` iterations at the full grid, extrapolated to
@@ -203,11 +223,13 @@ def main():
     planes = tuple(torch.tensor(p, dtype=torch.float64, device=dev) for p in (bx, by, bz))
     torch.cuda.synchronize()
     if a.calibrate_copy:
+        # one elementwise kernel that reads 256 MiB and writes 256 MiB: the
+        # scale of the FETCH_SIZE / WRITE_SIZE counters
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.neg(src, out=dst)
         torch.cuda.synchronize()
-        log("calibration copy: 256 MiB read + 256 MiB written")
+        log("calibration kernel: neg over 256 MiB (256 MiB read + 256 MiB written)")
         del src, dst
 
     # headline: plain method at rho 0, 0.5 and 1 (value = rho 0.5)
@@ -272,6 +294,8 @@ def main():
                     avg_launch_us=round(t * 1e6, 2))
     roofline = roof(dom)
     stencil_roof = roof(dom_sten)
+    for r in (roofline, stencil_roof):
+        r["traffic"] = measured_traffic(r["kernel"], [s.Nz, s.Ny, s.Nx])
     log(f"dominant kernel {dom}: {roofline}")
 
     # error of the FP32 solve against an FP64 solve of the same boundary

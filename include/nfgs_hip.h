@@ -224,6 +224,38 @@ int nfgs_setup_gaussian(const nfgs_plan* plan, const void* bx, const void* by, c
                         void* dphi_target, void* d0, void* phi_above_hat, void* h_target,
                         double* diag, void* workspace, void* stream);
 
+/* ---- post-selection diagnostics ---------------------------------------- */
+
+/* out[0] = sum (a-b)^2, out[1] = sum b^2 over n elements (FP64 device
+ * doubles).  The relative rms differences of the toroidal boundary
+ * diagnostics (nfgs_solver.py:3047-3055) and of final_vs_best_B_relative
+ * (nfgs_solver.py:3820-3835). */
+int nfgs_sq_diff(const nfgs_plan* plan, const void* a, const void* b, long n, double* out,
+                 void* stream);
+
+/* Workspace bytes needed by nfgs_source_diagnostics (24 planes). */
+long nfgs_source_diagnostics_workspace_bytes(const nfgs_plan* plan);
+
+/* compute_source_diagnostics of a selected B (nfgs_solver.py:7014-7196).
+ * Writes NFGS_SD_COUNT device doubles (sums over one plane; the host takes
+ * rms = sqrt(sum / (nx*ny))).  Needs nz >= 3. */
+enum {
+  NFGS_SD_BOTTOM_RECON_SQ = 0, /* sum (B_bg,z(0) + Bz_above - Bz_obs)^2          :7030-7033 */
+  NFGS_SD_BOTTOM_RECON_MAX,    /* max |B_bg,z(0) + Bz_above - Bz_obs|                       */
+  NFGS_SD_CORR_BOTTOM_SQ,      /* sum (Bc_z(0) - Bz_above)^2                          :7033 */
+  NFGS_SD_TOP_ROBIN_SQ,        /* top Robin residual of Bc_z                     :7035-7042 */
+  NFGS_SD_TOP_PHI_ROBIN_SQ,    /* top Robin residual of Phi_c                    :7044-7053 */
+  NFGS_SD_TOP_UPWARD_SQ,       /* upward-going part at the top                   :7054-7059 */
+  NFGS_SD_TOP_BCZ_SQ,          /* sum Bc_z(top)^2 (ratio denominator)                      */
+  NFGS_SD_BOTTOM_DIV_SQ,       /* sum (div_xy B(0) - D0)^2                       :7067-7081 */
+  NFGS_SD_D0_SQ,               /* sum D0^2                                                 */
+  NFGS_SD_DPHI_RES_SQ,         /* sum (dPhi_c/dz(0) - dphi_target)^2             :7082-7090 */
+  NFGS_SD_COUNT
+};
+int nfgs_source_diagnostics(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* b,
+                            const void* d0, const void* dphi_target, const void* bz_above,
+                            void* workspace, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,4 +103,16 @@ __device__ __forceinline__ void block_sum(double (&v)[NQ], double* lds /* [NQ*16
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs, each
+// with its own L2.  xcd_tile maps launch-order workgroup w of n to a logical
+// tile so that every XCD owns one contiguous range of tiles: neighbouring
+// tiles (shared halo and cache lines) then meet in the same L2.  A bijection
+// on [0, n) for any n.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int w, int n) {
+  const int per = n / kXcds, extra = n % kXcds;
+  const int x = w % kXcds, i = w / kXcds;
+  return x * per + (x < extra ? x : extra) + i;
+}
+
 }  // namespace nfgs

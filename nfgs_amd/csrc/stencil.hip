@@ -10,12 +10,17 @@
 namespace nfgs {
 namespace {
 
+constexpr int TX = 32, TY = 8, HH = 2;
+constexpr int PW = TX + 2 * HH, PH = TY + 2 * HH, PS = PW * PH;
+constexpr int KCHUNK = 32;
+
 template <typename T>
 struct Grid {
   int nx, ny, nz;
   long plane, N;
   T dx, dy, dz;
   T idx2, idy2;
+  int tgx, tgy;  // tiles along x and y (the launch is 1-D over tgx*tgy*z-chunks)
 };
 
 template <typename T>
@@ -31,6 +36,8 @@ Grid<T> make_grid(const nfgs_plan* p) {
   g.dz = T(p->d.dz);
   g.idx2 = T(1.0 / (p->d.dx * p->d.dx));
   g.idy2 = T(1.0 / (p->d.dy * p->d.dy));
+  g.tgx = (g.nx + TX - 1) / TX;
+  g.tgy = (g.ny + TY - 1) / TY;
   return g;
 }
 
@@ -42,12 +49,10 @@ Grid<T> make_grid(const nfgs_plan* p) {
 // clamped in-domain value and are never used by a valid stencil (end stencils
 // are one-sided).
 // ---------------------------------------------------------------------------
-constexpr int TX = 32, TY = 8, HH = 2;
-constexpr int PW = TX + 2 * HH, PH = TY + 2 * HH, PS = PW * PH;
-constexpr int KCHUNK = 32;
 
 struct Tile {
-  int x0, y0, tx, ty, x, y;
+  int id;  // logical tile id (x fastest, then y, then z-chunk)
+  int x0, y0, k0, tx, ty, x, y;
   bool valid;
   int c;  // LDS index of this thread's point
 };
@@ -55,8 +60,11 @@ struct Tile {
 template <typename T>
 __device__ __forceinline__ Tile make_tile(const Grid<T>& g) {
   Tile t;
-  t.x0 = blockIdx.x * TX;
-  t.y0 = blockIdx.y * TY;
+  t.id = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = t.id % g.tgx, r = t.id / g.tgx;
+  t.x0 = bx * TX;
+  t.y0 = (r % g.tgy) * TY;
+  t.k0 = (r / g.tgy) * KCHUNK;
   t.tx = threadIdx.x % TX;
   t.ty = threadIdx.x / TX;
   t.x = t.x0 + t.tx;
@@ -142,7 +150,7 @@ __global__ __launch_bounds__(256) void reconstruct_b_kernel(
   __shared__ T saz[PS];
   __shared__ T sdz[PS];
   const Tile t = make_tile(g);
-  const int k0 = blockIdx.z * KCHUNK, k1 = min(k0 + KCHUNK, g.nz);
+  const int k0 = t.k0, k1 = min(k0 + KCHUNK, g.nz);
   // Phi planes lo..lo+2 are resident while node k is built
   auto lo_of = [&](int k) { return max(0, min(k - 1, g.nz - 3)); };
   const long own = own_offset(t, g);
@@ -240,7 +248,7 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
   __shared__ T saz[PS];
   __shared__ double red[NFGS_NQ * 16];
   const Tile t = make_tile(g);
-  const int k0 = blockIdx.z * KCHUNK, k1 = min(k0 + KCHUNK, g.nz);
+  const int k0 = t.k0, k1 = min(k0 + KCHUNK, g.nz);
   const int ps = max(k0 - 1, 0), pe = min(k1, g.nz - 1);
   const long off = own_offset(t, g);
   const bool edge = t.y == 0 || t.y == g.ny - 1 || t.x == 0 || t.x == g.nx - 1;
@@ -400,7 +408,7 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
   if (!partials) return;
   block_sum<7>(acc, red);
   if (threadIdx.x == 0) {
-    const long bid = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    const long bid = t.id;
     partials[NFGS_Q_SIGMA_NUM * pstride + bid] = acc[0];
     partials[NFGS_Q_SIGMA_DEN * pstride + bid] = acc[1];
     partials[NFGS_Q_DIV_ABS * pstride + bid] = acc[2];
@@ -422,7 +430,7 @@ __global__ __launch_bounds__(256) void sources_kernel(Grid<T> g, const T* __rest
   __shared__ T sjz[PS];
   __shared__ double red[NFGS_NQ * 16];
   const Tile t = make_tile(g);
-  const int k0 = blockIdx.z * KCHUNK, k1 = min(k0 + KCHUNK, g.nz);
+  const int k0 = t.k0, k1 = min(k0 + KCHUNK, g.nz);
   const T* jx = jp;
   const T* jy = jp + g.N;
   const T* jzp = jp + 2 * g.N;
@@ -480,7 +488,7 @@ __global__ __launch_bounds__(256) void sources_kernel(Grid<T> g, const T* __rest
   if (!partials) return;
   block_sum<1>(acc, red);
   if (threadIdx.x == 0) {
-    const long bid = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    const long bid = t.id;
     partials[NFGS_Q_SRC2 * pstride + bid] = acc[0];
   }
 }
@@ -510,8 +518,8 @@ __global__ void fill_kernel(T* __restrict__ p, long n, T v) {
 }
 
 dim3 tiled_grid(const nfgs_plan* p) {
-  return dim3((unsigned)((p->d.nx + TX - 1) / TX), (unsigned)((p->d.ny + TY - 1) / TY),
-              (unsigned)((p->d.nz + KCHUNK - 1) / KCHUNK));
+  return dim3((unsigned)(((p->d.nx + TX - 1) / TX) * ((p->d.ny + TY - 1) / TY) *
+                         ((p->d.nz + KCHUNK - 1) / KCHUNK)));
 }
 
 template <typename T>

@@ -299,7 +299,7 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
       pf[i] = in[base + (long)jc * ies];
     }
   };
-  long tile = blockIdx.x;
+  long tile = xcd_tile(blockIdx.x, gridDim.x);
   long ctp = tile * V / nvec, ctq = tile * V - ctp * nvec;
   if (tile < ntiles) load_tile(tile, ctp, ctq);
   __syncthreads();
