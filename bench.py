@@ -96,6 +96,8 @@ def time_kernels(torch, s, L, reps):
     es = phi.element_size()
     part = L.ptr(s._partials)
     calls = {
+        "state_pass": lambda: lib.nfgs_state_pass(plan, bnd, L.ptr(phi), L.ptr(az), None,
+                                                  L.ptr(wt["jp"]), None, part, st),
         "face_pass": lambda: lib.nfgs_face_pass(plan, bnd, L.ptr(s._Bwork), L.ptr(az),
                                                 L.ptr(wt["jp"]), None, part, st),
         "sources": lambda: lib.nfgs_sources(plan, L.ptr(wt["jp"]), L.ptr(wt["s_bcz"]),
@@ -142,6 +144,8 @@ def kernel_model(s, esize):
     NI = (Nz - 2) * (Ny - 2) * (Nx - 2)
     return {
         # words per node: reads + writes
+        # state_pass: Phi_c, Az, B_bg (3) read; J_par (3) written
+        "state_pass": dict(bytes=(1 + 1 + 3 + 3) * N * esize, words=8),
         "face_pass": dict(bytes=(3 + 3 + 1 + 3) * N * esize, words=10),
         "sources": dict(bytes=(3 + 1) * N * esize + NI * esize, words=5),
         "reconstruct_b": dict(bytes=(1 + 1 + 3 + 3) * N * esize, words=8),
@@ -155,9 +159,12 @@ def kernel_model(s, esize):
 
 
 # launches of each kernel class in one iteration (forward and inverse passes
-# of each transform run the same kernel on the same shapes)
-PER_ITER = {"face_pass": 1, "sources": 1, "reconstruct_b": 1, "robin_sweep": 1,
-            "dirichlet_sweep": 1, "dct_x": 2, "dct_y": 2, "dst_x": 2, "dst_y": 2}
+# of each transform run the same kernel on the same shapes).  reconstruct_b
+# and face_pass are the unfused API kernels (iteration_step on a given B,
+# final fields); the iteration runs the fused state_pass instead.
+PER_ITER = {"state_pass": 1, "face_pass": 0, "sources": 1, "reconstruct_b": 0,
+            "robin_sweep": 1, "dirichlet_sweep": 1, "dct_x": 2, "dct_y": 2, "dst_x": 2,
+            "dst_y": 2}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_258x258x257_f32.json")

@@ -153,14 +153,25 @@ int nfgs_face_pass(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* 
 int nfgs_sources(const nfgs_plan* plan, const void* jp, void* s_bcz, void* s_jz,
                  double* partials, void* stream);
 
+/* Fused reconstruction + face pass: B of (phi, az) as nfgs_reconstruct_b
+ * builds it, consumed plane by plane by the face pass of nfgs_face_pass
+ * without a round trip through memory.  Outputs as nfgs_face_pass; b_out
+ * (3*N, may be NULL) receives B.  Results equal nfgs_reconstruct_b followed
+ * by nfgs_face_pass.  Replaces _reconstruct_state_from_scalars plus
+ * _staggered_face_current and the monitors (nfgs_solver.py:5901-5942,
+ * :5738-5839, :5882-5895, :2656-2697). */
+int nfgs_state_pass(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* phi,
+                    const void* az, void* b_out, void* jp_out, void* j_out, double* partials,
+                    void* stream);
+
 /* Deterministic final reduction of the partials into NFGS_NQ metrics. */
 int nfgs_reduce(const nfgs_plan* plan, const double* partials, double* metrics, void* stream);
 
 /* One full plain iteration at mu = 1 from ws->jp (the parallel current of the
  * current state): sources, Phi_c and Az solves into phi_out/az_out (whose
  * fixed planes must already hold Phi_above_bottom, Az_bottom and zeros),
- * reconstruction of B into b_out, and the face pass that leaves the next
- * J_par in ws->jp and the metrics of the new state in ws->metrics.
+ * then nfgs_state_pass, which leaves the next J_par in ws->jp, the metrics
+ * of the new state in ws->metrics and, when b_out is not NULL, B in b_out.
  * Replaces NFGSSolverGPU.iteration_step (nfgs_solver.py:6647-6935) plus the
  * per-iteration monitor calls of the driver (:3694-3702).  With
  * zero_source != 0 the sources are zero (the gs_cfcf start, :6613-6645). */

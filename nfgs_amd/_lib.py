@@ -65,6 +65,7 @@ def _declare(lib):
         "nfgs_dirichlet_solve": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp]),
         "nfgs_reconstruct_b": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp]),
         "nfgs_face_pass": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp]),
+        "nfgs_state_pass": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp]),
         "nfgs_sources": (C.c_int, [vp, vp, vp, vp, vp, vp]),
         "nfgs_reduce": (C.c_int, [vp, vp, vp, vp]),
         "nfgs_advance": (C.c_int, [vp, P(Boundary), P(Workspace), vp, vp, vp, C.c_int, vp]),

@@ -58,6 +58,22 @@ __device__ __forceinline__ T fd1g(G g, int i, int n, T h) {
   return (T(3) * g(0) - T(4) * g(-1) + g(-2)) / (T(2) * h);
 }
 
+// The same derivatives scaled by a precomputed r = 1/(2h) instead of a
+// division by 2h (the hot stencils; differs from the division by at most an
+// ulp per derivative).
+template <typename T>
+__device__ __forceinline__ T fd1r(const T* f, long s, int i, int n, T r) {
+  if (i > 0 && i < n - 1) return (f[s] - f[-s]) * r;
+  if (i == 0) return (T(-3) * f[0] + T(4) * f[s] - f[2 * s]) * r;
+  return (T(3) * f[0] - T(4) * f[-s] + f[-2 * s]) * r;
+}
+template <typename T, typename G>
+__device__ __forceinline__ T fd1gr(G g, int i, int n, T r) {
+  if (i > 0 && i < n - 1) return (g(1) - g(-1)) * r;
+  if (i == 0) return (T(-3) * g(0) + T(4) * g(1) - g(2)) * r;
+  return (T(3) * g(0) - T(4) * g(-1) + g(-2)) * r;
+}
+
 // Mirror-Neumann 5-point lateral Laplacian at (y, x) of a (ny, nx) plane with
 // row stride ld, accumulated in the reference's term order.
 template <typename T>
@@ -102,6 +118,17 @@ __device__ __forceinline__ void block_sum(double (&v)[NQ], double* lds /* [NQ*16
 }
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Workgroup barrier that orders LDS only.  __syncthreads() is a release
+// fence on all address spaces, which waits for every outstanding global load
+// and store (vmcnt(0)) and so drains register prefetches at each barrier.
+// Kernels that never read, within one launch, global memory written by
+// another thread of that launch only need LDS ordering.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 // Workgroups of a launch are dispatched round-robin over the 8 XCDs, each
 // with its own L2.  xcd_tile maps launch-order workgroup w of n to a logical

@@ -23,8 +23,7 @@ extern "C" int nfgs_advance(const nfgs_plan* p, const nfgs_boundary* bnd,
                             const nfgs_workspace* ws, void* phi_out, void* az_out, void* b_out,
                             int zero_source, void* stream) {
   using namespace nfgs;
-  NFGS_REQUIRE(p && bnd && ws && phi_out && az_out && b_out, NFGS_ERR_ARG,
-               "advance: null argument");
+  NFGS_REQUIRE(p && bnd && ws && phi_out && az_out, NFGS_ERR_ARG, "advance: null argument");
   NFGS_REQUIRE(ws->jp && ws->s_bcz && ws->s_jz && ws->hat0 && ws->hat1 && ws->cscr &&
                    ws->partials && ws->metrics,
                NFGS_ERR_ARG, "advance: workspace buffer missing");
@@ -64,9 +63,9 @@ extern "C" int nfgs_advance(const nfgs_plan* p, const nfgs_boundary* bnd,
   NFGS_TRY(launch_transform(p, &p->dst_y, NFGS_INVERSE, at(p, ws->hat1, ip), nxi, 1, ip,
                             at(p, az_out, pl + nx + 1), nx, 1, pl, nxi, nz - 2, s));
 
-  // New state, its face pass (next J_par and the monitors) and the reduction.
-  NFGS_TRY(nfgs_reconstruct_b(p, bnd, phi_out, az_out, b_out, stream));
-  NFGS_TRY(nfgs_face_pass(p, bnd, b_out, az_out, ws->jp, nullptr, ws->partials, stream));
+  // New state: B and its face pass in one z-march (next J_par and the
+  // monitors; B itself only when b_out is given), then the reduction.
+  NFGS_TRY(nfgs_state_pass(p, bnd, phi_out, az_out, b_out, ws->jp, nullptr, ws->partials, stream));
   NFGS_TRY(nfgs_reduce(p, ws->partials, ws->metrics, stream));
   return NFGS_OK;
 }

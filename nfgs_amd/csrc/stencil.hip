@@ -20,7 +20,9 @@ struct Grid {
   long plane, N;
   T dx, dy, dz;
   T idx2, idy2;
+  T i2dx, i2dy, i2dz, idz;  // 1/(2 dx), 1/(2 dy), 1/(2 dz), 1/dz
   int tgx, tgy;  // tiles along x and y (the launch is 1-D over tgx*tgy*z-chunks)
+  int sx, sy;    // tiling shift: 1 when a last tile would be one node wide
 };
 
 template <typename T>
@@ -36,8 +38,17 @@ Grid<T> make_grid(const nfgs_plan* p) {
   g.dz = T(p->d.dz);
   g.idx2 = T(1.0 / (p->d.dx * p->d.dx));
   g.idy2 = T(1.0 / (p->d.dy * p->d.dy));
-  g.tgx = (g.nx + TX - 1) / TX;
-  g.tgy = (g.ny + TY - 1) / TY;
+  g.i2dx = T(1.0 / (2.0 * p->d.dx));
+  g.i2dy = T(1.0 / (2.0 * p->d.dy));
+  g.i2dz = T(1.0 / (2.0 * p->d.dz));
+  g.idz = T(1.0 / p->d.dz);
+  // Tiles start at -s so that no tile is a single node wide (the fused
+  // state pass needs two nodes of a tile for its one-sided end stencils);
+  // the tile count does not change.
+  g.sx = g.nx % TX == 1 ? 1 : 0;
+  g.sy = g.ny % TY == 1 ? 1 : 0;
+  g.tgx = (g.nx + g.sx + TX - 1) / TX;
+  g.tgy = (g.ny + g.sy + TY - 1) / TY;
   return g;
 }
 
@@ -62,40 +73,40 @@ __device__ __forceinline__ Tile make_tile(const Grid<T>& g) {
   Tile t;
   t.id = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = t.id % g.tgx, r = t.id / g.tgx;
-  t.x0 = bx * TX;
-  t.y0 = (r % g.tgy) * TY;
+  t.x0 = bx * TX - g.sx;
+  t.y0 = (r % g.tgy) * TY - g.sy;
   t.k0 = (r / g.tgy) * KCHUNK;
   t.tx = threadIdx.x % TX;
   t.ty = threadIdx.x / TX;
   t.x = t.x0 + t.tx;
   t.y = t.y0 + t.ty;
-  t.valid = t.x < g.nx && t.y < g.ny;
+  t.valid = t.x >= 0 && t.y >= 0 && t.x < g.nx && t.y < g.ny;
   t.c = (t.ty + HH) * PW + t.tx + HH;
   return t;
-}
-
-// in-domain offset of a thread's own point (clamped for threads past the edge)
-template <typename T>
-__device__ __forceinline__ long own_offset(const Tile& t, const Grid<T>& g) {
-  const int x = t.x < g.nx ? t.x : g.nx - 1, y = t.y < g.ny ? t.y : g.ny - 1;
-  return (long)y * g.nx + x;
 }
 
 // Two-step staging of a plane region (tile + halo): fetch() issues the
 // global loads into registers, commit() writes them to an LDS slot later, so
 // the loads of plane p+1 are in flight while plane p is being computed.
 constexpr int PPT = (PS + 255) / 256;  // staged elements per thread (256 threads)
+constexpr int PSA = PPT * 256;         // LDS slot size: every thread stores PPT words
 
 template <typename T>
 struct Fetch {
   T v[PPT];
 };
 
+// In-plane byte offsets of the elements this thread stages, computed once per
+// block.  Halo points outside the domain take a clamped in-domain offset
+// (never used by a valid stencil); slots past PS repeat the last element.
+struct Halo {
+  unsigned off[PPT];
+  unsigned own;  // byte offset of the thread's own point (clamped)
+};
+
 template <typename T>
-__device__ __forceinline__ void fetch(Fetch<T>& r, const T* plane, const Tile& t,
-                                      const Grid<T>& g) {
-  // Unconditional loads from clamped coordinates: halo points outside the
-  // domain get a neighbour's value, which no valid stencil reads.
+__device__ __forceinline__ Halo make_halo(const Tile& t, const Grid<T>& g) {
+  Halo h;
 #pragma unroll
   for (int i = 0; i < PPT; ++i) {
     int e = threadIdx.x + 256 * i;
@@ -104,8 +115,29 @@ __device__ __forceinline__ void fetch(Fetch<T>& r, const T* plane, const Tile& t
     int y = t.y0 + ly - HH, x = t.x0 + lx - HH;
     y = y < 0 ? 0 : (y >= g.ny ? g.ny - 1 : y);
     x = x < 0 ? 0 : (x >= g.nx ? g.nx - 1 : x);
-    r.v[i] = plane[(long)y * g.nx + x];
+    h.off[i] = (unsigned)((y * g.nx + x) * (int)sizeof(T));
   }
+  int x = t.x, y = t.y;
+  x = x < 0 ? 0 : (x >= g.nx ? g.nx - 1 : x);
+  y = y < 0 ? 0 : (y >= g.ny ? g.ny - 1 : y);
+  h.own = (unsigned)((y * g.nx + x) * (int)sizeof(T));
+  return h;
+}
+
+// element at byte offset o of a plane (uniform base + 32-bit lane offset)
+template <typename T>
+__device__ __forceinline__ T ld(const T* base, unsigned o) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + o);
+}
+template <typename T>
+__device__ __forceinline__ void st(T* base, unsigned o, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + o) = v;
+}
+
+template <typename T>
+__device__ __forceinline__ void fetch(Fetch<T>& r, const T* plane, const Halo& h) {
+#pragma unroll
+  for (int i = 0; i < PPT; ++i) r.v[i] = ld(plane, h.off[i]);
 }
 
 template <typename T>
@@ -113,7 +145,7 @@ __device__ __forceinline__ void commit(T* slot, const Fetch<T>& r) {
 #pragma unroll
   for (int i = 0; i < PPT; ++i) {
     const int e = threadIdx.x + 256 * i;
-    if (e < PS) slot[e] = r.v[i];
+    slot[e] = r.v[i];
   }
 }
 
@@ -122,17 +154,17 @@ __device__ __forceinline__ void commit_diff(T* slot, const Fetch<T>& a, const Fe
 #pragma unroll
   for (int i = 0; i < PPT; ++i) {
     const int e = threadIdx.x + 256 * i;
-    if (e < PS) slot[e] = a.v[i] - b.v[i];
+    slot[e] = a.v[i] - b.v[i];
   }
 }
 
 template <typename T>
 __device__ __forceinline__ T lds_dx(const T* s, const Tile& t, const Grid<T>& g) {
-  return fd1(s + t.c, 1, t.x, g.nx, g.dx);
+  return fd1r(s + t.c, 1, t.x, g.nx, g.i2dx);
 }
 template <typename T>
 __device__ __forceinline__ T lds_dy(const T* s, const Tile& t, const Grid<T>& g) {
-  return fd1(s + t.c, (long)PW, t.y, g.ny, g.dy);
+  return fd1r(s + t.c, (long)PW, t.y, g.ny, g.i2dy);
 }
 template <typename T>
 __device__ __forceinline__ T lds_lap(const T* s, const Tile& t, const Grid<T>& g) {
@@ -140,89 +172,102 @@ __device__ __forceinline__ T lds_lap(const T* s, const Tile& t, const Grid<T>& g
                    g.idy2);
 }
 
+// Register set of the loads one node of the reconstruction needs.
+template <typename T>
+struct ReconSet {
+  Fetch<T> phi, az;  // top Phi plane of the node's ring, Az plane
+  T gx, gy, gz;      // B_bg at the thread's own point
+};
+
 // Reconstruction of B from the scalars (nfgs_solver.py:1711-1715, :3311-3341,
 // :5901-5916, :6069-6078).
 template <typename T>
 __global__ __launch_bounds__(256) void reconstruct_b_kernel(
     Grid<T> g, const T* __restrict__ phi, const T* __restrict__ az, const T* __restrict__ bg,
     const T* __restrict__ bz_obs, const T* __restrict__ hprof, T* __restrict__ b) {
-  __shared__ T sphi[3][PS];
-  __shared__ T saz[PS];
-  __shared__ T sdz[PS];
+  __shared__ T sphi[3][PSA];
+  __shared__ T saz[PSA];
+  __shared__ T sdz[PSA];
+  __shared__ T sown[3][256];  // B_bg at the own points of the current node
   const Tile t = make_tile(g);
+  const Halo h = make_halo(t, g);
   const int k0 = t.k0, k1 = min(k0 + KCHUNK, g.nz);
+  auto pl = [&](auto* base, int k) { return base + (long)k * g.plane; };
   // Phi planes lo..lo+2 are resident while node k is built
   auto lo_of = [&](int k) { return max(0, min(k - 1, g.nz - 3)); };
-  const long own = own_offset(t, g);
-  int lo = lo_of(k0);
   {
+    const int lo = lo_of(k0);
     Fetch<T> f0, f1, f2;
-    fetch(f0, phi + (long)lo * g.plane, t, g);
-    fetch(f1, phi + (long)(lo + 1) * g.plane, t, g);
-    fetch(f2, phi + (long)(lo + 2) * g.plane, t, g);
+    fetch(f0, pl(phi, lo), h);
+    fetch(f1, pl(phi, lo + 1), h);
+    fetch(f2, pl(phi, lo + 2), h);
     commit(sphi[lo % 3], f0);
     commit(sphi[(lo + 1) % 3], f1);
     commit(sphi[(lo + 2) % 3], f2);
   }
-  Fetch<T> fphi, faz;
-  fetch(faz, az + (long)k0 * g.plane, t, g);
-  bool have_phi = false;
-  T gx, gy, gz;
-  {
-    const long i = (long)k0 * g.plane + own;
-    gx = bg[i];
-    gy = bg[g.N + i];
-    gz = bg[2 * g.N + i];
-  }
-  for (int k = k0; k < k1; ++k) {
-    if (have_phi) {
-      commit(sphi[(lo + 2) % 3], fphi);
-      have_phi = false;
+  // Loads of node k+2 are issued while node k is computed (two register
+  // sets, alternating), so two planes of loads per block are in flight.
+  // Every step loads (node indices clamped to the grid) so that the wait
+  // counts before each commit are exact.
+  auto fetch_set = [&](ReconSet<T>& q, int k) {
+    const int kc = min(k, g.nz - 1);
+    fetch(q.phi, pl(phi, lo_of(kc) + 2), h);
+    fetch(q.az, pl(az, kc), h);
+    q.gx = ld(pl(bg, kc), h.own);
+    q.gy = ld(pl(bg + g.N, kc), h.own);
+    q.gz = ld(pl(bg + 2 * g.N, kc), h.own);
+  };
+  auto step = [&](ReconSet<T>& q, int k) {
+    const bool act = k < k1;
+    if (act) {
+      if (k > k0 && lo_of(k) > lo_of(k - 1)) commit(sphi[(lo_of(k) + 2) % 3], q.phi);
+      commit(saz, q.az);
+      // own-point values also pass through LDS, so their registers are free
+      // for the next loads as soon as they are committed
+      sown[0][threadIdx.x] = q.gx;
+      sown[1][threadIdx.x] = q.gy;
+      sown[2][threadIdx.x] = q.gz;
     }
-    commit(saz, faz);
-    const T cgx = gx, cgy = gy, cgz = gz;
-    __syncthreads();
-    // prefetch what node k+1 needs
-    if (k + 1 < k1) {
-      const int nlo = lo_of(k + 1);
-      if (nlo > lo) {
-        fetch(fphi, phi + (long)(nlo + 2) * g.plane, t, g);
-        have_phi = true;
-        lo = nlo;
+    lds_barrier();
+    fetch_set(q, k + 2);
+    if (act) {
+      // dPhi/dz on the tile with halo
+#pragma unroll
+      for (int i = 0; i < PPT; ++i) {
+        const int e = threadIdx.x + 256 * i;  // padding slots hold unused values
+        T d;
+        if (k == 0)
+          d = (T(-3) * sphi[0][e] + T(4) * sphi[1][e] - sphi[2][e]) * g.i2dz;
+        else if (k == g.nz - 1)
+          d = (T(3) * sphi[k % 3][e] - T(4) * sphi[(k - 1) % 3][e] + sphi[(k - 2) % 3][e]) *
+              g.i2dz;
+        else
+          d = (sphi[(k + 1) % 3][e] - sphi[(k - 1) % 3][e]) * g.i2dz;
+        sdz[e] = d;
       }
-      fetch(faz, az + (long)(k + 1) * g.plane, t, g);
-      const long i = (long)(k + 1) * g.plane + own;
-      gx = bg[i];
-      gy = bg[g.N + i];
-      gz = bg[2 * g.N + i];
     }
-    // dPhi/dz on the tile with halo
-    for (int e = threadIdx.x; e < PS; e += blockDim.x) {
-      T d;
-      if (k == 0)
-        d = (T(-3) * sphi[0][e] + T(4) * sphi[1][e] - sphi[2][e]) / (T(2) * g.dz);
-      else if (k == g.nz - 1)
-        d = (T(3) * sphi[k % 3][e] - T(4) * sphi[(k - 1) % 3][e] + sphi[(k - 2) % 3][e]) /
-            (T(2) * g.dz);
-      else
-        d = (sphi[(k + 1) % 3][e] - sphi[(k - 1) % 3][e]) / (T(2) * g.dz);
-      sdz[e] = d;
-    }
-    __syncthreads();
-    if (t.valid) {
-      const long i = (long)k * g.plane + own;
+    lds_barrier();
+    if (act && t.valid) {
+      const T cgx = sown[0][threadIdx.x], cgy = sown[1][threadIdx.x], cgz = sown[2][threadIdx.x];
       const T bcx = -lds_dx(sdz, t, g) + lds_dy(saz, t, g);
       const T bcy = -lds_dy(sdz, t, g) - lds_dx(saz, t, g);
       const T bcz = lds_lap(sphi[k % 3], t, g);
       T bx = cgx + bcx;
       T by = cgy + bcy;
       T bz = cgz + bcz;
-      if (k == 0) bz = bz_obs[own];
-      b[i] = bx + hprof[k];
-      b[g.N + i] = by + hprof[g.nz + k];
-      b[2 * g.N + i] = bz;
+      if (k == 0) bz = ld(bz_obs, h.own);
+      st(pl(b, k), h.own, bx + hprof[k]);
+      st(pl(b + g.N, k), h.own, by + hprof[g.nz + k]);
+      st(pl(b + 2 * g.N, k), h.own, bz);
     }
-    __syncthreads();
+    lds_barrier();
+  };
+  ReconSet<T> sa, sb;
+  fetch_set(sa, k0);
+  fetch_set(sb, k0 + 1);
+  for (int k = k0; k < k1; k += 2) {
+    step(sa, k);
+    step(sb, k + 1);
   }
 }
 
@@ -242,35 +287,41 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
     Grid<T> g, const T* __restrict__ B, const T* __restrict__ bg, const T* __restrict__ az,
     const T* __restrict__ jzt, T* __restrict__ jp, T* __restrict__ jout, double* partials,
     long pstride, T floor_b2, T amax, Window w) {
-  __shared__ T scz[2][PS];
-  __shared__ T sbx[2][PS];
-  __shared__ T sby[2][PS];
-  __shared__ T saz[PS];
+  __shared__ T scz[2][PSA];
+  __shared__ T sbx[2][PSA];
+  __shared__ T sby[2][PSA];
+  __shared__ T saz[PSA];
+  __shared__ T sown[3][256];  // Bz, B_bg,x, B_bg,y at the own points of plane p
   __shared__ double red[NFGS_NQ * 16];
   const Tile t = make_tile(g);
+  const Halo h = make_halo(t, g);
   const int k0 = t.k0, k1 = min(k0 + KCHUNK, g.nz);
   const int ps = max(k0 - 1, 0), pe = min(k1, g.nz - 1);
-  const long off = own_offset(t, g);
+  auto pl = [&](auto* base, int k) { return base + (long)k * g.plane; };
   const bool edge = t.y == 0 || t.y == g.ny - 1 || t.x == 0 || t.x == g.nx - 1;
   const bool in_win = t.y >= w.y0 && t.y < w.y1 && t.x >= w.x0 && t.x < w.x1;
   const T* Bx = B;
   const T* By = B + g.N;
   const T* Bz = B + 2 * g.N;
+  const T jz0 = ld(jzt, h.own);  // toroidal bottom current at the own point
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  T pb[3] = {0, 0, 0}, pg[2] = {0, 0}, pjz = 0;  // previous plane, own point
+  // previous plane at the own point; pz2 = Bz two planes below
+  T pb[3] = {0, 0, 0}, pg[2] = {0, 0}, pjz = 0, pz2 = 0;
   FaceVal<T> fprev;
   Fetch<T> fbz, fgz, fbx, fby, faz;
-  T obz = 0, ogx = 0, ogy = 0;  // own Bz, bg_x, bg_y of the fetched plane
+  T obz, ogx, ogy;  // own Bz, bg_x, bg_y of the fetched plane
+  // Every iteration loads a plane (clamped to pe) so the waits before each
+  // commit are exact.
   auto fetch_plane = [&](int q) {
-    const long qo = (long)q * g.plane;
-    fetch(fbz, Bz + qo, t, g);
-    fetch(fgz, bg + 2 * g.N + qo, t, g);
-    fetch(fbx, Bx + qo, t, g);
-    fetch(fby, By + qo, t, g);
-    fetch(faz, az + qo, t, g);
-    obz = Bz[qo + off];
-    ogx = bg[qo + off];
-    ogy = bg[g.N + qo + off];
+    q = min(q, pe);
+    fetch(fbz, pl(Bz, q), h);
+    fetch(fgz, pl(bg + 2 * g.N, q), h);
+    fetch(fbx, pl(Bx, q), h);
+    fetch(fby, pl(By, q), h);
+    fetch(faz, pl(az, q), h);
+    obz = ld(pl(Bz, q), h.own);
+    ogx = ld(pl(bg, q), h.own);
+    ogy = ld(pl(bg + g.N, q), h.own);
   };
   fetch_plane(ps);
   for (int p = ps; p <= pe; ++p) {
@@ -278,9 +329,12 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
     commit(sbx[p & 1], fbx);
     commit(sby[p & 1], fby);
     commit(saz, faz);
-    const T cbz = obz, cgx = ogx, cgy = ogy;
-    __syncthreads();
-    if (p + 1 <= pe) fetch_plane(p + 1);
+    sown[0][threadIdx.x] = obz;
+    sown[1][threadIdx.x] = ogx;
+    sown[2][threadIdx.x] = ogy;
+    lds_barrier();
+    fetch_plane(p + 1);
+    const T cbz = sown[0][threadIdx.x], cgx = sown[1][threadIdx.x], cgy = sown[2][threadIdx.x];
     T cb[3] = {0, 0, 0}, cg[2] = {0, 0}, cjz = 0;
     if (t.valid) {
       cb[0] = sbx[p & 1][t.c];
@@ -289,7 +343,7 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
       cg[0] = cgx;
       cg[1] = cgy;
       if (p == 0)
-        cjz = jzt[off];
+        cjz = jz0;
       else if (p == g.nz - 1 || edge)
         cjz = T(0);
       else
@@ -301,13 +355,13 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
       F.b[0] = T(0.5) * (cb[0] + pb[0]);
       F.b[1] = T(0.5) * (cb[1] + pb[1]);
       F.b[2] = T(0.5) * (cb[2] + pb[2]);
-      const T dbx = ((cb[0] - cg[0]) - (pb[0] - pg[0])) / g.dz;
-      const T dby = ((cb[1] - cg[1]) - (pb[1] - pg[1])) / g.dz;
+      const T dbx = ((cb[0] - cg[0]) - (pb[0] - pg[0])) * g.idz;
+      const T dby = ((cb[1] - cg[1]) - (pb[1] - pg[1])) * g.idz;
       const T* s1 = scz[p & 1];
       const T* s0 = scz[f & 1];
       auto cfz = [&](int o) { return T(0.5) * (s1[t.c + o] + s0[t.c + o]); };
-      const T dyc = fd1g<T>([&](int o) { return cfz(o * PW); }, t.y, g.ny, g.dy);
-      const T dxc = fd1g<T>([&](int o) { return cfz(o); }, t.x, g.nx, g.dx);
+      const T dyc = fd1gr<T>([&](int o) { return cfz(o * PW); }, t.y, g.ny, g.i2dy);
+      const T dxc = fd1gr<T>([&](int o) { return cfz(o); }, t.x, g.nx, g.i2dx);
       F.j[0] = dyc - dby;
       F.j[1] = dbx - dxc;
       F.j[2] = T(0.5) * (cjz + pjz);
@@ -323,16 +377,15 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
       }
       const int k = f;  // node finalised now
       if (k >= k0 && k < k1) {
-        const long i = (long)k * g.plane + off;
         if (jp) {
           if (k == 0) {
-            jp[i] = edge ? T(0) : F.j[0];
-            jp[g.N + i] = edge ? T(0) : F.j[1];
-            jp[2 * g.N + i] = jzt[off];
+            st(pl(jp, k), h.own, edge ? T(0) : F.j[0]);
+            st(pl(jp + g.N, k), h.own, edge ? T(0) : F.j[1]);
+            st(pl(jp + 2 * g.N, k), h.own, jz0);
           } else {
-            jp[i] = T(0.5) * (fprev.p[0] + F.p[0]);
-            jp[g.N + i] = T(0.5) * (fprev.p[1] + F.p[1]);
-            jp[2 * g.N + i] = T(0.5) * (fprev.p[2] + F.p[2]);
+            st(pl(jp, k), h.own, T(0.5) * (fprev.p[0] + F.p[0]));
+            st(pl(jp + g.N, k), h.own, T(0.5) * (fprev.p[1] + F.p[1]));
+            st(pl(jp + 2 * g.N, k), h.own, T(0.5) * (fprev.p[2] + F.p[2]));
           }
         }
         if (jout) {
@@ -346,9 +399,9 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
           }
           acc[6] += (double)c[0] * c[0] + (double)c[1] * c[1] + (double)c[2] * c[2];
           acc[5] += (double)F.j[0] * F.j[0] + (double)F.j[1] * F.j[1] + (double)F.j[2] * F.j[2];
-          jout[i] = edge ? T(0) : c[0];
-          jout[g.N + i] = edge ? T(0) : c[1];
-          jout[2 * g.N + i] = pjz;
+          st(pl(jout, k), h.own, edge ? T(0) : c[0]);
+          st(pl(jout + g.N, k), h.own, edge ? T(0) : c[1]);
+          st(pl(jout + 2 * g.N, k), h.own, pjz);
         }
         if (partials) {
           if (k >= w.z0 && k < w.z1 && in_win) {
@@ -363,7 +416,13 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
           }
           const T dvx = lds_dx(sbx[k & 1], t, g);
           const T dvy = lds_dy(sby[k & 1], t, g);
-          const T dvz = fd1(Bz + i, g.plane, k, g.nz, g.dz);
+          // d(Bz)/dz at node k from the planes held in registers (the
+          // bottom node also needs plane 2)
+          T dvz;
+          if (k == 0)
+            dvz = (T(-3) * pb[2] + T(4) * cb[2] - ld(pl(Bz, 2), h.own)) * g.i2dz;
+          else
+            dvz = (cb[2] - pz2) * g.i2dz;
           const T div = dvx + dvy + dvz;
           const T b2 = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
           acc[2] += (double)(div < T(0) ? -div : div);
@@ -373,31 +432,32 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
       }
       fprev = F;
     }
+    pz2 = pb[2];
     pb[0] = cb[0]; pb[1] = cb[1]; pb[2] = cb[2];
     pg[0] = cg[0]; pg[1] = cg[1];
     pjz = cjz;
-    __syncthreads();
+    lds_barrier();
   }
   // top node: no face above it
   const int kt = g.nz - 1;
   if (k1 == g.nz && t.valid) {
-    const long i = (long)kt * g.plane + off;
     if (jp) {
-      jp[i] = fprev.p[0];
-      jp[g.N + i] = fprev.p[1];
-      jp[2 * g.N + i] = fprev.p[2];
+      st(pl(jp, kt), h.own, fprev.p[0]);
+      st(pl(jp + g.N, kt), h.own, fprev.p[1]);
+      st(pl(jp + 2 * g.N, kt), h.own, fprev.p[2]);
     }
     if (jout) {
       acc[6] += (double)fprev.j[0] * fprev.j[0] + (double)fprev.j[1] * fprev.j[1] +
                 (double)fprev.j[2] * fprev.j[2];
-      jout[i] = T(0);
-      jout[g.N + i] = T(0);
-      jout[2 * g.N + i] = T(0);
+      st(pl(jout, kt), h.own, T(0));
+      st(pl(jout + g.N, kt), h.own, T(0));
+      st(pl(jout + 2 * g.N, kt), h.own, T(0));
     }
     if (partials) {
       const T dvx = lds_dx(sbx[kt & 1], t, g);
       const T dvy = lds_dy(sby[kt & 1], t, g);
-      const T dvz = fd1(Bz + i, g.plane, kt, g.nz, g.dz);
+      // one-sided d(Bz)/dz: planes kt (pb), kt-1 (pz2) and kt-2
+      const T dvz = (T(3) * pb[2] - T(4) * pz2 + ld(pl(Bz, kt - 2), h.own)) * g.i2dz;
       const T div = dvx + dvy + dvz;
       const T b2 = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
       acc[2] += (double)(div < T(0) ? -div : div);
@@ -419,55 +479,355 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused state pass: reconstruction of B (nfgs_solver.py:1711-1715,
+// :3311-3341, :5901-5916, :6069-6078) and the vertical-face pass
+// (:5738-5839, :5882-5895, :2689-2697, :5917-5941, :6702-6710) in one z-march.
+// B of plane p is built in LDS on the tile plus a one-node ring (R1) from
+// Phi_c and Az staged with a two-node ring, then consumed by the face pass of
+// the same block; B reaches HBM only when b_out is given.  Arithmetic is the
+// same, operation by operation, as reconstruct_b_kernel followed by
+// face_pass_kernel.
+// ---------------------------------------------------------------------------
+constexpr int R1W = PW - 2, R1H = PH - 2, R1N = R1W * R1H;  // ring-1 region
+constexpr int R1PT = (R1N + 255) / 256;
+
+template <typename T>
+struct StateSet {
+  Fetch<T> phi, az, gx, gy;  // top Phi plane of the ring, Az, B_bg x and y
+  T gz;                      // B_bg z at the own point
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void state_pass_kernel(
+    Grid<T> g, const T* __restrict__ phi, const T* __restrict__ az, const T* __restrict__ bg,
+    const T* __restrict__ bz_obs, const T* __restrict__ hprof, const T* __restrict__ jzt,
+    T* __restrict__ bout, T* __restrict__ jp, T* __restrict__ jout, double* partials,
+    long pstride, T floor_b2, T amax, Window w) {
+  __shared__ T sphi[3][PSA];
+  __shared__ T saz[2][PSA];
+  __shared__ T sgx[2][PSA];
+  __shared__ T sgy[2][PSA];
+  __shared__ T sdz[PSA];
+  __shared__ T sbx[2][PSA];
+  __shared__ T sby[2][PSA];
+  __shared__ T scz[2][PSA];  // Bz - B_bg,z on R1
+  __shared__ T sgz[2][256];
+  __shared__ double red[NFGS_NQ * 16];
+  const Tile t = make_tile(g);
+  const Halo h = make_halo(t, g);
+  const int k0 = t.k0, k1 = min(k0 + KCHUNK, g.nz);
+  const int ps = max(k0 - 1, 0), pe = min(k1, g.nz - 1);
+  auto pl = [&](auto* base, int k) { return base + (long)k * g.plane; };
+  auto lo_of = [&](int k) { return max(0, min(k - 1, g.nz - 3)); };
+  const bool edge = t.y == 0 || t.y == g.ny - 1 || t.x == 0 || t.x == g.nx - 1;
+  const bool in_win = t.y >= w.y0 && t.y < w.y1 && t.x >= w.x0 && t.x < w.x1;
+  const T jz0 = ld(jzt, h.own);
+
+  // ring of Phi planes lo..lo+2 for the first plane
+  {
+    const int lo = lo_of(ps);
+    Fetch<T> f0, f1, f2;
+    fetch(f0, pl(phi, lo), h);
+    fetch(f1, pl(phi, lo + 1), h);
+    fetch(f2, pl(phi, lo + 2), h);
+    commit(sphi[lo % 3], f0);
+    commit(sphi[(lo + 1) % 3], f1);
+    commit(sphi[(lo + 2) % 3], f2);
+  }
+  auto fetch_set = [&](StateSet<T>& q, int p) {
+    const int pc = min(p, pe);
+    fetch(q.phi, pl(phi, lo_of(pc) + 2), h);
+    fetch(q.az, pl(az, pc), h);
+    fetch(q.gx, pl(bg, pc), h);
+    fetch(q.gy, pl(bg + g.N, pc), h);
+    q.gz = ld(pl(bg + 2 * g.N, pc), h.own);
+  };
+
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  T pb[3] = {0, 0, 0}, pg[2] = {0, 0}, pjz = 0, pz2 = 0, pz3 = 0;
+  FaceVal<T> fprev;
+
+  auto step = [&](StateSet<T>& q, int p) {
+    const bool act = p <= pe;
+    const int sl = p & 1;
+    if (act) {
+      if (p > ps && lo_of(p) > lo_of(p - 1)) commit(sphi[(lo_of(p) + 2) % 3], q.phi);
+      commit(saz[sl], q.az);
+      commit(sgx[sl], q.gx);
+      commit(sgy[sl], q.gy);
+      sgz[sl][threadIdx.x] = q.gz;
+    }
+    lds_barrier();
+    fetch_set(q, p + 1);
+    if (act) {
+      // (A) dPhi/dz on every staged point
+#pragma unroll
+      for (int i = 0; i < PPT; ++i) {
+        const int e = threadIdx.x + 256 * i;
+        T d;
+        if (p == 0)
+          d = (T(-3) * sphi[0][e] + T(4) * sphi[1][e] - sphi[2][e]) * g.i2dz;
+        else if (p == g.nz - 1)
+          d = (T(3) * sphi[p % 3][e] - T(4) * sphi[(p - 1) % 3][e] + sphi[(p - 2) % 3][e]) * g.i2dz;
+        else
+          d = (sphi[(p + 1) % 3][e] - sphi[(p - 1) % 3][e]) * g.i2dz;
+        sdz[e] = d;
+      }
+    }
+    lds_barrier();
+    if (act) {
+      // (B) B on R1; points outside the grid hold 0 and are never read
+      const T hx = hprof[p], hy = hprof[g.nz + p];
+#pragma unroll
+      for (int i = 0; i < R1PT; ++i) {
+        const int r = threadIdx.x + 256 * i;
+        if (r < R1N) {
+          const int ly = 1 + r / R1W, lx = 1 + r % R1W;
+          const int e = ly * PW + lx;
+          const int y = t.y0 + ly - HH, x = t.x0 + lx - HH;
+          T vbx = T(0), vby = T(0), vcz = T(0);
+          if (x >= 0 && x < g.nx && y >= 0 && y < g.ny) {
+            const T dxd = fd1r(sdz + e, 1, x, g.nx, g.i2dx);
+            const T dyd = fd1r(sdz + e, (long)PW, y, g.ny, g.i2dy);
+            const T dxa = fd1r(saz[sl] + e, 1, x, g.nx, g.i2dx);
+            const T dya = fd1r(saz[sl] + e, (long)PW, y, g.ny, g.i2dy);
+            const T bcx = -dxd + dya;
+            const T bcy = -dyd - dxa;
+            vbx = (sgx[sl][e] + bcx) + hx;
+            vby = (sgy[sl][e] + bcy) + hy;
+            if (p == 0) {
+              const unsigned o = (unsigned)((y * g.nx + x) * (int)sizeof(T));
+              vcz = ld(bz_obs, o) - ld(bg + 2 * g.N, o);
+            } else {
+              vcz = lap_xy_at(sphi[p % 3] + e - (y * PW + x), y, x, g.ny, g.nx, (long)PW, g.idx2,
+                              g.idy2);
+            }
+          }
+          sbx[sl][e] = vbx;
+          sby[sl][e] = vby;
+          scz[sl][e] = vcz;
+        }
+      }
+    }
+    lds_barrier();
+    if (!act) return;
+    // (C) face pass at the own point
+    T cb[3] = {0, 0, 0}, cg[2] = {0, 0}, cjz = 0;
+    if (t.valid) {
+      cb[0] = sbx[sl][t.c];
+      cb[1] = sby[sl][t.c];
+      cb[2] = p == 0 ? ld(bz_obs, h.own) : sgz[sl][threadIdx.x] + scz[sl][t.c];
+      cg[0] = sgx[sl][t.c];
+      cg[1] = sgy[sl][t.c];
+      if (p == 0)
+        cjz = jz0;
+      else if (p == g.nz - 1 || edge)
+        cjz = T(0);
+      else
+        cjz = -lap_xy_at(saz[sl] + t.c - (t.y * PW + t.x), t.y, t.x, g.ny, g.nx, (long)PW,
+                         g.idx2, g.idy2);
+      if (bout && p >= k0 && p < k1) {
+        st(pl(bout, p), h.own, cb[0]);
+        st(pl(bout + g.N, p), h.own, cb[1]);
+        st(pl(bout + 2 * g.N, p), h.own, cb[2]);
+      }
+    }
+    if (p > ps && t.valid) {
+      const int f = p - 1;  // face between planes f and p
+      FaceVal<T> F;
+      F.b[0] = T(0.5) * (cb[0] + pb[0]);
+      F.b[1] = T(0.5) * (cb[1] + pb[1]);
+      F.b[2] = T(0.5) * (cb[2] + pb[2]);
+      const T dbx = ((cb[0] - cg[0]) - (pb[0] - pg[0])) * g.idz;
+      const T dby = ((cb[1] - cg[1]) - (pb[1] - pg[1])) * g.idz;
+      const T* s1 = scz[sl];
+      const T* s0 = scz[f & 1];
+      auto cfz = [&](int o) { return T(0.5) * (s1[t.c + o] + s0[t.c + o]); };
+      const T dyc = fd1gr<T>([&](int o) { return cfz(o * PW); }, t.y, g.ny, g.i2dy);
+      const T dxc = fd1gr<T>([&](int o) { return cfz(o); }, t.x, g.nx, g.i2dx);
+      F.j[0] = dyc - dby;
+      F.j[1] = dbx - dxc;
+      F.j[2] = T(0.5) * (cjz + pjz);
+      {
+        T b2 = F.b[0] * F.b[0] + F.b[1] * F.b[1] + F.b[2] * F.b[2];
+        b2 = b2 < floor_b2 ? floor_b2 : b2;
+        const T jb = F.j[0] * F.b[0] + F.j[1] * F.b[1] + F.j[2] * F.b[2];
+        T a = jb / b2;
+        a = a < -amax ? -amax : (a > amax ? amax : a);
+        F.p[0] = a * F.b[0];
+        F.p[1] = a * F.b[1];
+        F.p[2] = a * F.b[2];
+      }
+      const int k = f;  // node finalised now
+      if (k >= k0 && k < k1) {
+        if (jp) {
+          if (k == 0) {
+            st(pl(jp, k), h.own, edge ? T(0) : F.j[0]);
+            st(pl(jp + g.N, k), h.own, edge ? T(0) : F.j[1]);
+            st(pl(jp + 2 * g.N, k), h.own, jz0);
+          } else {
+            st(pl(jp, k), h.own, T(0.5) * (fprev.p[0] + F.p[0]));
+            st(pl(jp + g.N, k), h.own, T(0.5) * (fprev.p[1] + F.p[1]));
+            st(pl(jp + 2 * g.N, k), h.own, T(0.5) * (fprev.p[2] + F.p[2]));
+          }
+        }
+        if (jout) {
+          T c[3];
+          if (k == 0) {
+            c[0] = F.j[0]; c[1] = F.j[1]; c[2] = F.j[2];
+          } else {
+            c[0] = T(0.5) * (fprev.j[0] + F.j[0]);
+            c[1] = T(0.5) * (fprev.j[1] + F.j[1]);
+            c[2] = T(0.5) * (fprev.j[2] + F.j[2]);
+          }
+          acc[6] += (double)c[0] * c[0] + (double)c[1] * c[1] + (double)c[2] * c[2];
+          acc[5] += (double)F.j[0] * F.j[0] + (double)F.j[1] * F.j[1] + (double)F.j[2] * F.j[2];
+          st(pl(jout, k), h.own, edge ? T(0) : c[0]);
+          st(pl(jout + g.N, k), h.own, edge ? T(0) : c[1]);
+          st(pl(jout + 2 * g.N, k), h.own, pjz);
+        }
+        if (partials) {
+          if (k >= w.z0 && k < w.z1 && in_win) {
+            T bn = sqrt(F.b[0] * F.b[0] + F.b[1] * F.b[1] + F.b[2] * F.b[2]);
+            bn = bn < T(1e-10) ? T(1e-10) : bn;
+            const T jn = sqrt(F.j[0] * F.j[0] + F.j[1] * F.j[1] + F.j[2] * F.j[2]);
+            const T c0 = F.j[1] * F.b[2] - F.j[2] * F.b[1];
+            const T c1 = F.j[2] * F.b[0] - F.j[0] * F.b[2];
+            const T c2 = F.j[0] * F.b[1] - F.j[1] * F.b[0];
+            acc[0] += (double)(sqrt(c0 * c0 + c1 * c1 + c2 * c2) / bn);
+            acc[1] += (double)jn;
+          }
+          const T dvx = fd1r(sbx[k & 1] + t.c, 1, t.x, g.nx, g.i2dx);
+          const T dvy = fd1r(sby[k & 1] + t.c, (long)PW, t.y, g.ny, g.i2dy);
+          T dvz;
+          if (k == 0) {
+            // Bz of plane 2 = B_bg,z + lap_xy(Phi_c) there (Phi ring holds 0..2)
+            const T bz2 = ld(pl(bg + 2 * g.N, 2), h.own) +
+                          lap_xy_at(sphi[2] + t.c - (t.y * PW + t.x), t.y, t.x, g.ny, g.nx,
+                                    (long)PW, g.idx2, g.idy2);
+            dvz = (T(-3) * pb[2] + T(4) * cb[2] - bz2) * g.i2dz;
+          } else {
+            dvz = (cb[2] - pz2) * g.i2dz;
+          }
+          const T div = dvx + dvy + dvz;
+          const T b2 = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
+          acc[2] += (double)(div < T(0) ? -div : div);
+          acc[3] += (double)sqrt(b2);
+          acc[4] += (double)b2;
+        }
+      }
+      fprev = F;
+    }
+    pz3 = pz2;
+    pz2 = pb[2];
+    pb[0] = cb[0]; pb[1] = cb[1]; pb[2] = cb[2];
+    pg[0] = cg[0]; pg[1] = cg[1];
+    pjz = cjz;
+  };
+  StateSet<T> sa;
+  fetch_set(sa, ps);
+  for (int p = ps; p <= pe; ++p) step(sa, p);
+  // top node: no face above it (planes kt, kt-1, kt-2 of Bz are pb, pz2, pz3)
+  const int kt = g.nz - 1;
+  if (k1 == g.nz && t.valid) {
+    if (jp) {
+      st(pl(jp, kt), h.own, fprev.p[0]);
+      st(pl(jp + g.N, kt), h.own, fprev.p[1]);
+      st(pl(jp + 2 * g.N, kt), h.own, fprev.p[2]);
+    }
+    if (jout) {
+      acc[6] += (double)fprev.j[0] * fprev.j[0] + (double)fprev.j[1] * fprev.j[1] +
+                (double)fprev.j[2] * fprev.j[2];
+      st(pl(jout, kt), h.own, T(0));
+      st(pl(jout + g.N, kt), h.own, T(0));
+      st(pl(jout + 2 * g.N, kt), h.own, T(0));
+    }
+    if (partials) {
+      const T dvx = fd1r(sbx[kt & 1] + t.c, 1, t.x, g.nx, g.i2dx);
+      const T dvy = fd1r(sby[kt & 1] + t.c, (long)PW, t.y, g.ny, g.i2dy);
+      const T dvz = (T(3) * pb[2] - T(4) * pz2 + pz3) * g.i2dz;
+      const T div = dvx + dvy + dvz;
+      const T b2 = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
+      acc[2] += (double)(div < T(0) ? -div : div);
+      acc[3] += (double)sqrt(b2);
+      acc[4] += (double)b2;
+    }
+  }
+  if (!partials) return;
+  block_sum<7>(acc, red);
+  if (threadIdx.x == 0) {
+    const long bid = t.id;
+    partials[NFGS_Q_SIGMA_NUM * pstride + bid] = acc[0];
+    partials[NFGS_Q_SIGMA_DEN * pstride + bid] = acc[1];
+    partials[NFGS_Q_DIV_ABS * pstride + bid] = acc[2];
+    partials[NFGS_Q_BMAG * pstride + bid] = acc[3];
+    partials[NFGS_Q_B2 * pstride + bid] = acc[4];
+    partials[NFGS_Q_FACE_J2 * pstride + bid] = acc[5];
+    partials[NFGS_Q_CELL_J2 * pstride + bid] = acc[6];
+  }
+}
+
+// Register set of the loads one node of the sources needs.
+template <typename T>
+struct SourceSet {
+  Fetch<T> x, y, z;  // J_x, J_y of plane k+1; J_z of plane k
+};
+
 // Sources of the two Poisson problems (nfgs_solver.py:6726-6745, :1729-1749).
 template <typename T>
 __global__ __launch_bounds__(256) void sources_kernel(Grid<T> g, const T* __restrict__ jp,
                                                       T* __restrict__ sb, T* __restrict__ sj,
                                                       double* partials, long pstride, T i4dxdz,
                                                       T i4dydz) {
-  __shared__ T sjx[3][PS];
-  __shared__ T sjy[3][PS];
-  __shared__ T sjz[PS];
+  __shared__ T sjx[3][PSA];
+  __shared__ T sjy[3][PSA];
+  __shared__ T sjz[PSA];
   __shared__ double red[NFGS_NQ * 16];
   const Tile t = make_tile(g);
+  const Halo h = make_halo(t, g);
   const int k0 = t.k0, k1 = min(k0 + KCHUNK, g.nz);
+  auto pl = [&](auto* base, int k) { return base + (long)k * g.plane; };
   const T* jx = jp;
   const T* jy = jp + g.N;
   const T* jzp = jp + 2 * g.N;
   const int nxi = g.nx - 2;
   const long ip = (long)(g.ny - 2) * nxi;
   const bool interior = t.y >= 1 && t.y <= g.ny - 2 && t.x >= 1 && t.x <= g.nx - 2;
+  // byte offset of the own point in the compact interior planes of S_Jz
+  const unsigned sjo = interior ? (unsigned)(((t.y - 1) * nxi + (t.x - 1)) * (int)sizeof(T)) : 0u;
   double acc[1] = {0.0};
-  Fetch<T> fx, fy, fz;
   for (int q = max(k0 - 1, 0); q <= k0; ++q) {
-    fetch(fx, jx + (long)q * g.plane, t, g);
-    fetch(fy, jy + (long)q * g.plane, t, g);
+    Fetch<T> fx, fy;
+    fetch(fx, pl(jx, q), h);
+    fetch(fy, pl(jy, q), h);
     commit(sjx[q % 3], fx);
     commit(sjy[q % 3], fy);
   }
-  if (k0 + 1 <= g.nz - 1) {
-    fetch(fx, jx + (long)(k0 + 1) * g.plane, t, g);
-    fetch(fy, jy + (long)(k0 + 1) * g.plane, t, g);
-  }
-  fetch(fz, jzp + (long)k0 * g.plane, t, g);
-  for (int k = k0; k < k1; ++k) {
-    if (k + 1 <= g.nz - 1) {
-      commit(sjx[(k + 1) % 3], fx);
-      commit(sjy[(k + 1) % 3], fy);
-    }
-    commit(sjz, fz);
-    __syncthreads();
-    if (k + 1 < k1) {
-      if (k + 2 <= g.nz - 1) {
-        fetch(fx, jx + (long)(k + 2) * g.plane, t, g);
-        fetch(fy, jy + (long)(k + 2) * g.plane, t, g);
+  // Node k needs J_x, J_y of plane k+1 and J_z of plane k; the loads of node
+  // k+2 are issued while node k is computed (two register sets).  Every step
+  // loads (planes clamped to the grid) so the waits before each commit are
+  // exact.
+  auto fetch_set = [&](SourceSet<T>& q, int k) {
+    const int kx = min(k + 1, g.nz - 1), kz = min(k, g.nz - 1);
+    fetch(q.x, pl(jx, kx), h);
+    fetch(q.y, pl(jy, kx), h);
+    fetch(q.z, pl(jzp, kz), h);
+  };
+  auto step = [&](SourceSet<T>& q, int k) {
+    const bool act = k < k1;
+    if (act) {
+      if (k + 1 <= g.nz - 1) {
+        commit(sjx[(k + 1) % 3], q.x);
+        commit(sjy[(k + 1) % 3], q.y);
       }
-      fetch(fz, jzp + (long)(k + 1) * g.plane, t, g);
+      commit(sjz, q.z);
     }
-    if (t.valid) {
-      const long i = (long)k * g.plane + (long)t.y * g.nx + t.x;
+    lds_barrier();
+    fetch_set(q, k + 2);
+    if (act && t.valid) {
       const T s = lds_dy(sjx[k % 3], t, g) - lds_dx(sjy[k % 3], t, g);
-      sb[i] = s;
+      st(pl(sb, k), h.own, s);
       acc[0] += (double)s * s;
       if (interior) {
         T v = lds_lap(sjz, t, g);
@@ -480,10 +840,17 @@ __global__ __launch_bounds__(256) void sources_kernel(Grid<T> g, const T* __rest
           const T dyz = (yu[PW] - yd[PW] - yu[-PW] + yd[-PW]) * i4dydz;
           v = v - dxz - dyz;
         }
-        sj[(long)k * ip + (long)(t.y - 1) * nxi + (t.x - 1)] = v;
+        st(sj + (long)k * ip, sjo, v);
       }
     }
-    __syncthreads();
+    lds_barrier();
+  };
+  SourceSet<T> sa, sb2;
+  fetch_set(sa, k0);
+  fetch_set(sb2, k0 + 1);
+  for (int k = k0; k < k1; k += 2) {
+    step(sa, k);
+    step(sb2, k + 1);
   }
   if (!partials) return;
   block_sum<1>(acc, red);
@@ -518,7 +885,8 @@ __global__ void fill_kernel(T* __restrict__ p, long n, T v) {
 }
 
 dim3 tiled_grid(const nfgs_plan* p) {
-  return dim3((unsigned)(((p->d.nx + TX - 1) / TX) * ((p->d.ny + TY - 1) / TY) *
+  const int sx = p->d.nx % TX == 1 ? 1 : 0, sy = p->d.ny % TY == 1 ? 1 : 0;
+  return dim3((unsigned)(((p->d.nx + sx + TX - 1) / TX) * ((p->d.ny + sy + TY - 1) / TY) *
                          ((p->d.nz + KCHUNK - 1) / KCHUNK)));
 }
 
@@ -543,6 +911,21 @@ int face_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* b, const vo
                      static_cast<const T*>(az), static_cast<const T*>(bnd->jz_t),
                      static_cast<T*>(jp), static_cast<T*>(jout), partials, (long)p->pblocks,
                      T(p->d.weak_field_floor_b2), T(p->d.alpha_max), w);
+  NFGS_LAUNCH_CHECK();
+  return NFGS_OK;
+}
+
+template <typename T>
+int state_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi, const void* az,
+            void* b, void* jp, void* jout, double* partials, hipStream_t s) {
+  const Grid<T> g = make_grid<T>(p);
+  const Window w{p->d.win_z0, p->d.win_z1, p->d.win_y0, p->d.win_y1, p->d.win_x0, p->d.win_x1};
+  hipLaunchKernelGGL((state_pass_kernel<T>), tiled_grid(p), dim3(TX * TY), 0, s, g,
+                     static_cast<const T*>(phi), static_cast<const T*>(az),
+                     static_cast<const T*>(bnd->b_bg), static_cast<const T*>(bnd->bz_obs),
+                     static_cast<const T*>(bnd->h_profile), static_cast<const T*>(bnd->jz_t),
+                     static_cast<T*>(b), static_cast<T*>(jp), static_cast<T*>(jout), partials,
+                     (long)p->pblocks, T(p->d.weak_field_floor_b2), T(p->d.alpha_max), w);
   NFGS_LAUNCH_CHECK();
   return NFGS_OK;
 }
@@ -610,6 +993,18 @@ extern "C" int nfgs_face_pass(const nfgs_plan* p, const nfgs_boundary* bnd, cons
   if (p->d.dtype == NFGS_F32)
     return face_t<float>(p, bnd, b, az, jp_out, j_out, partials, as_stream(stream));
   return face_t<double>(p, bnd, b, az, jp_out, j_out, partials, as_stream(stream));
+}
+
+extern "C" int nfgs_state_pass(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi,
+                               const void* az, void* b_out, void* jp_out, void* j_out,
+                               double* partials, void* stream) {
+  NFGS_REQUIRE(p && bnd && phi && az, NFGS_ERR_ARG, "state_pass: null argument");
+  NFGS_REQUIRE(p->d.nz >= 3, NFGS_ERR_ARG, "state_pass: nz must be >= 3");
+  NFGS_REQUIRE(bnd->b_bg && bnd->bz_obs && bnd->h_profile && bnd->jz_t, NFGS_ERR_STATE,
+               "state_pass: boundary is not set");
+  if (p->d.dtype == NFGS_F32)
+    return state_t<float>(p, bnd, phi, az, b_out, jp_out, j_out, partials, as_stream(stream));
+  return state_t<double>(p, bnd, phi, az, b_out, jp_out, j_out, partials, as_stream(stream));
 }
 
 extern "C" int nfgs_sources(const nfgs_plan* p, const void* jp, void* s_bcz, void* s_jz,

@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void paired_transform_kernel(
     }
 
     for (int c = 0; c < nchunks; ++c) {
-      __syncthreads();  // previous chunk fully consumed
+      lds_barrier();  // previous chunk fully consumed
       // ---- stage the data chunk: original j in [32c, 32c+32) ----
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void paired_transform_kernel(
         me[e] = ve;
         mo[e] = vo;
       }
-      __syncthreads();
+      lds_barrier();
 
       const int valid = min(kChunk, rows - c * kChunk);
       const int ksteps = (valid + 3) >> 2;
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
   long tile = xcd_tile(blockIdx.x, gridDim.x);
   long ctp = tile * V / nvec, ctq = tile * V - ctp * nvec;
   if (tile < ntiles) load_tile(tile, ctp, ctq);
-  __syncthreads();
+  lds_barrier();
 
   for (; tile < ntiles; tile += gridDim.x) {
     // Every register is stored (elements past n go to a trash word after
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
       const int idx = j < n ? (j & 1) * V * S + myv * S + pos(j >> 1) : trash;
       xe[idx] = pvalid ? pf[i] : T(0);
     }
-    __syncthreads();
+    lds_barrier();
     // the last tile re-reads itself instead of branching around the loads
     const long next = tile + gridDim.x;
     const long otp = ctp, otq = ctq;
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
         if (k2 != k) o[(long)k2 * oes] = e - od;
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
 }
 

@@ -14,30 +14,69 @@ namespace {
 
 constexpr int KB = 32;
 
-// u_r = d'_r - c'_r u_{r+1} for r = top .. 0 (row r lives on plane r+1);
-// blocks of KB rows load d' and c' before any store of the block.
+// Rows are processed in blocks of KB; the loads of block b+1 are issued
+// before block b is computed, so two blocks of loads per lane are in flight.
+// Loads past the row range are clamped to the last row and never used.
+//
+// Forward rows r = lo .. hi (row r lives on plane r+1): step(r, v) consumes
+// the loaded value of row r.
+template <typename T, typename Step>
+__device__ __forceinline__ void forward_rows(const T* __restrict__ src, long plane, long m, int lo,
+                                             int hi, Step step) {
+  if (hi < lo) return;
+  T va[KB], vb[KB];
+  auto load = [&](T (&v)[KB], int rb) {
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+      const int r = rb + i < hi ? rb + i : hi;
+      v[i] = src[(long)(r + 1) * plane + m];
+    }
+  };
+  auto run = [&](const T (&v)[KB], int rb) {
+#pragma unroll
+    for (int i = 0; i < KB; ++i)
+      if (rb + i <= hi) step(rb + i, v[i]);
+  };
+  load(va, lo);
+  for (int rb = lo; rb <= hi; rb += 2 * KB) {
+    load(vb, rb + KB);
+    run(va, rb);
+    load(va, rb + 2 * KB);
+    run(vb, rb + KB);
+  }
+}
+
+// u_r = d'_r - c'_r u_{r+1} for r = top .. 0 (row r lives on plane r+1).
 template <typename T>
 __device__ __forceinline__ void back_substitute(T* out, const T* cs, long plane, long m, int top,
                                                 T u) {
-  for (int rb = top; rb >= 0; rb -= KB) {
-    T dv[KB], cv[KB];
+  if (top < 0) return;
+  T da[KB], ca[KB], db[KB], cb[KB];
+  auto load = [&](T (&d)[KB], T (&c)[KB], int rb) {
 #pragma unroll
     for (int i = 0; i < KB; ++i) {
-      const int r = rb - i;
-      if (r >= 0) {
-        const long o = (long)(r + 1) * plane + m;
-        dv[i] = out[o];
-        cv[i] = cs[o];
-      }
+      const int r = rb - i > 0 ? rb - i : 0;
+      const long o = (long)(r + 1) * plane + m;
+      d[i] = out[o];
+      c[i] = cs[o];
     }
+  };
+  auto run = [&](const T (&d)[KB], const T (&c)[KB], int rb) {
 #pragma unroll
     for (int i = 0; i < KB; ++i) {
       const int r = rb - i;
       if (r >= 0) {
-        u = dv[i] - cv[i] * u;
+        u = d[i] - c[i] * u;
         out[(long)(r + 1) * plane + m] = u;
       }
     }
+  };
+  load(da, ca, top);
+  for (int rb = top; rb >= 0; rb -= 2 * KB) {
+    load(db, cb, rb - KB);
+    run(da, ca, rb);
+    load(da, ca, rb - 2 * KB);
+    run(db, cb, rb - KB);
   }
 }
 
@@ -51,12 +90,13 @@ __global__ __launch_bounds__(256) void robin_sweep_kernel(
   if (m >= plane) return;
   if (m == 0) {
     // The (0,0) mode carries no correction; it is the zero-mode diagnostic.
+    // rows r = -1 .. nz-2 cover planes 0 .. nz-1 (blocked like every lane)
     double acc = 0.0;
-    for (int z = 0; z < nz; ++z) {
-      const double v = (double)src[(long)z * plane] / norm;
+    forward_rows(src, plane, m, -1, nz - 2, [&](int r, T sv) {
+      const double v = (double)sv / norm;
       acc += v * v;
-      out[(long)z * plane] = T(0);
-    }
+      out[(long)(r + 1) * plane] = T(0);
+    });
     if (zm_partial) *zm_partial = acc;
     return;
   }
@@ -85,28 +125,14 @@ __global__ __launch_bounds__(256) void robin_sweep_kernel(
     out[plane + m] = dp;
   }
   const int nu = nz - 1;
-  // Rows are processed in blocks whose loads are all issued before the
-  // block's stores, so KB row loads per lane are in flight at once.
-  for (int rb = 1; rb < nu - 1; rb += KB) {
-    T sv[KB];
-#pragma unroll
-    for (int i = 0; i < KB; ++i) {
-      const int r = rb + i;
-      sv[i] = r < nu - 1 ? src[(long)(r + 1) * plane + m] : T(0);
-    }
-#pragma unroll
-    for (int i = 0; i < KB; ++i) {
-      const int r = rb + i;
-      if (r < nu - 1) {
-        const T s = sv[i] / l;
-        const T den = diag - lo * cp;
-        cp = up / den;
-        dp = (s - lo * dp) / den;
-        cs[(long)(r + 1) * plane + m] = cp;
-        out[(long)(r + 1) * plane + m] = dp;
-      }
-    }
-  }
+  forward_rows(src, plane, m, 1, nu - 2, [&](int r, T sv) {
+    const T s = sv / l;
+    const T den = diag - lo * cp;
+    cp = up / den;
+    dp = (s - lo * dp) / den;
+    cs[(long)(r + 1) * plane + m] = cp;
+    out[(long)(r + 1) * plane + m] = dp;
+  });
   // Top Robin row with u_{N-2} eliminated through the PDE at plane N-1.
   {
     const T s = src[(long)(nz - 2) * plane + m] / l;
@@ -141,27 +167,15 @@ __global__ __launch_bounds__(256) void dirichlet_sweep_kernel(
     T dp = (rhs(1) - a * u0) / bd;
     cs[plane + m] = cp;
     out[plane + m] = dp;
-    for (int ib = 1; ib < n; ib += KB) {
-      T sv[KB];
-#pragma unroll
-      for (int q = 0; q < KB; ++q) {
-        const int i = ib + q;
-        sv[q] = i < n ? src[(long)(i + 1) * plane + m] : T(0);
-      }
-#pragma unroll
-      for (int q = 0; q < KB; ++q) {
-        const int i = ib + q;
-        if (i < n) {
-          const T d = (-sv[q]) / l;
-          const T ce = (i == n - 1) ? T(0) : c;
-          const T den = bd - a * cp;
-          cp = ce / den;
-          dp = (d - a * dp) / den;
-          cs[(long)(i + 1) * plane + m] = cp;
-          out[(long)(i + 1) * plane + m] = dp;
-        }
-      }
-    }
+    forward_rows(src, plane, m, 1, n - 1, [&](int i, T sv) {
+      const T d = (-sv) / l;
+      const T ce = (i == n - 1) ? T(0) : c;
+      const T den = bd - a * cp;
+      cp = ce / den;
+      dp = (d - a * dp) / den;
+      cs[(long)(i + 1) * plane + m] = cp;
+      out[(long)(i + 1) * plane + m] = dp;
+    });
     back_substitute(out, cs, plane, m, n - 2, dp);
   }
   out[m] = u0;

@@ -616,10 +616,13 @@ class NFGSSolverGPU:
         if self.B_background is None:
             raise RuntimeError("NFGS boundary is not set. Call set_bottom_vector_boundary() first.")
 
-    def _advance(self, slot: int, zero_source: bool):
+    def _advance(self, slot: int, zero_source: bool, want_b: bool = False):
+        """One iteration into `slot`; B of the new state lands in _Bwork only
+        when want_b (the iteration itself never needs B in memory)."""
         phi, az = self._slots[slot]
         L.check(self._lib.nfgs_advance(self._plan, L.C.byref(self._bnd), L.C.byref(self._ws),
-                                       L.ptr(phi), L.ptr(az), L.ptr(self._Bwork), int(zero_source),
+                                       L.ptr(phi), L.ptr(az),
+                                       L.ptr(self._Bwork) if want_b else None, int(zero_source),
                                        self._stream), "iteration")
 
     def _read_metrics(self):
@@ -648,11 +651,10 @@ class NFGSSolverGPU:
         phi, az = self._slots[slot]
         B = self._empty(3, self.Nz, self.Ny, self.Nx) if B_out is None else B_out
         J = self._empty(3, self.Nz, self.Ny, self.Nx) if J_out is None else J_out
-        L.check(self._lib.nfgs_reconstruct_b(self._plan, L.C.byref(self._bnd), L.ptr(phi),
-                                             L.ptr(az), L.ptr(B), self._stream), "reconstruct")
-        L.check(self._lib.nfgs_face_pass(self._plan, L.C.byref(self._bnd), L.ptr(B), L.ptr(az),
-                                         None, L.ptr(J), L.ptr(self._partials) if face_diag else None,
-                                         self._stream), "face pass")
+        L.check(self._lib.nfgs_state_pass(self._plan, L.C.byref(self._bnd), L.ptr(phi), L.ptr(az),
+                                          L.ptr(B), None, L.ptr(J),
+                                          L.ptr(self._partials) if face_diag else None,
+                                          self._stream), "state pass")
         if face_diag:
             L.check(self._lib.nfgs_reduce(self._plan, L.ptr(self._partials), L.ptr(self._metrics),
                                           self._stream), "reduce")
@@ -770,7 +772,7 @@ class NFGSSolverGPU:
         J_cb = None
         for n in range(1, int(max_iter) + 1):
             slot = 1 - best_slot
-            self._advance(slot, zero_source=False)
+            self._advance(slot, zero_source=False, want_b=per_step_callback is not None)
             sigma, divB, energy, ratio = self._read_metrics()
             self._check_zero_mode(ratio)
             final_slot = slot
