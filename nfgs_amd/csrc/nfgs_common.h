@@ -74,6 +74,15 @@ __device__ __forceinline__ T fd1gr(G g, int i, int n, T r) {
   return (T(3) * g(0) - T(4) * g(-1) + g(-2)) * r;
 }
 
+// Square root and quotient of the convergence monitors (sigma_J, |B|).  In
+// FP32 they use the hardware instructions (about 1 ulp, like the reference's
+// own FP32 monitors, without the correctly rounded sequences); in FP64 they
+// are the IEEE operations.
+__device__ __forceinline__ float msqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double msqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float mdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double mdiv(double a, double b) { return a / b; }
+
 // Mirror-Neumann 5-point lateral Laplacian at (y, x) of a (ny, nx) plane with
 // row stride ld, accumulated in the reference's term order.
 template <typename T>

@@ -304,7 +304,9 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
   const T* By = B + g.N;
   const T* Bz = B + 2 * g.N;
   const T jz0 = ld(jzt, h.own);  // toroidal bottom current at the own point
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  // per-thread sums over the z-chunk in the solver dtype; blocks combine
+  // them in FP64
+  T acc[7] = {0, 0, 0, 0, 0, 0, 0};
   // previous plane at the own point; pz2 = Bz two planes below
   T pb[3] = {0, 0, 0}, pg[2] = {0, 0}, pjz = 0, pz2 = 0;
   FaceVal<T> fprev;
@@ -397,22 +399,22 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
             c[1] = T(0.5) * (fprev.j[1] + F.j[1]);
             c[2] = T(0.5) * (fprev.j[2] + F.j[2]);
           }
-          acc[6] += (double)c[0] * c[0] + (double)c[1] * c[1] + (double)c[2] * c[2];
-          acc[5] += (double)F.j[0] * F.j[0] + (double)F.j[1] * F.j[1] + (double)F.j[2] * F.j[2];
+          acc[6] += c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+          acc[5] += F.j[0] * F.j[0] + F.j[1] * F.j[1] + F.j[2] * F.j[2];
           st(pl(jout, k), h.own, edge ? T(0) : c[0]);
           st(pl(jout + g.N, k), h.own, edge ? T(0) : c[1]);
           st(pl(jout + 2 * g.N, k), h.own, pjz);
         }
         if (partials) {
           if (k >= w.z0 && k < w.z1 && in_win) {
-            T bn = sqrt(F.b[0] * F.b[0] + F.b[1] * F.b[1] + F.b[2] * F.b[2]);
+            T bn = msqrt(F.b[0] * F.b[0] + F.b[1] * F.b[1] + F.b[2] * F.b[2]);
             bn = bn < T(1e-10) ? T(1e-10) : bn;
-            const T jn = sqrt(F.j[0] * F.j[0] + F.j[1] * F.j[1] + F.j[2] * F.j[2]);
+            const T jn = msqrt(F.j[0] * F.j[0] + F.j[1] * F.j[1] + F.j[2] * F.j[2]);
             const T c0 = F.j[1] * F.b[2] - F.j[2] * F.b[1];
             const T c1 = F.j[2] * F.b[0] - F.j[0] * F.b[2];
             const T c2 = F.j[0] * F.b[1] - F.j[1] * F.b[0];
-            acc[0] += (double)(sqrt(c0 * c0 + c1 * c1 + c2 * c2) / bn);
-            acc[1] += (double)jn;
+            acc[0] += mdiv(msqrt(c0 * c0 + c1 * c1 + c2 * c2), bn);
+            acc[1] += jn;
           }
           const T dvx = lds_dx(sbx[k & 1], t, g);
           const T dvy = lds_dy(sby[k & 1], t, g);
@@ -425,9 +427,9 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
             dvz = (cb[2] - pz2) * g.i2dz;
           const T div = dvx + dvy + dvz;
           const T b2 = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
-          acc[2] += (double)(div < T(0) ? -div : div);
-          acc[3] += (double)sqrt(b2);
-          acc[4] += (double)b2;
+          acc[2] += div < T(0) ? -div : div;
+          acc[3] += msqrt(b2);
+          acc[4] += b2;
         }
       }
       fprev = F;
@@ -447,8 +449,7 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
       st(pl(jp + 2 * g.N, kt), h.own, fprev.p[2]);
     }
     if (jout) {
-      acc[6] += (double)fprev.j[0] * fprev.j[0] + (double)fprev.j[1] * fprev.j[1] +
-                (double)fprev.j[2] * fprev.j[2];
+      acc[6] += fprev.j[0] * fprev.j[0] + fprev.j[1] * fprev.j[1] + fprev.j[2] * fprev.j[2];
       st(pl(jout, kt), h.own, T(0));
       st(pl(jout + g.N, kt), h.own, T(0));
       st(pl(jout + 2 * g.N, kt), h.own, T(0));
@@ -460,22 +461,25 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
       const T dvz = (T(3) * pb[2] - T(4) * pz2 + ld(pl(Bz, kt - 2), h.own)) * g.i2dz;
       const T div = dvx + dvy + dvz;
       const T b2 = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
-      acc[2] += (double)(div < T(0) ? -div : div);
-      acc[3] += (double)sqrt(b2);
-      acc[4] += (double)b2;
+      acc[2] += div < T(0) ? -div : div;
+      acc[3] += msqrt(b2);
+      acc[4] += b2;
     }
   }
   if (!partials) return;
-  block_sum<7>(acc, red);
+  double accd[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) accd[q] = (double)acc[q];
+  block_sum<7>(accd, red);
   if (threadIdx.x == 0) {
     const long bid = t.id;
-    partials[NFGS_Q_SIGMA_NUM * pstride + bid] = acc[0];
-    partials[NFGS_Q_SIGMA_DEN * pstride + bid] = acc[1];
-    partials[NFGS_Q_DIV_ABS * pstride + bid] = acc[2];
-    partials[NFGS_Q_BMAG * pstride + bid] = acc[3];
-    partials[NFGS_Q_B2 * pstride + bid] = acc[4];
-    partials[NFGS_Q_FACE_J2 * pstride + bid] = acc[5];
-    partials[NFGS_Q_CELL_J2 * pstride + bid] = acc[6];
+    partials[NFGS_Q_SIGMA_NUM * pstride + bid] = accd[0];
+    partials[NFGS_Q_SIGMA_DEN * pstride + bid] = accd[1];
+    partials[NFGS_Q_DIV_ABS * pstride + bid] = accd[2];
+    partials[NFGS_Q_BMAG * pstride + bid] = accd[3];
+    partials[NFGS_Q_B2 * pstride + bid] = accd[4];
+    partials[NFGS_Q_FACE_J2 * pstride + bid] = accd[5];
+    partials[NFGS_Q_CELL_J2 * pstride + bid] = accd[6];
   }
 }
 
@@ -499,28 +503,58 @@ struct StateSet {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void state_pass_kernel(
-    Grid<T> g, const T* __restrict__ phi, const T* __restrict__ az, const T* __restrict__ bg,
-    const T* __restrict__ bz_obs, const T* __restrict__ hprof, const T* __restrict__ jzt,
-    T* __restrict__ bout, T* __restrict__ jp, T* __restrict__ jout, double* partials,
-    long pstride, T floor_b2, T amax, Window w) {
-  __shared__ T sphi[3][PSA];
-  __shared__ T saz[2][PSA];
-  __shared__ T sgx[2][PSA];
-  __shared__ T sgy[2][PSA];
-  __shared__ T sdz[PSA];
-  __shared__ T sbx[2][PSA];
-  __shared__ T sby[2][PSA];
-  __shared__ T scz[2][PSA];  // Bz - B_bg,z on R1
-  __shared__ T sgz[2][256];
-  __shared__ double red[NFGS_NQ * 16];
-  const Tile t = make_tile(g);
+struct StateLds {
+  T sphi[3][PSA];
+  T saz[2][PSA];
+  T sgx[2][PSA];
+  T sgy[2][PSA];
+  T sdz[PSA];
+  T sbx[2][PSA];
+  T sby[2][PSA];
+  T scz[2][PSA];  // Bz - B_bg,z on R1
+  T sgz[2][256];
+  double red[NFGS_NQ * 16];
+};
+
+// Derivative helpers of the state pass: INNER tiles (every point the pass
+// touches is at least one node from the grid edge) take the centred forms
+// without per-lane edge branches; the arithmetic of those forms is the same.
+template <bool INNER, typename T>
+__device__ __forceinline__ T fdr(const T* f, long s, int i, int n, T r) {
+  if (INNER) return (f[s] - f[-s]) * r;
+  return fd1r(f, s, i, n, r);
+}
+template <typename T, bool INNER, typename G>
+__device__ __forceinline__ T fdgr(G g, int i, int n, T r) {
+  if (INNER) return (g(1) - g(-1)) * r;
+  return fd1gr<T>(g, i, n, r);
+}
+template <bool INNER, typename T>
+__device__ __forceinline__ T lapr(const T* p, int y, int x, int ny, int nx, long ld, T idx2,
+                                  T idy2) {
+  if (INNER) {
+    const T* c = p + (long)y * ld + x;
+    T acc = T(0);
+    acc += (c[1] - T(2) * c[0] + c[-1]) * idx2;
+    acc += (c[ld] - T(2) * c[0] + c[-ld]) * idy2;
+    return acc;
+  }
+  return lap_xy_at(p, y, x, ny, nx, ld, idx2, idy2);
+}
+
+template <typename T, bool INNER>
+__device__ __forceinline__ void state_pass_body(
+    StateLds<T>& L, const Tile& t, const Grid<T>& g, const T* __restrict__ phi,
+    const T* __restrict__ az, const T* __restrict__ bg, const T* __restrict__ bz_obs,
+    const T* __restrict__ hprof, const T* __restrict__ jzt, T* __restrict__ bout,
+    T* __restrict__ jp, T* __restrict__ jout, double* partials, long pstride, T floor_b2,
+    T amax, const Window& w) {
   const Halo h = make_halo(t, g);
   const int k0 = t.k0, k1 = min(k0 + KCHUNK, g.nz);
   const int ps = max(k0 - 1, 0), pe = min(k1, g.nz - 1);
   auto pl = [&](auto* base, int k) { return base + (long)k * g.plane; };
   auto lo_of = [&](int k) { return max(0, min(k - 1, g.nz - 3)); };
-  const bool edge = t.y == 0 || t.y == g.ny - 1 || t.x == 0 || t.x == g.nx - 1;
+  const bool edge = !INNER && (t.y == 0 || t.y == g.ny - 1 || t.x == 0 || t.x == g.nx - 1);
   const bool in_win = t.y >= w.y0 && t.y < w.y1 && t.x >= w.x0 && t.x < w.x1;
   const T jz0 = ld(jzt, h.own);
 
@@ -531,9 +565,9 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
     fetch(f0, pl(phi, lo), h);
     fetch(f1, pl(phi, lo + 1), h);
     fetch(f2, pl(phi, lo + 2), h);
-    commit(sphi[lo % 3], f0);
-    commit(sphi[(lo + 1) % 3], f1);
-    commit(sphi[(lo + 2) % 3], f2);
+    commit(L.sphi[lo % 3], f0);
+    commit(L.sphi[(lo + 1) % 3], f1);
+    commit(L.sphi[(lo + 2) % 3], f2);
   }
   auto fetch_set = [&](StateSet<T>& q, int p) {
     const int pc = min(p, pe);
@@ -544,7 +578,9 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
     q.gz = ld(pl(bg + 2 * g.N, pc), h.own);
   };
 
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  // per-thread sums over the z-chunk in the solver dtype; blocks combine
+  // them in FP64
+  T acc[7] = {0, 0, 0, 0, 0, 0, 0};
   T pb[3] = {0, 0, 0}, pg[2] = {0, 0}, pjz = 0, pz2 = 0, pz3 = 0;
   FaceVal<T> fprev;
 
@@ -552,11 +588,11 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
     const bool act = p <= pe;
     const int sl = p & 1;
     if (act) {
-      if (p > ps && lo_of(p) > lo_of(p - 1)) commit(sphi[(lo_of(p) + 2) % 3], q.phi);
-      commit(saz[sl], q.az);
-      commit(sgx[sl], q.gx);
-      commit(sgy[sl], q.gy);
-      sgz[sl][threadIdx.x] = q.gz;
+      if (p > ps && lo_of(p) > lo_of(p - 1)) commit(L.sphi[(lo_of(p) + 2) % 3], q.phi);
+      commit(L.saz[sl], q.az);
+      commit(L.sgx[sl], q.gx);
+      commit(L.sgy[sl], q.gy);
+      L.sgz[sl][threadIdx.x] = q.gz;
     }
     lds_barrier();
     fetch_set(q, p + 1);
@@ -567,12 +603,12 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
         const int e = threadIdx.x + 256 * i;
         T d;
         if (p == 0)
-          d = (T(-3) * sphi[0][e] + T(4) * sphi[1][e] - sphi[2][e]) * g.i2dz;
+          d = (T(-3) * L.sphi[0][e] + T(4) * L.sphi[1][e] - L.sphi[2][e]) * g.i2dz;
         else if (p == g.nz - 1)
-          d = (T(3) * sphi[p % 3][e] - T(4) * sphi[(p - 1) % 3][e] + sphi[(p - 2) % 3][e]) * g.i2dz;
+          d = (T(3) * L.sphi[p % 3][e] - T(4) * L.sphi[(p - 1) % 3][e] + L.sphi[(p - 2) % 3][e]) * g.i2dz;
         else
-          d = (sphi[(p + 1) % 3][e] - sphi[(p - 1) % 3][e]) * g.i2dz;
-        sdz[e] = d;
+          d = (L.sphi[(p + 1) % 3][e] - L.sphi[(p - 1) % 3][e]) * g.i2dz;
+        L.sdz[e] = d;
       }
     }
     lds_barrier();
@@ -587,26 +623,26 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
           const int e = ly * PW + lx;
           const int y = t.y0 + ly - HH, x = t.x0 + lx - HH;
           T vbx = T(0), vby = T(0), vcz = T(0);
-          if (x >= 0 && x < g.nx && y >= 0 && y < g.ny) {
-            const T dxd = fd1r(sdz + e, 1, x, g.nx, g.i2dx);
-            const T dyd = fd1r(sdz + e, (long)PW, y, g.ny, g.i2dy);
-            const T dxa = fd1r(saz[sl] + e, 1, x, g.nx, g.i2dx);
-            const T dya = fd1r(saz[sl] + e, (long)PW, y, g.ny, g.i2dy);
+          if (INNER || (x >= 0 && x < g.nx && y >= 0 && y < g.ny)) {
+            const T dxd = fdr<INNER>(L.sdz + e, 1, x, g.nx, g.i2dx);
+            const T dyd = fdr<INNER>(L.sdz + e, (long)PW, y, g.ny, g.i2dy);
+            const T dxa = fdr<INNER>(L.saz[sl] + e, 1, x, g.nx, g.i2dx);
+            const T dya = fdr<INNER>(L.saz[sl] + e, (long)PW, y, g.ny, g.i2dy);
             const T bcx = -dxd + dya;
             const T bcy = -dyd - dxa;
-            vbx = (sgx[sl][e] + bcx) + hx;
-            vby = (sgy[sl][e] + bcy) + hy;
+            vbx = (L.sgx[sl][e] + bcx) + hx;
+            vby = (L.sgy[sl][e] + bcy) + hy;
             if (p == 0) {
               const unsigned o = (unsigned)((y * g.nx + x) * (int)sizeof(T));
               vcz = ld(bz_obs, o) - ld(bg + 2 * g.N, o);
             } else {
-              vcz = lap_xy_at(sphi[p % 3] + e - (y * PW + x), y, x, g.ny, g.nx, (long)PW, g.idx2,
+              vcz = lapr<INNER>(L.sphi[p % 3] + e - (y * PW + x), y, x, g.ny, g.nx, (long)PW, g.idx2,
                               g.idy2);
             }
           }
-          sbx[sl][e] = vbx;
-          sby[sl][e] = vby;
-          scz[sl][e] = vcz;
+          L.sbx[sl][e] = vbx;
+          L.sby[sl][e] = vby;
+          L.scz[sl][e] = vcz;
         }
       }
     }
@@ -614,18 +650,18 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
     if (!act) return;
     // (C) face pass at the own point
     T cb[3] = {0, 0, 0}, cg[2] = {0, 0}, cjz = 0;
-    if (t.valid) {
-      cb[0] = sbx[sl][t.c];
-      cb[1] = sby[sl][t.c];
-      cb[2] = p == 0 ? ld(bz_obs, h.own) : sgz[sl][threadIdx.x] + scz[sl][t.c];
-      cg[0] = sgx[sl][t.c];
-      cg[1] = sgy[sl][t.c];
+    if ((INNER || t.valid)) {
+      cb[0] = L.sbx[sl][t.c];
+      cb[1] = L.sby[sl][t.c];
+      cb[2] = p == 0 ? ld(bz_obs, h.own) : L.sgz[sl][threadIdx.x] + L.scz[sl][t.c];
+      cg[0] = L.sgx[sl][t.c];
+      cg[1] = L.sgy[sl][t.c];
       if (p == 0)
         cjz = jz0;
       else if (p == g.nz - 1 || edge)
         cjz = T(0);
       else
-        cjz = -lap_xy_at(saz[sl] + t.c - (t.y * PW + t.x), t.y, t.x, g.ny, g.nx, (long)PW,
+        cjz = -lapr<INNER>(L.saz[sl] + t.c - (t.y * PW + t.x), t.y, t.x, g.ny, g.nx, (long)PW,
                          g.idx2, g.idy2);
       if (bout && p >= k0 && p < k1) {
         st(pl(bout, p), h.own, cb[0]);
@@ -633,7 +669,7 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
         st(pl(bout + 2 * g.N, p), h.own, cb[2]);
       }
     }
-    if (p > ps && t.valid) {
+    if (p > ps && (INNER || t.valid)) {
       const int f = p - 1;  // face between planes f and p
       FaceVal<T> F;
       F.b[0] = T(0.5) * (cb[0] + pb[0]);
@@ -641,11 +677,11 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
       F.b[2] = T(0.5) * (cb[2] + pb[2]);
       const T dbx = ((cb[0] - cg[0]) - (pb[0] - pg[0])) * g.idz;
       const T dby = ((cb[1] - cg[1]) - (pb[1] - pg[1])) * g.idz;
-      const T* s1 = scz[sl];
-      const T* s0 = scz[f & 1];
+      const T* s1 = L.scz[sl];
+      const T* s0 = L.scz[f & 1];
       auto cfz = [&](int o) { return T(0.5) * (s1[t.c + o] + s0[t.c + o]); };
-      const T dyc = fd1gr<T>([&](int o) { return cfz(o * PW); }, t.y, g.ny, g.i2dy);
-      const T dxc = fd1gr<T>([&](int o) { return cfz(o); }, t.x, g.nx, g.i2dx);
+      const T dyc = fdgr<T, INNER>([&](int o) { return cfz(o * PW); }, t.y, g.ny, g.i2dy);
+      const T dxc = fdgr<T, INNER>([&](int o) { return cfz(o); }, t.x, g.nx, g.i2dx);
       F.j[0] = dyc - dby;
       F.j[1] = dbx - dxc;
       F.j[2] = T(0.5) * (cjz + pjz);
@@ -681,30 +717,30 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
             c[1] = T(0.5) * (fprev.j[1] + F.j[1]);
             c[2] = T(0.5) * (fprev.j[2] + F.j[2]);
           }
-          acc[6] += (double)c[0] * c[0] + (double)c[1] * c[1] + (double)c[2] * c[2];
-          acc[5] += (double)F.j[0] * F.j[0] + (double)F.j[1] * F.j[1] + (double)F.j[2] * F.j[2];
+          acc[6] += c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+          acc[5] += F.j[0] * F.j[0] + F.j[1] * F.j[1] + F.j[2] * F.j[2];
           st(pl(jout, k), h.own, edge ? T(0) : c[0]);
           st(pl(jout + g.N, k), h.own, edge ? T(0) : c[1]);
           st(pl(jout + 2 * g.N, k), h.own, pjz);
         }
         if (partials) {
           if (k >= w.z0 && k < w.z1 && in_win) {
-            T bn = sqrt(F.b[0] * F.b[0] + F.b[1] * F.b[1] + F.b[2] * F.b[2]);
+            T bn = msqrt(F.b[0] * F.b[0] + F.b[1] * F.b[1] + F.b[2] * F.b[2]);
             bn = bn < T(1e-10) ? T(1e-10) : bn;
-            const T jn = sqrt(F.j[0] * F.j[0] + F.j[1] * F.j[1] + F.j[2] * F.j[2]);
+            const T jn = msqrt(F.j[0] * F.j[0] + F.j[1] * F.j[1] + F.j[2] * F.j[2]);
             const T c0 = F.j[1] * F.b[2] - F.j[2] * F.b[1];
             const T c1 = F.j[2] * F.b[0] - F.j[0] * F.b[2];
             const T c2 = F.j[0] * F.b[1] - F.j[1] * F.b[0];
-            acc[0] += (double)(sqrt(c0 * c0 + c1 * c1 + c2 * c2) / bn);
-            acc[1] += (double)jn;
+            acc[0] += mdiv(msqrt(c0 * c0 + c1 * c1 + c2 * c2), bn);
+            acc[1] += jn;
           }
-          const T dvx = fd1r(sbx[k & 1] + t.c, 1, t.x, g.nx, g.i2dx);
-          const T dvy = fd1r(sby[k & 1] + t.c, (long)PW, t.y, g.ny, g.i2dy);
+          const T dvx = fdr<INNER>(L.sbx[k & 1] + t.c, 1, t.x, g.nx, g.i2dx);
+          const T dvy = fdr<INNER>(L.sby[k & 1] + t.c, (long)PW, t.y, g.ny, g.i2dy);
           T dvz;
           if (k == 0) {
             // Bz of plane 2 = B_bg,z + lap_xy(Phi_c) there (Phi ring holds 0..2)
             const T bz2 = ld(pl(bg + 2 * g.N, 2), h.own) +
-                          lap_xy_at(sphi[2] + t.c - (t.y * PW + t.x), t.y, t.x, g.ny, g.nx,
+                          lapr<INNER>(L.sphi[2] + t.c - (t.y * PW + t.x), t.y, t.x, g.ny, g.nx,
                                     (long)PW, g.idx2, g.idy2);
             dvz = (T(-3) * pb[2] + T(4) * cb[2] - bz2) * g.i2dz;
           } else {
@@ -712,9 +748,9 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
           }
           const T div = dvx + dvy + dvz;
           const T b2 = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
-          acc[2] += (double)(div < T(0) ? -div : div);
-          acc[3] += (double)sqrt(b2);
-          acc[4] += (double)b2;
+          acc[2] += div < T(0) ? -div : div;
+          acc[3] += msqrt(b2);
+          acc[4] += b2;
         }
       }
       fprev = F;
@@ -730,42 +766,61 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
   for (int p = ps; p <= pe; ++p) step(sa, p);
   // top node: no face above it (planes kt, kt-1, kt-2 of Bz are pb, pz2, pz3)
   const int kt = g.nz - 1;
-  if (k1 == g.nz && t.valid) {
+  if (k1 == g.nz && (INNER || t.valid)) {
     if (jp) {
       st(pl(jp, kt), h.own, fprev.p[0]);
       st(pl(jp + g.N, kt), h.own, fprev.p[1]);
       st(pl(jp + 2 * g.N, kt), h.own, fprev.p[2]);
     }
     if (jout) {
-      acc[6] += (double)fprev.j[0] * fprev.j[0] + (double)fprev.j[1] * fprev.j[1] +
-                (double)fprev.j[2] * fprev.j[2];
+      acc[6] += fprev.j[0] * fprev.j[0] + fprev.j[1] * fprev.j[1] + fprev.j[2] * fprev.j[2];
       st(pl(jout, kt), h.own, T(0));
       st(pl(jout + g.N, kt), h.own, T(0));
       st(pl(jout + 2 * g.N, kt), h.own, T(0));
     }
     if (partials) {
-      const T dvx = fd1r(sbx[kt & 1] + t.c, 1, t.x, g.nx, g.i2dx);
-      const T dvy = fd1r(sby[kt & 1] + t.c, (long)PW, t.y, g.ny, g.i2dy);
+      const T dvx = fdr<INNER>(L.sbx[kt & 1] + t.c, 1, t.x, g.nx, g.i2dx);
+      const T dvy = fdr<INNER>(L.sby[kt & 1] + t.c, (long)PW, t.y, g.ny, g.i2dy);
       const T dvz = (T(3) * pb[2] - T(4) * pz2 + pz3) * g.i2dz;
       const T div = dvx + dvy + dvz;
       const T b2 = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
-      acc[2] += (double)(div < T(0) ? -div : div);
-      acc[3] += (double)sqrt(b2);
-      acc[4] += (double)b2;
+      acc[2] += div < T(0) ? -div : div;
+      acc[3] += msqrt(b2);
+      acc[4] += b2;
     }
   }
   if (!partials) return;
-  block_sum<7>(acc, red);
+  double accd[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) accd[q] = (double)acc[q];
+  block_sum<7>(accd, L.red);
   if (threadIdx.x == 0) {
     const long bid = t.id;
-    partials[NFGS_Q_SIGMA_NUM * pstride + bid] = acc[0];
-    partials[NFGS_Q_SIGMA_DEN * pstride + bid] = acc[1];
-    partials[NFGS_Q_DIV_ABS * pstride + bid] = acc[2];
-    partials[NFGS_Q_BMAG * pstride + bid] = acc[3];
-    partials[NFGS_Q_B2 * pstride + bid] = acc[4];
-    partials[NFGS_Q_FACE_J2 * pstride + bid] = acc[5];
-    partials[NFGS_Q_CELL_J2 * pstride + bid] = acc[6];
+    partials[NFGS_Q_SIGMA_NUM * pstride + bid] = accd[0];
+    partials[NFGS_Q_SIGMA_DEN * pstride + bid] = accd[1];
+    partials[NFGS_Q_DIV_ABS * pstride + bid] = accd[2];
+    partials[NFGS_Q_BMAG * pstride + bid] = accd[3];
+    partials[NFGS_Q_B2 * pstride + bid] = accd[4];
+    partials[NFGS_Q_FACE_J2 * pstride + bid] = accd[5];
+    partials[NFGS_Q_CELL_J2 * pstride + bid] = accd[6];
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void state_pass_kernel(
+    Grid<T> g, const T* __restrict__ phi, const T* __restrict__ az, const T* __restrict__ bg,
+    const T* __restrict__ bz_obs, const T* __restrict__ hprof, const T* __restrict__ jzt,
+    T* __restrict__ bout, T* __restrict__ jp, T* __restrict__ jout, double* partials,
+    long pstride, T floor_b2, T amax, Window w) {
+  __shared__ StateLds<T> L;
+  const Tile t = make_tile(g);
+  const bool inner = t.x0 >= 2 && t.x0 + TX <= g.nx - 2 && t.y0 >= 2 && t.y0 + TY <= g.ny - 2;
+  if (inner)
+    state_pass_body<T, true>(L, t, g, phi, az, bg, bz_obs, hprof, jzt, bout, jp, jout, partials,
+                             pstride, floor_b2, amax, w);
+  else
+    state_pass_body<T, false>(L, t, g, phi, az, bg, bz_obs, hprof, jzt, bout, jp, jout, partials,
+                              pstride, floor_b2, amax, w);
 }
 
 // Register set of the loads one node of the sources needs.
@@ -796,7 +851,7 @@ __global__ __launch_bounds__(256) void sources_kernel(Grid<T> g, const T* __rest
   const bool interior = t.y >= 1 && t.y <= g.ny - 2 && t.x >= 1 && t.x <= g.nx - 2;
   // byte offset of the own point in the compact interior planes of S_Jz
   const unsigned sjo = interior ? (unsigned)(((t.y - 1) * nxi + (t.x - 1)) * (int)sizeof(T)) : 0u;
-  double acc[1] = {0.0};
+  T acc1 = T(0);  // sum of S_Bcz^2 over the thread's nodes (FP64 per block)
   for (int q = max(k0 - 1, 0); q <= k0; ++q) {
     Fetch<T> fx, fy;
     fetch(fx, pl(jx, q), h);
@@ -828,7 +883,7 @@ __global__ __launch_bounds__(256) void sources_kernel(Grid<T> g, const T* __rest
     if (act && t.valid) {
       const T s = lds_dy(sjx[k % 3], t, g) - lds_dx(sjy[k % 3], t, g);
       st(pl(sb, k), h.own, s);
-      acc[0] += (double)s * s;
+      acc1 += s * s;
       if (interior) {
         T v = lds_lap(sjz, t, g);
         if (k >= 1 && k <= g.nz - 2) {
@@ -853,6 +908,7 @@ __global__ __launch_bounds__(256) void sources_kernel(Grid<T> g, const T* __rest
     step(sb2, k + 1);
   }
   if (!partials) return;
+  double acc[1] = {(double)acc1};
   block_sum<1>(acc, red);
   if (threadIdx.x == 0) {
     const long bid = t.id;
