@@ -59,9 +59,11 @@ int build_paired(int dtype, int kind, int n, PairedMatrix* m) {
         if (jo < n) ot[(size_t)jp * m->cols_pad + k] = sc * type1_entry(kind, n, k, jo);
       }
     }
-    int rc = upload_dtype(dtype, et, &m->et[dir]);
+    // FP64 for every plan dtype: the transforms run their products in FP64
+    (void)dtype;
+    int rc = upload<double>(et, &m->et[dir]);
     if (rc) return rc;
-    rc = upload_dtype(dtype, ot, &m->ot[dir]);
+    rc = upload<double>(ot, &m->ot[dir]);
     if (rc) return rc;
   }
   return NFGS_OK;

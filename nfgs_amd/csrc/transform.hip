@@ -1,5 +1,8 @@
 // Lateral type-I transforms (DCT-I / DST-I and inverses) as paired MFMA GEMMs.
 //
+// Both plan dtypes compute in FP64 (v_mfma_f64_16x16x4); FP32 data is widened
+// when staged in LDS and rounded once when stored (see launch_t).
+//
 // A workgroup of 256 lanes (4 wavefronts) takes 64 vectors of length n.  The
 // transform matrix is split into its even-column half E and odd-column half O
 // (see PairedMatrix), so each vector costs two (h x n/2) products instead of
@@ -50,9 +53,9 @@ constexpr int kVec = 64;    // vectors per workgroup
 constexpr int kChunk = 16;  // paired rows (j') per LDS chunk
 constexpr int kNB = 9;      // 16-row output blocks per group (144 outputs)
 
-template <typename T, bool ELEM_CONTIG>
+template <typename T, bool ELEM_CONTIG, typename TS = T>
 __global__ __launch_bounds__(256) void paired_transform_kernel(
-    const T* __restrict__ in, long ies, long ivs, long ips, T* __restrict__ out, long oes,
+    const TS* __restrict__ in, long ies, long ivs, long ips, TS* __restrict__ out, long oes,
     long ovs, long ops, const T* __restrict__ et, const T* __restrict__ ot, int n, int h,
     int rows, int cols_pad, int nvec, long nvec_total) {
   using A = typename Mfma<T>::acc;
@@ -111,7 +114,7 @@ __global__ __launch_bounds__(256) void paired_transform_kernel(
       for (int i = 0; i < 8; ++i) {
         const int j = c * 2 * kChunk + islot_j[i];
         T val = T(0);
-        if (ibase[i] >= 0 && j < n) val = in[ibase[i] + (long)j * ies];
+        if (ibase[i] >= 0 && j < n) val = T(in[ibase[i] + (long)j * ies]);
         const int jp = islot_j[i] >> 1;
         const int v = islot_v[i];
         const int idx = ELEM_CONTIG ? v * (kChunk + 1) + jp : jp * kVec + v;
@@ -184,11 +187,11 @@ __global__ __launch_bounds__(256) void paired_transform_kernel(
         const long g = g0 + v;
         if (k < h && g < nvec_total) {
           const long p = g / nvec, q = g - p * nvec;
-          T* o = out + p * ops + q * ovs;
+          TS* o = out + p * ops + q * ovs;
           const T e = accE[b][r], od = accO[b][r];
-          o[(long)k * oes] = e + od;
+          o[(long)k * oes] = TS(e + od);
           const int k2 = n - 1 - k;
-          if (k2 != k) o[(long)k2 * oes] = e - od;
+          if (k2 != k) o[(long)k2 * oes] = TS(e - od);
         }
       }
     }
@@ -203,9 +206,9 @@ __global__ __launch_bounds__(256) void paired_transform_kernel(
 // issued before the current tile's MFMAs, so HBM latency hides behind the
 // matrix work.  LDS holds the even and odd halves of the tile de-interleaved.
 // ---------------------------------------------------------------------------
-template <typename T, bool EC, int QMAX, int NKB, int PF, int V, int QC>
+template <typename T, bool EC, int QMAX, int NKB, int PF, int V, int QC, typename TS = T>
 __global__ __launch_bounds__(512) void paired_transform_v2(
-    const T* __restrict__ in, long ies, long ivs, long ips, T* __restrict__ out, long oes,
+    const TS* __restrict__ in, long ies, long ivs, long ips, TS* __restrict__ out, long oes,
     long ovs, long ops, const T* __restrict__ et, const T* __restrict__ ot, int n, int h,
     int ne, int cols_pad, int nvec, long nvec_total, int S, int hb, int rem) {
   using A = typename Mfma<T>::acc;
@@ -255,7 +258,7 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
   const int myv = EC ? tid / (nthr / V) : tid % V;
   const int j0 = EC ? tid % (nthr / V) : tid / V;
   const int jstep = nthr / V;
-  T pf[PF];
+  TS pf[PF];  // prefetched elements in the storage type (widened at commit)
   bool pvalid = false;
   // Loads are unconditional: a vector past the end reads vector 0 and
   // elements past n re-read element n-1.  The raw values go to registers and
@@ -311,7 +314,7 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
     for (int i = 0; i < PF; ++i) {
       const int j = j0 + jstep * i;
       const int idx = j < n ? (j & 1) * V * S + myv * S + pos(j >> 1) : trash;
-      xe[idx] = pvalid ? pf[i] : T(0);
+      xe[idx] = pvalid ? T(pf[i]) : T(0);
     }
     lds_barrier();
     // the last tile re-reads itself instead of branching around the loads
@@ -428,11 +431,11 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
           }
           const long gi = tile * V + v;
           if (k < h && gi < nvec_total) {
-            T* o = out + vec_off(otp, otq, v, ops, ovs);
+            TS* o = out + vec_off(otp, otq, v, ops, ovs);
             const T e = accE[g][kb][r], od = accO[g][kb][r];
-            o[(long)k * oes] = e + od;
+            o[(long)k * oes] = TS(e + od);
             const int k2 = n - 1 - k;
-            if (k2 != k) o[(long)k2 * oes] = e - od;
+            if (k2 != k) o[(long)k2 * oes] = TS(e - od);
           }
         }
       }
@@ -456,10 +459,10 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
       const int k = hb * 16 + rr;
       const long gi = tile * V + v;
       if (part == 0 && k < h && gi < nvec_total) {
-        T* o = out + vec_off(otp, otq, v, ops, ovs);
-        o[(long)k * oes] = e + od;
+        TS* o = out + vec_off(otp, otq, v, ops, ovs);
+        o[(long)k * oes] = TS(e + od);
         const int k2 = n - 1 - k;
-        if (k2 != k) o[(long)k2 * oes] = e - od;
+        if (k2 != k) o[(long)k2 * oes] = TS(e - od);
       }
     }
     lds_barrier();
@@ -479,11 +482,13 @@ int cu_count() {
 }
 
 // Launch v2 when the shape fits its register plan; returns -1 when it does not.
-template <typename T, bool EC>
-int launch_v2(const PairedMatrix* m, int dir, const T* in, long ies, long ivs, long ips, T* out,
+template <typename T, bool EC, typename TS>
+int launch_v2(const PairedMatrix* m, int dir, const TS* in, long ies, long ivs, long ips, TS* out,
               long oes, long ovs, long ops, int nvec, long total, hipStream_t s) {
-  constexpr int V = sizeof(T) == 4 ? 32 : 16;
-  constexpr int QMAX = 33, NKB = 1, PF = 18;
+  // 16 vectors per tile: with FP64 operands wider tiles do not fit the
+  // register file next to the matrix fragments.
+  constexpr int V = 16;
+  constexpr int QMAX = 33, NKB = 1, PF = 9;
   const int h = m->h, ne = m->rows, Q = (ne + 3) / 4;
   int hb = (h + 15) / 16, rem = 0;
   if (h >= 16 && h % 16 <= 2) {
@@ -503,7 +508,7 @@ int launch_v2(const PairedMatrix* m, int dir, const T* in, long ies, long ivs, l
   const long ntiles = (total + V - 1) / V;
   const long grid = std::min<long>(ntiles, 2L * cu_count());
 #define NFGS_V2(QC_)                                                                      \
-  hipLaunchKernelGGL((paired_transform_v2<T, EC, QMAX, NKB, PF, V, QC_>), dim3((unsigned)grid), \
+  hipLaunchKernelGGL((paired_transform_v2<T, EC, QMAX, NKB, PF, V, QC_, TS>), dim3((unsigned)grid), \
                      dim3(nthr), lds, s, in, ies, ivs, ips, out, oes, ovs, ops,                 \
                      static_cast<const T*>(m->et[dir]), static_cast<const T*>(m->ot[dir]),     \
                      m->n, h, ne, m->cols_pad, nvec, total, S, hb, rem)
@@ -525,31 +530,39 @@ int launch_v2(const PairedMatrix* m, int dir, const T* in, long ies, long ivs, l
   return NFGS_OK;
 }
 
-template <typename T>
+// Storage type TS (the plan dtype) with arithmetic type T.  FP32 plans run
+// the products in FP64 (T = double): the matrices are FP64, elements are
+// widened on their way into LDS and each output is rounded once when it is
+// stored.  With FP32 MFMA every product-add rounds (about 3 ulp per length-
+// 258 pass, three times an FFT's), and the modal solves amplify that noise
+// into the derivatives of B and J.
+template <typename T, typename TS>
 int launch_t(const PairedMatrix* m, int dir, const void* in, long ies, long ivs, long ips,
              void* out, long oes, long ovs, long ops, int nvec, int nplanes, hipStream_t s) {
   const long total = (long)nvec * nplanes;
   if (total == 0) return NFGS_OK;
   const long nblk = (total + kVec - 1) / kVec;
   NFGS_REQUIRE(nblk < (1L << 31), NFGS_ERR_ARG, "transform: too many vectors");
-  const T* ti = static_cast<const T*>(in);
-  T* to = static_cast<T*>(out);
+  const TS* ti = static_cast<const TS*>(in);
+  TS* to = static_cast<TS*>(out);
   const T* et = static_cast<const T*>(m->et[dir]);
   const T* ot = static_cast<const T*>(m->ot[dir]);
   if (ies == 1 && oes == 1) {
-    const int rc = launch_v2<T, true>(m, dir, ti, ies, ivs, ips, to, oes, ovs, ops, nvec, total, s);
+    const int rc =
+        launch_v2<T, true, TS>(m, dir, ti, ies, ivs, ips, to, oes, ovs, ops, nvec, total, s);
     if (rc >= 0) return rc;
   } else if (ivs == 1 && ovs == 1) {
-    const int rc = launch_v2<T, false>(m, dir, ti, ies, ivs, ips, to, oes, ovs, ops, nvec, total, s);
+    const int rc =
+        launch_v2<T, false, TS>(m, dir, ti, ies, ivs, ips, to, oes, ovs, ops, nvec, total, s);
     if (rc >= 0) return rc;
   }
   if (ies == 1 && oes == 1) {
-    hipLaunchKernelGGL((paired_transform_kernel<T, true>), dim3((unsigned)nblk), dim3(256), 0, s,
-                       ti, ies, ivs, ips, to, oes, ovs, ops, et, ot, m->n, m->h, m->rows,
+    hipLaunchKernelGGL((paired_transform_kernel<T, true, TS>), dim3((unsigned)nblk), dim3(256), 0,
+                       s, ti, ies, ivs, ips, to, oes, ovs, ops, et, ot, m->n, m->h, m->rows,
                        m->cols_pad, nvec, total);
   } else if (ivs == 1 && ovs == 1) {
-    hipLaunchKernelGGL((paired_transform_kernel<T, false>), dim3((unsigned)nblk), dim3(256), 0, s,
-                       ti, ies, ivs, ips, to, oes, ovs, ops, et, ot, m->n, m->h, m->rows,
+    hipLaunchKernelGGL((paired_transform_kernel<T, false, TS>), dim3((unsigned)nblk), dim3(256), 0,
+                       s, ti, ies, ivs, ips, to, oes, ovs, ops, et, ot, m->n, m->h, m->rows,
                        m->cols_pad, nvec, total);
   } else {
     set_error("transform: need unit element strides or unit vector strides on both sides");
@@ -570,8 +583,10 @@ int launch_transform(const nfgs_plan* p, const PairedMatrix* m, int direction, c
                "transform: in and out must be distinct non-null buffers");
   NFGS_REQUIRE(nvec >= 0 && nplanes >= 0, NFGS_ERR_ARG, "transform: negative counts");
   if (p->d.dtype == NFGS_F32)
-    return launch_t<float>(m, direction, in, ies, ivs, ips, out, oes, ovs, ops, nvec, nplanes, s);
-  return launch_t<double>(m, direction, in, ies, ivs, ips, out, oes, ovs, ops, nvec, nplanes, s);
+    return launch_t<double, float>(m, direction, in, ies, ivs, ips, out, oes, ovs, ops, nvec,
+                                   nplanes, s);
+  return launch_t<double, double>(m, direction, in, ies, ivs, ips, out, oes, ovs, ops, nvec,
+                                  nplanes, s);
 }
 
 }  // namespace nfgs
