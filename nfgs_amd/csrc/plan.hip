@@ -1,6 +1,8 @@
 // Plan lifecycle: grid constants, paired transform matrices, modal tables.
 
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -41,6 +43,64 @@ int upload_dtype(int dtype, const std::vector<double>& host, void** dev) {
   return dtype == NFGS_F32 ? upload<float>(host, dev) : upload<double>(host, dev);
 }
 
+// Digit form of E and O for the exact-integer FP32 transform.  Entry (k, j')
+// of a half becomes the integer round(T[k][2j'+half] * 2^shift) (|.| <= 2^30)
+// written as four signed base-256 digits; digit p of lane l of 16-row block b,
+// K chunk c, holds in byte 4w+i the paired row j' = 4(l/16) + w + 16(4c+i)
+// of output row k = 16b + l%16 (the order in which the kernel packs the
+// vector digits).  Paired rows past 64*nch (at most one) form the FP64 tail.
+int build_digits(int kind, int n, PairedMatrix* m) {
+  const int h = m->h, ne = m->rows;
+  const int nch = std::max(1, (ne + 62) / 64);
+  const int hb = (h + 15) / 16;
+  if (nch > 2 || hb > 9) return NFGS_OK;  // longer vectors keep the FP64 kernel
+  m->nch = nch;
+  m->tail = ne > 64 * nch ? 1 : 0;
+  m->hb = hb;
+  const double inv_scale = kind == NFGS_DCT1 ? 1.0 / (2.0 * (n - 1)) : 1.0 / (2.0 * (n + 1));
+  for (int dir = 0; dir < 2; ++dir) {
+    const double sc = dir == NFGS_FORWARD ? 1.0 : inv_scale;
+    auto entry = [&](int half, int k, int jp) -> double {
+      const int j = 2 * jp + half;
+      return (k < h && j < n) ? sc * type1_entry(kind, n, k, j) : 0.0;
+    };
+    double mx = 0.0;
+    for (int half = 0; half < 2; ++half)
+      for (int k = 0; k < h; ++k)
+        for (int jp = 0; jp < ne; ++jp) mx = std::max(mx, std::fabs(entry(half, k, jp)));
+    int e = 0;
+    (void)std::frexp(mx > 0.0 ? mx : 1.0, &e);  // mx < 2^e
+    int sh = 30 - e;
+    if (std::ldexp(mx, sh + 1) <= 1073741824.0) ++sh;
+    m->shift[dir] = sh;
+    const size_t nd = (size_t)2 * 4 * nch * hb * 64 * 16;
+    std::vector<int8_t> dg(nd);
+    size_t idx = 0;
+    for (int half = 0; half < 2; ++half)
+      for (int p = 0; p < 4; ++p)
+        for (int c = 0; c < nch; ++c)
+          for (int b = 0; b < hb; ++b)
+            for (int l = 0; l < 64; ++l)
+              for (int byte = 0; byte < 16; ++byte) {
+                const int w = byte >> 2, i = byte & 3;
+                const int k = 16 * b + (l & 15);
+                const int jp = 4 * (l >> 4) + w + 16 * (4 * c + i);
+                const long mi = std::lrint(std::ldexp(entry(half, k, jp), sh));
+                const uint32_t d = ((uint32_t)(int32_t)mi + 0x808080u) ^ 0x808080u;
+                dg[idx++] = (int8_t)(uint8_t)(d >> (8 * (3 - p)));
+              }
+    NFGS_HIP_CHECK(hipMalloc(&m->dig[dir], nd));
+    NFGS_HIP_CHECK(hipMemcpy(m->dig[dir], dg.data(), nd, hipMemcpyHostToDevice));
+    std::vector<double> mt((size_t)2 * 16 * hb, 0.0);
+    if (m->tail)
+      for (int half = 0; half < 2; ++half)
+        for (int k = 0; k < 16 * hb; ++k) mt[(size_t)half * 16 * hb + k] = entry(half, k, 64 * nch);
+    const int rc = upload<double>(mt, &m->mtail[dir]);
+    if (rc) return rc;
+  }
+  return NFGS_OK;
+}
+
 int build_paired(int dtype, int kind, int n, PairedMatrix* m) {
   m->n = n;
   m->h = (n + 1) / 2;
@@ -59,21 +119,23 @@ int build_paired(int dtype, int kind, int n, PairedMatrix* m) {
         if (jo < n) ot[(size_t)jp * m->cols_pad + k] = sc * type1_entry(kind, n, k, jo);
       }
     }
-    // FP64 for every plan dtype: the transforms run their products in FP64
-    (void)dtype;
+    // FP64 for every plan dtype: FP64 plans (and FP32 vectors too long for
+    // the digit form) run their products in FP64
     int rc = upload<double>(et, &m->et[dir]);
     if (rc) return rc;
     rc = upload<double>(ot, &m->ot[dir]);
     if (rc) return rc;
   }
-  return NFGS_OK;
+  return dtype == NFGS_F32 ? build_digits(kind, n, m) : NFGS_OK;
 }
 
 void free_paired(PairedMatrix* m) {
   for (int d = 0; d < 2; ++d) {
     if (m->et[d]) (void)hipFree(m->et[d]);
     if (m->ot[d]) (void)hipFree(m->ot[d]);
-    m->et[d] = m->ot[d] = nullptr;
+    if (m->dig[d]) (void)hipFree(m->dig[d]);
+    if (m->mtail[d]) (void)hipFree(m->mtail[d]);
+    m->et[d] = m->ot[d] = m->dig[d] = m->mtail[d] = nullptr;
   }
 }
 

@@ -18,6 +18,7 @@
 // 64-byte runs.
 
 #include <algorithm>
+#include <type_traits>
 
 #include "plan.h"
 
@@ -469,6 +470,267 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 plans: exact-integer products on the int8 MFMA.
+//
+// Each tile of 16 vectors is brought to fixed point with a power-of-two scale
+// per vector (|X| < 2^30) and split into four signed base-256 digits,
+// X = sum_p a_p 256^(3-p) with a_p in [-128, 127]; the matrices are held the
+// same way (PairedMatrix::dig).  v_mfma_i32_16x16x64_i8 sums digit products
+// exactly in int32, and the ten digit pairs of weight >= 256^-3 relative to
+// the leading pair are kept, in four int32 accumulators by weight.  The
+// accumulators combine exactly in FP64 and each output is rounded once to
+// FP32, so the result is as accurate as an FP64-arithmetic product (the
+// dropped pairs weigh about 2^-30 of the vector's largest element) while the
+// products run at int8 rate.  A tail paired row (n = 128*nch + 1 or + 2) is
+// added in FP64.
+//
+// Loader lanes are (vector v, offset ro < 32) pairs holding elements
+// j = ro + 32r: all of one parity, so four of them (r = 4c..4c+3) pack into
+// one digit word per digit plane with two rounds of v_perm_b32.  The K order
+// this defines (paired row j' = ro/2 + 16r) is the order of the matrix
+// digits; integer sums do not depend on it.  ELEM_CONTIG puts a vector's 32
+// offsets in one half-wavefront (coalesced along the vector) and otherwise
+// 16 vectors times 4 offsets per wavefront (coalesced across vectors).  Wave w
+// owns output rows [16w, 16w+16) with its matrix digits in registers; the
+// data digits of a tile are read from LDS by every wave.
+// ---------------------------------------------------------------------------
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef f32x4 f32x4u __attribute__((aligned(4)));
+
+template <bool EC, int NCH, bool TAIL>
+__global__ __launch_bounds__(576) void paired_transform_i8(
+    const float* __restrict__ in, long ies, long ivs, long ips, float* __restrict__ out,
+    long oes, long ovs, long ops, const i32x4* __restrict__ dig, const double* __restrict__ mtail,
+    int shift, int n, int h, int hb, int nvec, int nvec_total) {
+  constexpr int RSB = 64 * NCH + 16;  // bytes per vector row of one digit plane
+  constexpr int PLANE = 16 * RSB;     // one (half, digit) plane of a tile
+  constexpr int DBUF = 8 * PLANE;     // one tile: 2 halves x 4 digits
+  constexpr int NR = 4 * NCH;         // main elements per loader lane
+  constexpr int kBad = -0x40000000;   // exponent marking a non-finite vector
+  __shared__ __align__(16) unsigned char dl[2 * DBUF];
+  __shared__ unsigned smax[2][16];
+  __shared__ int sexp[2][16];
+  __shared__ float stail[2][2][16];
+  __shared__ double smt[2][16 * 9];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, g4 = lane >> 4;
+  const bool loader = wave < 8;
+  const int lv = EC ? 2 * wave + (lane >> 5) : l16;
+  const int lro = EC ? (lane & 31) : 4 * wave + g4;
+  const int lhalf = lro & 1, lm = lro >> 1;
+  const bool mw = wave < hb;
+
+  i32x4 md[2][4][NCH];
+#pragma unroll
+  for (int hf = 0; hf < 2; ++hf)
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int c = 0; c < NCH; ++c)
+        md[hf][p][c] = mw ? dig[(((hf * 4 + p) * NCH + c) * hb + wave) * 64 + lane]
+                          : i32x4{0, 0, 0, 0};
+  if (TAIL)
+    for (int e = tid; e < 32 * hb; e += blockDim.x) smt[e / (16 * hb)][e % (16 * hb)] = mtail[e];
+  if (tid < 32) smax[tid >> 4][tid & 15] = 0u;
+
+  const int ntiles = (nvec_total + 15) >> 4;
+  const int G = gridDim.x;
+
+  // Loads are unconditional: lanes past the last vector read vector 0 and
+  // elements past n re-read element n-1 (masked when digits are formed).
+  auto load = [&](int t, float (&r)[NR], float& rt) {
+    int gv = t * 16 + lv;
+    gv = gv < nvec_total ? gv : 0;
+    const int p = gv / nvec, q = gv - p * nvec;
+    const float* src = in + (long)p * ips + (long)q * ivs;
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const int j = lro + 32 * i;
+      r[i] = src[(long)(j < n ? j : n - 1) * ies];
+    }
+    if (TAIL) {
+      const int j = 128 * NCH + lro;
+      rt = src[(long)(j < n ? j : n - 1) * ies];
+    }
+  };
+
+  auto combine = [&](const i32x4 (&a)[4], int r, int sv) -> double {
+    const int t01 = a[0][r] * 256 + a[1][r];
+    const int t23 = a[2][r] * 256 + a[3][r];
+    return ldexp(fma((double)t01, 65536.0, (double)t23), 24 - shift - sv);
+  };
+
+  auto body = [&](int t, int buf, float (&rc)[NR], float& tc, float (&rn)[NR], float& tn) {
+    // ---- largest magnitude per vector (|x| bits compare as unsigned) ----
+    if (loader) {
+      unsigned mx = 0u;
+#pragma unroll
+      for (int i = 0; i < NR; ++i)
+        if (lro + 32 * i < n) mx = max(mx, __float_as_uint(rc[i]) & 0x7fffffffu);
+      if (TAIL && lro < 2 && 128 * NCH + lro < n) mx = max(mx, __float_as_uint(tc) & 0x7fffffffu);
+      if (EC) {
+#pragma unroll
+        for (int off = 16; off > 0; off >>= 1) mx = max(mx, (unsigned)__shfl_xor((int)mx, off, 32));
+      } else {
+        mx = max(mx, (unsigned)__shfl_xor((int)mx, 16));
+        mx = max(mx, (unsigned)__shfl_xor((int)mx, 32));
+      }
+      if (EC ? lro == 0 : g4 == 0) atomicMax(&smax[buf][lv], mx);
+      // next tile's loads overlap the rest of this one (the last re-reads itself)
+      load(t + G < ntiles ? t + G : t, rn, tn);
+    }
+    lds_barrier();
+    // ---- digits of this tile into LDS ----
+    if (loader) {
+      const unsigned mb = smax[buf][lv];
+      const bool bad = mb >= 0x7f800000u;
+      const int s = (mb == 0u || bad) ? 0 : 156 - (int)(mb >> 23);  // max|x| * 2^s < 2^30
+      if (EC ? lro == 0 : (wave == 0 && g4 == 0)) sexp[buf][lv] = bad ? kBad : s;
+      unsigned* dst = reinterpret_cast<unsigned*>(dl + buf * DBUF + lhalf * 4 * PLANE + lv * RSB) + lm;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        unsigned d[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = 4 * c + i;
+          const float x = lro + 32 * r < n ? rc[r] : 0.0f;
+          const int X = (int)ldexpf(x, s);
+          d[i] = ((unsigned)X + 0x808080u) ^ 0x808080u;  // bytes = signed digits
+        }
+        const unsigned p01l = __builtin_amdgcn_perm(d[1], d[0], 0x05010400u);
+        const unsigned p01h = __builtin_amdgcn_perm(d[1], d[0], 0x07030602u);
+        const unsigned p23l = __builtin_amdgcn_perm(d[3], d[2], 0x05010400u);
+        const unsigned p23h = __builtin_amdgcn_perm(d[3], d[2], 0x07030602u);
+        // digit plane p holds byte 3-p of every element
+        dst[(0 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23h, p01h, 0x07060302u);
+        dst[(1 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23h, p01h, 0x05040100u);
+        dst[(2 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23l, p01l, 0x07060302u);
+        dst[(3 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23l, p01l, 0x05040100u);
+      }
+      if (TAIL && lro < 2) stail[buf][lro][lv] = 128 * NCH + lro < n ? tc : 0.0f;
+    }
+    if (tid < 16) smax[buf ^ 1][tid] = 0u;
+    lds_barrier();
+    if (!mw) return;
+    // ---- products: ten digit pairs into four accumulators by weight ----
+    i32x4 acc[2][4];
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf)
+#pragma unroll
+      for (int lv_ = 0; lv_ < 4; ++lv_) acc[hf][lv_] = i32x4{0, 0, 0, 0};
+    const unsigned char* xb = dl + buf * DBUF + l16 * RSB + 16 * g4;
+    constexpr int PI[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};  // matrix digit
+    constexpr int PJ[10] = {3, 2, 1, 0, 2, 1, 0, 1, 0, 0};  // data digit
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf) {
+        i32x4 xd[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          xd[p] = *reinterpret_cast<const i32x4*>(xb + (hf * 4 + p) * PLANE + c * 64);
+#pragma unroll
+        for (int u = 0; u < 10; ++u) {
+          const int L = PI[u] + PJ[u];
+          acc[hf][L] = EC ? __builtin_amdgcn_mfma_i32_16x16x64_i8(md[hf][PI[u]][c], xd[PJ[u]],
+                                                                 acc[hf][L], 0, 0, 0)
+                          : __builtin_amdgcn_mfma_i32_16x16x64_i8(xd[PJ[u]], md[hf][PI[u]][c],
+                                                                 acc[hf][L], 0, 0, 0);
+        }
+      }
+    // ---- out[k] = E + O, out[n-1-k] = E - O ----
+    // Accumulator register r of lane l holds row 4*(l/16) + r, column l%16:
+    // ELEM_CONTIG rows are output rows (4 adjacent k per lane), otherwise
+    // rows are vectors (4 adjacent vectors per lane).
+    const float qnan = __int_as_float(0x7fc00000);
+    if (EC) {
+      const int v = l16, gv = t * 16 + v;
+      if (gv < nvec_total) {
+        const int sv = sexp[buf][v];
+        const int p = gv / nvec, q = gv - p * nvec;
+        float* o = out + (long)p * ops + (long)q * ovs;
+        const int k0 = 16 * wave + 4 * g4;
+        f32x4 vp, vm;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int k = k0 + r;
+          double e = combine(acc[0], r, sv), od = combine(acc[1], r, sv);
+          if (TAIL) {
+            e = fma(smt[0][k], (double)stail[buf][0][v], e);
+            od = fma(smt[1][k], (double)stail[buf][1][v], od);
+          }
+          vp[r] = sv == kBad ? qnan : (float)(e + od);
+          vm[r] = sv == kBad ? qnan : (float)(e - od);
+        }
+        if (k0 + 3 < h && n - 4 - k0 > k0 + 3) {
+          *reinterpret_cast<f32x4u*>(o + k0) = vp;
+          *reinterpret_cast<f32x4u*>(o + (n - 4 - k0)) = f32x4{vm[3], vm[2], vm[1], vm[0]};
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int k = k0 + r;
+            if (k < h) {
+              o[k] = vp[r];
+              if (n - 1 - k != k) o[n - 1 - k] = vm[r];
+            }
+          }
+        }
+      }
+    } else {
+      const int k = 16 * wave + l16;
+      if (k < h) {
+        const int v0 = 4 * g4, gv0 = t * 16 + v0;
+        f32x4 vp, vm;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int sv = sexp[buf][v0 + r];
+          double e = combine(acc[0], r, sv), od = combine(acc[1], r, sv);
+          if (TAIL) {
+            e = fma(smt[0][k], (double)stail[buf][0][v0 + r], e);
+            od = fma(smt[1][k], (double)stail[buf][1][v0 + r], od);
+          }
+          vp[r] = sv == kBad ? qnan : (float)(e + od);
+          vm[r] = sv == kBad ? qnan : (float)(e - od);
+        }
+        const int k2 = n - 1 - k;
+        const int p0 = gv0 < nvec_total ? gv0 / nvec : 0, q0 = gv0 - p0 * nvec;
+        if (gv0 + 3 < nvec_total && q0 + 3 < nvec) {
+          float* o = out + (long)p0 * ops + q0;
+          *reinterpret_cast<f32x4u*>(o + (long)k * oes) = vp;
+          if (k2 != k) *reinterpret_cast<f32x4u*>(o + (long)k2 * oes) = vm;
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int gv = gv0 + r;
+            if (gv < nvec_total) {
+              const int p = gv / nvec, q = gv - p * nvec;
+              float* o = out + (long)p * ops + q;
+              o[(long)k * oes] = vp[r];
+              if (k2 != k) o[(long)k2 * oes] = vm[r];
+            }
+          }
+        }
+      }
+    }
+  };
+
+  float ra[NR], rb[NR];
+  float ta = 0.0f, tb = 0.0f;
+  int tile = xcd_tile(blockIdx.x, G);
+  if (tile < ntiles && loader) load(tile, ra, ta);
+  lds_barrier();
+  // two tiles per trip so the register sets alternate without copies
+  while (tile < ntiles) {
+    body(tile, 0, ra, ta, rb, tb);
+    tile += G;
+    if (tile >= ntiles) break;
+    body(tile, 1, rb, tb, ra, ta);
+    tile += G;
+  }
+}
+
 int cu_count() {
   static int cus = 0;
   if (cus == 0) {
@@ -530,6 +792,46 @@ int launch_v2(const PairedMatrix* m, int dir, const TS* in, long ies, long ivs, 
   return NFGS_OK;
 }
 
+// Exact-integer FP32 transform (paired_transform_i8); one persistent
+// workgroup per tile stream, as many per CU as fit.
+template <bool EC, int NCH, bool TAIL>
+int launch_i8_t(const PairedMatrix* m, int dir, const float* in, long ies, long ivs, long ips,
+                float* out, long oes, long ovs, long ops, int nvec, int total, hipStream_t s) {
+  const int nthr = 64 * std::max(8, m->hb);
+  static int per_cu[2] = {0, 0};
+  int& pc = per_cu[nthr > 512 ? 1 : 0];
+  if (pc == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, paired_transform_i8<EC, NCH, TAIL>,
+                                                     nthr, 0) != hipSuccess ||
+        nb < 1)
+      nb = 1;
+    pc = nb;
+  }
+  const int ntiles = (total + 15) / 16;
+  const int grid = std::min(ntiles, pc * cu_count());
+  hipLaunchKernelGGL((paired_transform_i8<EC, NCH, TAIL>), dim3(grid), dim3(nthr), 0, s, in, ies,
+                     ivs, ips, out, oes, ovs, ops, static_cast<const i32x4*>(m->dig[dir]),
+                     static_cast<const double*>(m->mtail[dir]), m->shift[dir], m->n, m->h, m->hb,
+                     nvec, total);
+  NFGS_LAUNCH_CHECK();
+  return NFGS_OK;
+}
+
+template <bool EC>
+int launch_i8(const PairedMatrix* m, int dir, const float* in, long ies, long ivs, long ips,
+              float* out, long oes, long ovs, long ops, int nvec, int total, hipStream_t s) {
+#define NFGS_I8(NCH_, TAIL_)                                                                  \
+  return launch_i8_t<EC, NCH_, TAIL_>(m, dir, in, ies, ivs, ips, out, oes, ovs, ops, nvec, total, s)
+  if (m->tail) {
+    if (m->nch == 1) NFGS_I8(1, true);
+    NFGS_I8(2, true);
+  }
+  if (m->nch == 1) NFGS_I8(1, false);
+  NFGS_I8(2, false);
+#undef NFGS_I8
+}
+
 // Storage type TS (the plan dtype) with arithmetic type T.  FP32 plans run
 // the products in FP64 (T = double): the matrices are FP64, elements are
 // widened on their way into LDS and each output is rounded once when it is
@@ -547,6 +849,14 @@ int launch_t(const PairedMatrix* m, int dir, const void* in, long ies, long ivs,
   TS* to = static_cast<TS*>(out);
   const T* et = static_cast<const T*>(m->et[dir]);
   const T* ot = static_cast<const T*>(m->ot[dir]);
+  if constexpr (std::is_same<TS, float>::value) {
+    if (m->hb > 0 && total < (1L << 31)) {
+      if (ies == 1 && oes == 1)
+        return launch_i8<true>(m, dir, ti, ies, ivs, ips, to, oes, ovs, ops, nvec, (int)total, s);
+      if (ivs == 1 && ovs == 1)
+        return launch_i8<false>(m, dir, ti, ies, ivs, ips, to, oes, ovs, ops, nvec, (int)total, s);
+    }
+  }
   if (ies == 1 && oes == 1) {
     const int rc =
         launch_v2<T, true, TS>(m, dir, ti, ies, ivs, ips, to, oes, ovs, ops, nvec, total, s);

@@ -475,6 +475,10 @@ class NFGSSolverGPU:
                                      dtype=torch.float64)
         self._metrics = torch.zeros(L.NQ, device=self.device, dtype=torch.float64)
         self._metrics_host = torch.zeros(L.NQ, dtype=torch.float64).pin_memory()
+        # monitors of in-flight steps: one pinned copy and one event each
+        self._metrics_ring = [torch.zeros(L.NQ, dtype=torch.float64).pin_memory()
+                              for _ in range(2)]
+        self._metrics_events = [None, None]
 
     def _configure_gaussian_diagnostics(self, g, h_target):
         o = self.opts
@@ -584,7 +588,8 @@ class NFGSSolverGPU:
             L.ptr(self.Phi_above_bottom), L.ptr(self.Az_bottom), L.ptr(self._phi_bottom_hat),
             L.ptr(self._q_hat) if self._q_hat is not None else None, L.ptr(self._az_bottom_hat),
             L.ptr(self._h_profile))
-        # work buffers and two (Phi_c, Az) slots whose fixed planes never change
+        # work buffers and three (Phi_c, Az) slots whose fixed planes never
+        # change: the best state, the last state read and one step in flight
         ip = nyi * nxi
         self._ws_t = dict(jp=self._empty(3, Nz, Ny, Nx), s_bcz=self._empty(N),
                           s_jz=self._empty(Nz * ip), hat0=self._empty(N), hat1=self._empty(N),
@@ -593,7 +598,7 @@ class NFGSSolverGPU:
                                  ("jp", "s_bcz", "s_jz", "hat0", "hat1", "cscr")),
                                L.ptr(self._partials), L.ptr(self._metrics))
         self._slots = []
-        for _ in range(2):
+        for _ in range(3):
             phi = self._zeros(Nz, Ny, Nx)
             az = self._zeros(Nz, Ny, Nx)
             phi[0].copy_(self.Phi_above_bottom)
@@ -628,7 +633,22 @@ class NFGSSolverGPU:
     def _read_metrics(self):
         self._metrics_host.copy_(self._metrics, non_blocking=True)
         torch.cuda.current_stream(self.device).synchronize()
-        m = self._metrics_host.numpy()
+        return self._decode_metrics(self._metrics_host.numpy())
+
+    def _post_metrics(self, k: int):
+        """Queue a copy of the monitors of the step just enqueued into host
+        buffer k; the host reads it later with _take_metrics(k)."""
+        self._metrics_ring[k].copy_(self._metrics, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.device))
+        self._metrics_events[k] = ev
+
+    def _take_metrics(self, k: int):
+        self._metrics_events[k].synchronize()
+        self._metrics_events[k] = None
+        return self._decode_metrics(self._metrics_ring[k].numpy())
+
+    def _decode_metrics(self, m):
         Nz, Ny, Nx = self.Nz, self.Ny, self.Nx
         N = float(Nz * Ny * Nx)
         sigma = m[L.Q_SIGMA_NUM] / max(m[L.Q_SIGMA_DEN], 1.0e-30)
@@ -770,10 +790,27 @@ class NFGSSolverGPU:
         reason = "max_iter"
         final_iter = 0
         J_cb = None
+        # The loop keeps one step in flight: step n+1 is enqueued before the
+        # monitors of step n are read, so the device never waits on the host.
+        # Step n+1 goes into a slot that is neither the best state nor step
+        # n's, so whichever the stop rules keep survives it.  A callback
+        # needs B and J of the step it sees, so then steps run one at a time.
+        ahead = per_step_callback is None
+
+        def launch(n_step, avoid):
+            s_new = next(k for k in range(3) if k not in avoid)
+            self._advance(s_new, zero_source=False, want_b=not ahead)
+            self._post_metrics(n_step & 1)
+            return s_new
+
+        slot_next = launch(1, {best_slot}) if int(max_iter) >= 1 else best_slot
         for n in range(1, int(max_iter) + 1):
-            slot = 1 - best_slot
-            self._advance(slot, zero_source=False, want_b=per_step_callback is not None)
-            sigma, divB, energy, ratio = self._read_metrics()
+            if not ahead and n > 1:
+                slot_next = launch(n, {best_slot})
+            slot = slot_next
+            if ahead and n < int(max_iter):
+                slot_next = launch(n + 1, {best_slot, slot})
+            sigma, divB, energy, ratio = self._take_metrics(n & 1)
             self._check_zero_mode(ratio)
             final_slot = slot
             pe = history["energy"][-1] if history["energy"] else E0
