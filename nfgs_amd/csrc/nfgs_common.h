@@ -139,6 +139,16 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// element at byte offset o of a plane (uniform base + 32-bit lane offset)
+template <typename T>
+__device__ __forceinline__ T ld(const T* base, unsigned o) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + o);
+}
+template <typename T>
+__device__ __forceinline__ void st(T* base, unsigned o, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + o) = v;
+}
+
 // Workgroups of a launch are dispatched round-robin over the 8 XCDs, each
 // with its own L2.  xcd_tile maps launch-order workgroup w of n to a logical
 // tile so that every XCD owns one contiguous range of tiles: neighbouring

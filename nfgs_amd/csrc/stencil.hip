@@ -124,16 +124,6 @@ __device__ __forceinline__ Halo make_halo(const Tile& t, const Grid<T>& g) {
   return h;
 }
 
-// element at byte offset o of a plane (uniform base + 32-bit lane offset)
-template <typename T>
-__device__ __forceinline__ T ld(const T* base, unsigned o) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + o);
-}
-template <typename T>
-__device__ __forceinline__ void st(T* base, unsigned o, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + o) = v;
-}
-
 template <typename T>
 __device__ __forceinline__ void fetch(Fetch<T>& r, const T* plane, const Halo& h) {
 #pragma unroll
