@@ -16,12 +16,15 @@ struct PairedMatrix {
   void* ot[2] = {nullptr, nullptr};  // [direction] O^T
   // Signed base-256 digit form of E and O for the exact-integer FP32 kernel
   // (FP32 plans with n <= 258): K chunks of 64 paired rows, an optional tail
-  // row kept in FP64, 16-row output blocks.  dig is laid out as the MFMA
-  // operand registers of each block; entries are round(T * 2^shift).
-  int nch = 0, tail = 0, hb = 0;
+  // row kept in FP64, at most 8 16-row output blocks plus an optional
+  // remainder output row 16*hb (h = 16*hb + 1) computed in FP64.  dig is
+  // laid out as the MFMA operand registers of each block; entries are
+  // round(T * 2^shift).
+  int nch = 0, tail = 0, hb = 0, rem = 0;
   int shift[2] = {0, 0};
   void* dig[2] = {nullptr, nullptr};    // [direction] int8 [half][digit][chunk][block][lane][16]
   void* mtail[2] = {nullptr, nullptr};  // [direction] double [half][16*hb] tail row
+  void* mrem[2] = {nullptr, nullptr};   // [direction] double [n] row 16*hb of T
 };
 
 struct nfgs_plan {

@@ -48,15 +48,18 @@ int upload_dtype(int dtype, const std::vector<double>& host, void** dev) {
 // written as four signed base-256 digits; digit p of lane l of 16-row block b,
 // K chunk c, holds in byte 4w+i the paired row j' = 4(l/16) + w + 16(4c+i)
 // of output row k = 16b + l%16 (the order in which the kernel packs the
-// vector digits).  Paired rows past 64*nch (at most one) form the FP64 tail.
+// vector digits).  Paired rows past 64*nch (at most one) form the FP64 tail;
+// when h = 16*hb + 1 the last output row is kept whole (mrem) in FP64.
 int build_digits(int kind, int n, PairedMatrix* m) {
   const int h = m->h, ne = m->rows;
   const int nch = std::max(1, (ne + 62) / 64);
-  const int hb = (h + 15) / 16;
-  if (nch > 2 || hb > 9) return NFGS_OK;  // longer vectors keep the FP64 kernel
+  const int rem = (h > 16 && h % 16 == 1) ? 1 : 0;
+  const int hb = rem ? h / 16 : (h + 15) / 16;
+  if (nch > 2 || hb > 8) return NFGS_OK;  // longer vectors keep the FP64 kernel
   m->nch = nch;
   m->tail = ne > 64 * nch ? 1 : 0;
   m->hb = hb;
+  m->rem = rem;
   const double inv_scale = kind == NFGS_DCT1 ? 1.0 / (2.0 * (n - 1)) : 1.0 / (2.0 * (n + 1));
   for (int dir = 0; dir < 2; ++dir) {
     const double sc = dir == NFGS_FORWARD ? 1.0 : inv_scale;
@@ -95,7 +98,12 @@ int build_digits(int kind, int n, PairedMatrix* m) {
     if (m->tail)
       for (int half = 0; half < 2; ++half)
         for (int k = 0; k < 16 * hb; ++k) mt[(size_t)half * 16 * hb + k] = entry(half, k, 64 * nch);
-    const int rc = upload<double>(mt, &m->mtail[dir]);
+    int rc = upload<double>(mt, &m->mtail[dir]);
+    if (rc) return rc;
+    std::vector<double> mr((size_t)n, 0.0);
+    if (rem)
+      for (int j = 0; j < n; ++j) mr[j] = sc * type1_entry(kind, n, 16 * hb, j);
+    rc = upload<double>(mr, &m->mrem[dir]);
     if (rc) return rc;
   }
   return NFGS_OK;
@@ -135,7 +143,8 @@ void free_paired(PairedMatrix* m) {
     if (m->ot[d]) (void)hipFree(m->ot[d]);
     if (m->dig[d]) (void)hipFree(m->dig[d]);
     if (m->mtail[d]) (void)hipFree(m->mtail[d]);
-    m->et[d] = m->ot[d] = m->dig[d] = m->mtail[d] = nullptr;
+    if (m->mrem[d]) (void)hipFree(m->mrem[d]);
+    m->et[d] = m->ot[d] = m->dig[d] = m->mtail[d] = m->mrem[d] = nullptr;
   }
 }
 

@@ -483,7 +483,8 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
 // FP32, so the result is as accurate as an FP64-arithmetic product (the
 // dropped pairs weigh about 2^-30 of the vector's largest element) while the
 // products run at int8 rate.  A tail paired row (n = 128*nch + 1 or + 2) is
-// added in FP64.
+// added in FP64, and so is a remainder output row h-1 when h = 16*hb + 1: it
+// is a dot product of the raw elements with row h-1 of T.
 //
 // Loader lanes are (vector v, offset ro < 32) pairs holding elements
 // j = ro + 32r: all of one parity, so four of them (r = 4c..4c+3) pack into
@@ -494,33 +495,58 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
 // 16 vectors times 4 offsets per wavefront (coalesced across vectors).  Wave w
 // owns output rows [16w, 16w+16) with its matrix digits in registers; the
 // data digits of a tile are read from LDS by every wave.
+//
+// The tiles of a workgroup run as a pipeline with one barrier per step: step
+// i computes the products of tile i, the digits of tile i+1, the per-vector
+// maxima of tile i+2 and issues the loads of tile i+4 (four register sets,
+// rotated by unrolling the step loop four times).  Addresses of a full tile
+// are a uniform base (its first vector) plus a lane offset that does not
+// change between tiles except across a plane boundary.  FULL (n >= 128*nch)
+// drops the masks of elements past n.
 // ---------------------------------------------------------------------------
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef f32x4 f32x4u __attribute__((aligned(4)));
 
-template <bool EC, int NCH, bool TAIL>
-__global__ __launch_bounds__(576) void paired_transform_i8(
+// (plane, vector) of the first vector of a workgroup's successive tiles
+struct TileCursor {
+  int p, q;
+  __device__ __forceinline__ void next(int dp, int dq, int nvec) {
+    p += dp;
+    q += dq;
+    if (q >= nvec) {
+      q -= nvec;
+      ++p;
+    }
+  }
+};
+
+template <bool EC, int NCH, bool TAIL, bool REM, bool FULL>
+__global__ __launch_bounds__(512) void paired_transform_i8(
     const float* __restrict__ in, long ies, long ivs, long ips, float* __restrict__ out,
     long oes, long ovs, long ops, const i32x4* __restrict__ dig, const double* __restrict__ mtail,
-    int shift, int n, int h, int hb, int nvec, int nvec_total) {
+    const double* __restrict__ mrem, int shift, int n, int h, int hb, int nvec, int nvec_total) {
   constexpr int RSB = 64 * NCH + 16;  // bytes per vector row of one digit plane
   constexpr int PLANE = 16 * RSB;     // one (half, digit) plane of a tile
   constexpr int DBUF = 8 * PLANE;     // one tile: 2 halves x 4 digits
   constexpr int NR = 4 * NCH;         // main elements per loader lane
-  constexpr int kBad = -0x40000000;   // exponent marking a non-finite vector
+  constexpr int NJ = 128 * NCH + 2;   // elements a lane set can hold
   __shared__ __align__(16) unsigned char dl[2 * DBUF];
-  __shared__ unsigned smax[2][16];
-  __shared__ int sexp[2][16];
+  __shared__ unsigned smax[3][16];
+  __shared__ double ssc[2][16];  // 2^-(fixed-point exponent), NaN for non-finite vectors
   __shared__ float stail[2][2][16];
-  __shared__ double smt[2][16 * 9];
+  __shared__ double smt[2][16 * 8];
+  __shared__ double smr[REM ? NJ : 1];
+  __shared__ double rpart[REM && !EC ? 2 : 1][8][2][16];
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int l16 = lane & 15, g4 = lane >> 4;
-  const bool loader = wave < 8;
   const int lv = EC ? 2 * wave + (lane >> 5) : l16;
   const int lro = EC ? (lane & 31) : 4 * wave + g4;
   const int lhalf = lro & 1, lm = lro >> 1;
   const bool mw = wave < hb;
+  const int krem = 16 * hb;
+  const double qnan = __longlong_as_double(0x7ff8000000000000LL);
 
   i32x4 md[2][4][NCH];
 #pragma unroll
@@ -533,201 +559,291 @@ __global__ __launch_bounds__(576) void paired_transform_i8(
                           : i32x4{0, 0, 0, 0};
   if (TAIL)
     for (int e = tid; e < 32 * hb; e += blockDim.x) smt[e / (16 * hb)][e % (16 * hb)] = mtail[e];
-  if (tid < 32) smax[tid >> 4][tid & 15] = 0u;
+  if (REM)
+    for (int j = tid; j < NJ; j += blockDim.x) smr[j] = j < n ? mrem[j] : 0.0;
+  if (tid < 48) smax[tid >> 4][tid & 15] = 0u;
+  // The set-up loads are complete before the pipeline starts (vmcnt(0)), so
+  // the waits inside it count only tile loads.
+  __builtin_amdgcn_s_waitcnt(0x0F70);
 
   const int ntiles = (nvec_total + 15) >> 4;
   const int G = gridDim.x;
+  const int tile0 = xcd_tile(blockIdx.x, G);
+  const int cnt = tile0 < ntiles ? (ntiles - 1 - tile0) / G + 1 : 0;
+  // tile cursors for the loads, the digits (remainder row) and the products
+  const int dp = (int)((16L * G) / nvec), dq = (int)(16L * G - (long)dp * nvec);
+  TileCursor c0;
+  c0.p = (tile0 * 16) / nvec;
+  c0.q = tile0 * 16 - c0.p * nvec;
+  TileCursor cl = c0, cd = c0, cp = c0;
+  // a full tile whose vectors cross at most one plane boundary takes the
+  // uniform-base addressing
+  auto full_tile = [&](int j) { return nvec >= 16 && (tile0 + j * G) * 16 + 15 < nvec_total; };
+  const long iwrap = ips - (long)nvec * ivs, owrap = ops - (long)nvec * ovs;
 
-  // Loads are unconditional: lanes past the last vector read vector 0 and
-  // elements past n re-read element n-1 (masked when digits are formed).
-  auto load = [&](int t, float (&r)[NR], float& rt) {
-    int gv = t * 16 + lv;
-    gv = gv < nvec_total ? gv : 0;
+  auto voff = [&](int gv, long ps, long vs) {
     const int p = gv / nvec, q = gv - p * nvec;
-    const float* src = in + (long)p * ips + (long)q * ivs;
+    return (long)p * ps + (long)q * vs;
+  };
+  const int etail = min(128 * NCH + lro, n - 1);  // tail element (clamped)
+  // Loads of the j-th tile of this workgroup.  Lanes past the last vector
+  // read vector 0 and elements past n re-read element n-1 (masked when
+  // digits are formed).  Every step issues the same loads, so the waits on
+  // them are counted exactly; steps past the last tile read vector 0.
+  auto load = [&](int j, float (&r)[NR], float& rt) {
+    // byte offset of element ro of the lane's vector (arrays below 4 GiB)
+    unsigned o;
+    if (FULL && full_tile(j)) {
+      o = (unsigned)(((long)cl.p * ips + (long)(cl.q + lv) * ivs + (cl.q + lv >= nvec ? iwrap : 0L) +
+                      (long)lro * ies) * 4);
+    } else {
+      int gv = (tile0 + j * G) * 16 + lv;
+      gv = (j < cnt && gv < nvec_total) ? gv : 0;
+      o = (unsigned)((voff(gv, ips, ivs) + (long)lro * ies) * 4);
+    }
+    const unsigned es = (unsigned)(ies * 4);
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
-      const int j = lro + 32 * i;
-      r[i] = src[(long)(j < n ? j : n - 1) * ies];
+      const int e = lro + 32 * i;
+      r[i] = ld(in, o + (unsigned)((FULL || e < n ? 32 * i : n - 1 - lro)) * es);
     }
-    if (TAIL) {
-      const int j = 128 * NCH + lro;
-      rt = src[(long)(j < n ? j : n - 1) * ies];
+    if (TAIL || REM) rt = ld(in, o + (unsigned)(etail - lro) * es);
+    cl.next(dp, dq, nvec);
+  };
+  auto has_tail_elem = [&]() { return (TAIL || REM) && lro < 2 && 128 * NCH + lro < n; };
+  auto valid = [&](int i) { return FULL || lro + 32 * i < n; };
+
+  // ---- largest magnitude per vector (|x| bits compare as unsigned) ----
+  auto vmax = [&](int j, const float (&r)[NR], float rt) {
+    unsigned mx = 0u;
+#pragma unroll
+    for (int i = 0; i < NR; ++i)
+      if (valid(i)) mx = max(mx, __float_as_uint(r[i]) & 0x7fffffffu);
+    if (has_tail_elem()) mx = max(mx, __float_as_uint(rt) & 0x7fffffffu);
+    if (EC) {
+#pragma unroll
+      for (int off = 16; off > 0; off >>= 1) mx = max(mx, (unsigned)__shfl_xor((int)mx, off, 32));
+    } else {
+      mx = max(mx, (unsigned)__shfl_xor((int)mx, 16));
+      mx = max(mx, (unsigned)__shfl_xor((int)mx, 32));
     }
+    if (EC ? lro == 0 : g4 == 0) atomicMax(&smax[j % 3][lv], mx);
   };
 
-  auto combine = [&](const i32x4 (&a)[4], int r, int sv) -> double {
-    const int t01 = a[0][r] * 256 + a[1][r];
-    const int t23 = a[2][r] * 256 + a[3][r];
-    return ldexp(fma((double)t01, 65536.0, (double)t23), 24 - shift - sv);
-  };
-
-  auto body = [&](int t, int buf, float (&rc)[NR], float& tc, float (&rn)[NR], float& tn) {
-    // ---- largest magnitude per vector (|x| bits compare as unsigned) ----
-    if (loader) {
-      unsigned mx = 0u;
+  // ---- digits of tile j into LDS (and the remainder row) ----
+  auto digits = [&](int j, const float (&r)[NR], float rt) {
+    const int buf = j & 1;
+    const unsigned mb = smax[j % 3][lv];
+    const bool bad = mb >= 0x7f800000u;
+    const int s = (mb == 0u || bad) ? 0 : 156 - (int)(mb >> 23);  // max|x| * 2^s < 2^30
+    if (EC ? lro == 0 : (wave == 0 && g4 == 0))
+      ssc[buf][lv] = bad ? qnan : ldexp(1.0, 24 - shift - s);
+    unsigned* dst = reinterpret_cast<unsigned*>(dl + buf * DBUF + lhalf * 4 * PLANE + lv * RSB) + lm;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      unsigned d[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int ri = 4 * c + i;
+        const float x = valid(ri) ? r[ri] : 0.0f;
+        const int X = (int)ldexpf(x, s);
+        d[i] = ((unsigned)X + 0x808080u) ^ 0x808080u;  // bytes = signed digits
+      }
+      const unsigned p01l = __builtin_amdgcn_perm(d[1], d[0], 0x05010400u);
+      const unsigned p01h = __builtin_amdgcn_perm(d[1], d[0], 0x07030602u);
+      const unsigned p23l = __builtin_amdgcn_perm(d[3], d[2], 0x05010400u);
+      const unsigned p23h = __builtin_amdgcn_perm(d[3], d[2], 0x07030602u);
+      // digit plane p holds byte 3-p of every element
+      dst[(0 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23h, p01h, 0x07060302u);
+      dst[(1 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23h, p01h, 0x05040100u);
+      dst[(2 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23l, p01l, 0x07060302u);
+      dst[(3 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23l, p01l, 0x05040100u);
+    }
+    if (TAIL && lro < 2) stail[buf][lro][lv] = has_tail_elem() ? rt : 0.0f;
+    if (REM) {
+      // remainder row: FP64 dot product of the raw elements, per parity
+      double acc = 0.0;
 #pragma unroll
       for (int i = 0; i < NR; ++i)
-        if (lro + 32 * i < n) mx = max(mx, __float_as_uint(rc[i]) & 0x7fffffffu);
-      if (TAIL && lro < 2 && 128 * NCH + lro < n) mx = max(mx, __float_as_uint(tc) & 0x7fffffffu);
+        acc = fma(smr[lro + 32 * i], (double)(valid(i) ? r[i] : 0.0f), acc);
+      if (has_tail_elem()) acc = fma(smr[128 * NCH + lro], (double)rt, acc);
       if (EC) {
+        // same-parity lanes of the half-wavefront, then E (even) with O (odd)
 #pragma unroll
-        for (int off = 16; off > 0; off >>= 1) mx = max(mx, (unsigned)__shfl_xor((int)mx, off, 32));
+        for (int off = 16; off > 1; off >>= 1) acc += __shfl_xor(acc, off, 32);
+        const double other = __shfl_xor(acc, 1, 32);
+        const int gv = (tile0 + j * G) * 16 + lv;
+        if (lro == 0 && gv < nvec_total) {
+          const long vo = full_tile(j) ? (long)cd.p * ops + (long)(cd.q + lv) * ovs +
+                                             (cd.q + lv >= nvec ? owrap : 0L)
+                                       : voff(gv, ops, ovs);
+          const float nanf = __int_as_float(0x7fc00000);
+          st(out, (unsigned)((vo + krem) * 4), bad ? nanf : (float)(acc + other));
+          if (n - 1 - krem != krem)
+            st(out, (unsigned)((vo + n - 1 - krem) * 4), bad ? nanf : (float)(acc - other));
+        }
       } else {
-        mx = max(mx, (unsigned)__shfl_xor((int)mx, 16));
-        mx = max(mx, (unsigned)__shfl_xor((int)mx, 32));
+        acc += __shfl_xor(acc, 32);  // offsets g4 and g4 + 2 share a parity
+        if (g4 < 2) rpart[buf][wave][g4][lv] = acc;
       }
-      if (EC ? lro == 0 : g4 == 0) atomicMax(&smax[buf][lv], mx);
-      // next tile's loads overlap the rest of this one (the last re-reads itself)
-      load(t + G < ntiles ? t + G : t, rn, tn);
     }
-    lds_barrier();
-    // ---- digits of this tile into LDS ----
-    if (loader) {
-      const unsigned mb = smax[buf][lv];
-      const bool bad = mb >= 0x7f800000u;
-      const int s = (mb == 0u || bad) ? 0 : 156 - (int)(mb >> 23);  // max|x| * 2^s < 2^30
-      if (EC ? lro == 0 : (wave == 0 && g4 == 0)) sexp[buf][lv] = bad ? kBad : s;
-      unsigned* dst = reinterpret_cast<unsigned*>(dl + buf * DBUF + lhalf * 4 * PLANE + lv * RSB) + lm;
-#pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        unsigned d[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int r = 4 * c + i;
-          const float x = lro + 32 * r < n ? rc[r] : 0.0f;
-          const int X = (int)ldexpf(x, s);
-          d[i] = ((unsigned)X + 0x808080u) ^ 0x808080u;  // bytes = signed digits
-        }
-        const unsigned p01l = __builtin_amdgcn_perm(d[1], d[0], 0x05010400u);
-        const unsigned p01h = __builtin_amdgcn_perm(d[1], d[0], 0x07030602u);
-        const unsigned p23l = __builtin_amdgcn_perm(d[3], d[2], 0x05010400u);
-        const unsigned p23h = __builtin_amdgcn_perm(d[3], d[2], 0x07030602u);
-        // digit plane p holds byte 3-p of every element
-        dst[(0 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23h, p01h, 0x07060302u);
-        dst[(1 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23h, p01h, 0x05040100u);
-        dst[(2 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23l, p01l, 0x07060302u);
-        dst[(3 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23l, p01l, 0x05040100u);
-      }
-      if (TAIL && lro < 2) stail[buf][lro][lv] = 128 * NCH + lro < n ? tc : 0.0f;
-    }
-    if (tid < 16) smax[buf ^ 1][tid] = 0u;
-    lds_barrier();
-    if (!mw) return;
-    // ---- products: ten digit pairs into four accumulators by weight ----
-    i32x4 acc[2][4];
-#pragma unroll
-    for (int hf = 0; hf < 2; ++hf)
-#pragma unroll
-      for (int lv_ = 0; lv_ < 4; ++lv_) acc[hf][lv_] = i32x4{0, 0, 0, 0};
-    const unsigned char* xb = dl + buf * DBUF + l16 * RSB + 16 * g4;
-    constexpr int PI[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};  // matrix digit
-    constexpr int PJ[10] = {3, 2, 1, 0, 2, 1, 0, 1, 0, 0};  // data digit
-#pragma unroll
-    for (int c = 0; c < NCH; ++c)
-#pragma unroll
-      for (int hf = 0; hf < 2; ++hf) {
-        i32x4 xd[4];
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-          xd[p] = *reinterpret_cast<const i32x4*>(xb + (hf * 4 + p) * PLANE + c * 64);
-#pragma unroll
-        for (int u = 0; u < 10; ++u) {
-          const int L = PI[u] + PJ[u];
-          acc[hf][L] = EC ? __builtin_amdgcn_mfma_i32_16x16x64_i8(md[hf][PI[u]][c], xd[PJ[u]],
-                                                                 acc[hf][L], 0, 0, 0)
-                          : __builtin_amdgcn_mfma_i32_16x16x64_i8(xd[PJ[u]], md[hf][PI[u]][c],
-                                                                 acc[hf][L], 0, 0, 0);
-        }
-      }
-    // ---- out[k] = E + O, out[n-1-k] = E - O ----
-    // Accumulator register r of lane l holds row 4*(l/16) + r, column l%16:
-    // ELEM_CONTIG rows are output rows (4 adjacent k per lane), otherwise
-    // rows are vectors (4 adjacent vectors per lane).
-    const float qnan = __int_as_float(0x7fc00000);
-    if (EC) {
-      const int v = l16, gv = t * 16 + v;
+    cd.next(dp, dq, nvec);
+  };
+
+  // FP64 value of the four accumulators, times the vector's scale
+  auto combine = [&](const i32x4 (&a)[4], int r) -> double {
+    const int t01 = a[0][r] * 256 + a[1][r];
+    const int t23 = a[2][r] * 256 + a[3][r];
+    return fma((double)t01, 65536.0, (double)t23);
+  };
+
+  // ---- products of tile j and out[k] = E + O, out[n-1-k] = E - O ----
+  auto products = [&](int j) {
+    const int buf = j & 1;
+    const int t = tile0 + j * G;
+    const bool ft = full_tile(j);
+    const long obase = (long)cp.p * ops + (long)cp.q * ovs;  // first vector, when ft
+    if (REM && !EC && wave == 0 && lane < 16) {
+      const int gv = t * 16 + lane;
       if (gv < nvec_total) {
-        const int sv = sexp[buf][v];
-        const int p = gv / nvec, q = gv - p * nvec;
-        float* o = out + (long)p * ops + (long)q * ovs;
-        const int k0 = 16 * wave + 4 * g4;
-        f32x4 vp, vm;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int k = k0 + r;
-          double e = combine(acc[0], r, sv), od = combine(acc[1], r, sv);
-          if (TAIL) {
-            e = fma(smt[0][k], (double)stail[buf][0][v], e);
-            od = fma(smt[1][k], (double)stail[buf][1][v], od);
-          }
-          vp[r] = sv == kBad ? qnan : (float)(e + od);
-          vm[r] = sv == kBad ? qnan : (float)(e - od);
+        double e = 0.0, od = 0.0;
+        for (int w = 0; w < 8; ++w) {
+          e += rpart[buf][w][0][lane];
+          od += rpart[buf][w][1][lane];
         }
-        if (k0 + 3 < h && n - 4 - k0 > k0 + 3) {
-          *reinterpret_cast<f32x4u*>(o + k0) = vp;
-          *reinterpret_cast<f32x4u*>(o + (n - 4 - k0)) = f32x4{vm[3], vm[2], vm[1], vm[0]};
-        } else {
+        if (ssc[buf][lane] != ssc[buf][lane]) e = qnan;  // non-finite vector
+        const long vo = ft ? obase + lane + (cp.q + lane >= nvec ? owrap : 0L) : voff(gv, ops, 1);
+        st(out, (unsigned)((vo + (long)krem * oes) * 4), (float)(e + od));
+        if (n - 1 - krem != krem)
+          st(out, (unsigned)((vo + (long)(n - 1 - krem) * oes) * 4), (float)(e - od));
+      }
+    }
+    if (mw) {
+      i32x4 acc[2][4];
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[hf][q] = i32x4{0, 0, 0, 0};
+      const unsigned char* xb = dl + buf * DBUF + l16 * RSB + 16 * g4;
+      constexpr int PI[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};  // matrix digit
+      constexpr int PJ[10] = {3, 2, 1, 0, 2, 1, 0, 1, 0, 0};  // data digit
+#pragma unroll
+      for (int c = 0; c < NCH; ++c)
+#pragma unroll
+        for (int hf = 0; hf < 2; ++hf) {
+          i32x4 xd[4];
+#pragma unroll
+          for (int p = 0; p < 4; ++p)
+            xd[p] = *reinterpret_cast<const i32x4*>(xb + (hf * 4 + p) * PLANE + c * 64);
+#pragma unroll
+          for (int u = 0; u < 10; ++u) {
+            const int L = PI[u] + PJ[u];
+            acc[hf][L] = EC ? __builtin_amdgcn_mfma_i32_16x16x64_i8(md[hf][PI[u]][c], xd[PJ[u]],
+                                                                   acc[hf][L], 0, 0, 0)
+                            : __builtin_amdgcn_mfma_i32_16x16x64_i8(xd[PJ[u]], md[hf][PI[u]][c],
+                                                                   acc[hf][L], 0, 0, 0);
+          }
+        }
+      // Accumulator register r of lane l holds row 4*(l/16) + r, column l%16:
+      // ELEM_CONTIG rows are output rows (4 adjacent k per lane), otherwise
+      // rows are vectors (4 adjacent vectors per lane).
+      if (EC) {
+        const int v = l16, gv = t * 16 + v;
+        if (gv < nvec_total) {
+          const double sc = ssc[buf][v];
+          const int k0 = 16 * wave + 4 * g4;
+          f32x4 vp, vm;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int k = k0 + r;
-            if (k < h) {
-              o[k] = vp[r];
-              if (n - 1 - k != k) o[n - 1 - k] = vm[r];
+            double e = combine(acc[0], r) * sc, od = combine(acc[1], r) * sc;
+            if (TAIL) {
+              e = fma(smt[0][k], (double)stail[buf][0][v], e);
+              od = fma(smt[1][k], (double)stail[buf][1][v], od);
+            }
+            vp[r] = (float)(e + od);
+            vm[r] = (float)(e - od);
+          }
+          const long vo = ft ? obase + (long)v * ovs + (cp.q + v >= nvec ? owrap : 0L)
+                             : voff(gv, ops, ovs);
+          float* o = out + vo;
+          if (k0 + 3 < h && n - 4 - k0 > k0 + 3) {
+            st(reinterpret_cast<f32x4u*>(out), (unsigned)((vo + k0) * 4), vp);
+            st(reinterpret_cast<f32x4u*>(out), (unsigned)((vo + n - 4 - k0) * 4),
+               f32x4{vm[3], vm[2], vm[1], vm[0]});
+          } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int k = k0 + r;
+              if (k < h) {
+                o[k] = vp[r];
+                if (n - 1 - k != k) o[n - 1 - k] = vm[r];
+              }
             }
           }
         }
-      }
-    } else {
-      const int k = 16 * wave + l16;
-      if (k < h) {
-        const int v0 = 4 * g4, gv0 = t * 16 + v0;
-        f32x4 vp, vm;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int sv = sexp[buf][v0 + r];
-          double e = combine(acc[0], r, sv), od = combine(acc[1], r, sv);
-          if (TAIL) {
-            e = fma(smt[0][k], (double)stail[buf][0][v0 + r], e);
-            od = fma(smt[1][k], (double)stail[buf][1][v0 + r], od);
-          }
-          vp[r] = sv == kBad ? qnan : (float)(e + od);
-          vm[r] = sv == kBad ? qnan : (float)(e - od);
-        }
-        const int k2 = n - 1 - k;
-        const int p0 = gv0 < nvec_total ? gv0 / nvec : 0, q0 = gv0 - p0 * nvec;
-        if (gv0 + 3 < nvec_total && q0 + 3 < nvec) {
-          float* o = out + (long)p0 * ops + q0;
-          *reinterpret_cast<f32x4u*>(o + (long)k * oes) = vp;
-          if (k2 != k) *reinterpret_cast<f32x4u*>(o + (long)k2 * oes) = vm;
-        } else {
+      } else {
+        const int k = 16 * wave + l16;
+        if (k < h) {
+          const int v0 = 4 * g4, gv0 = t * 16 + v0;
+          f32x4 vp, vm;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int gv = gv0 + r;
-            if (gv < nvec_total) {
-              const int p = gv / nvec, q = gv - p * nvec;
-              float* o = out + (long)p * ops + q;
-              o[(long)k * oes] = vp[r];
-              if (k2 != k) o[(long)k2 * oes] = vm[r];
+            const double sc = ssc[buf][v0 + r];
+            double e = combine(acc[0], r) * sc, od = combine(acc[1], r) * sc;
+            if (TAIL) {
+              e = fma(smt[0][k], (double)stail[buf][0][v0 + r], e);
+              od = fma(smt[1][k], (double)stail[buf][1][v0 + r], od);
+            }
+            vp[r] = (float)(e + od);
+            vm[r] = (float)(e - od);
+          }
+          const int k2 = n - 1 - k;
+          // the four vectors sit in one plane unless a boundary falls inside them
+          const bool one = ft && (cp.q + v0 >= nvec || cp.q + v0 + 3 < nvec);
+          if (one) {
+            const long vo = obase + v0 + (cp.q + v0 >= nvec ? owrap : 0L);
+            st(reinterpret_cast<f32x4u*>(out), (unsigned)((vo + (long)k * oes) * 4), vp);
+            if (k2 != k) st(reinterpret_cast<f32x4u*>(out), (unsigned)((vo + (long)k2 * oes) * 4), vm);
+          } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int gv = gv0 + r;
+              if (gv < nvec_total) {
+                float* o = out + voff(gv, ops, 1);
+                o[(long)k * oes] = vp[r];
+                if (k2 != k) o[(long)k2 * oes] = vm[r];
+              }
             }
           }
         }
       }
     }
+    cp.next(dp, dq, nvec);
   };
 
-  float ra[NR], rb[NR];
-  float ta = 0.0f, tb = 0.0f;
-  int tile = xcd_tile(blockIdx.x, G);
-  if (tile < ntiles && loader) load(tile, ra, ta);
+  // One pipeline step: products of tile i, digits of i+1, maxima of i+2,
+  // loads of i+4.  Tile j's raw elements live in register set j % 4.
+  auto step = [&](int i, float (&rd)[NR], float& td, float (&rx)[NR], float& tx,
+                  float (&rl)[NR], float& tl) {
+    if (i >= 0 && i < cnt) products(i);
+    if (i + 1 >= 0 && i + 1 < cnt) digits(i + 1, rd, td);
+    if (i + 2 < cnt) vmax(i + 2, rx, tx);
+    if (i >= 0 && tid < 16) smax[i % 3][tid] = 0u;  // tile i+3 gathers here next step
+    load(i + 4, rl, tl);
+    lds_barrier();
+  };
+
+  float r0[NR], r1[NR], r2[NR], r3[NR];
+  float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
+  load(0, r0, t0);
+  load(1, r1, t1);
   lds_barrier();
-  // two tiles per trip so the register sets alternate without copies
-  while (tile < ntiles) {
-    body(tile, 0, ra, ta, rb, tb);
-    tile += G;
-    if (tile >= ntiles) break;
-    body(tile, 1, rb, tb, ra, ta);
-    tile += G;
+  for (int i = -2; i < cnt; i += 4) {
+    step(i, r3, t3, r0, t0, r2, t2);
+    step(i + 1, r0, t0, r1, t1, r3, t3);
+    step(i + 2, r1, t1, r2, t2, r0, t0);
+    step(i + 3, r2, t2, r3, t3, r1, t1);
   }
 }
 
@@ -792,43 +908,64 @@ int launch_v2(const PairedMatrix* m, int dir, const TS* in, long ies, long ivs, 
   return NFGS_OK;
 }
 
-// Exact-integer FP32 transform (paired_transform_i8); one persistent
-// workgroup per tile stream, as many per CU as fit.
-template <bool EC, int NCH, bool TAIL>
+// Exact-integer FP32 transform (paired_transform_i8); persistent workgroups,
+// as many per CU as fit.
+template <bool EC, int NCH, bool TAIL, bool REM, bool FULL>
 int launch_i8_t(const PairedMatrix* m, int dir, const float* in, long ies, long ivs, long ips,
                 float* out, long oes, long ovs, long ops, int nvec, int total, hipStream_t s) {
-  const int nthr = 64 * std::max(8, m->hb);
-  static int per_cu[2] = {0, 0};
-  int& pc = per_cu[nthr > 512 ? 1 : 0];
-  if (pc == 0) {
+  static int per_cu = 0;
+  if (per_cu == 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, paired_transform_i8<EC, NCH, TAIL>,
-                                                     nthr, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, paired_transform_i8<EC, NCH, TAIL, REM, FULL>, 512, 0) != hipSuccess ||
         nb < 1)
       nb = 1;
-    pc = nb;
+    per_cu = nb;
   }
   const int ntiles = (total + 15) / 16;
-  const int grid = std::min(ntiles, pc * cu_count());
-  hipLaunchKernelGGL((paired_transform_i8<EC, NCH, TAIL>), dim3(grid), dim3(nthr), 0, s, in, ies,
-                     ivs, ips, out, oes, ovs, ops, static_cast<const i32x4*>(m->dig[dir]),
-                     static_cast<const double*>(m->mtail[dir]), m->shift[dir], m->n, m->h, m->hb,
+  const int grid = std::min(ntiles, per_cu * cu_count());
+  hipLaunchKernelGGL((paired_transform_i8<EC, NCH, TAIL, REM, FULL>), dim3(grid), dim3(512), 0, s,
+                     in, ies, ivs, ips, out, oes, ovs, ops,
+                     static_cast<const i32x4*>(m->dig[dir]),
+                     static_cast<const double*>(m->mtail[dir]),
+                     static_cast<const double*>(m->mrem[dir]), m->shift[dir], m->n, m->h, m->hb,
                      nvec, total);
   NFGS_LAUNCH_CHECK();
   return NFGS_OK;
 }
 
+// Vectors of at least 128*nch elements use the unmasked kernels with the
+// exact tail/remainder variant; shorter ones share a masked kernel that
+// always adds the (then zero) tail row.
 template <bool EC>
 int launch_i8(const PairedMatrix* m, int dir, const float* in, long ies, long ivs, long ips,
               float* out, long oes, long ovs, long ops, int nvec, int total, hipStream_t s) {
-#define NFGS_I8(NCH_, TAIL_)                                                                  \
-  return launch_i8_t<EC, NCH_, TAIL_>(m, dir, in, ies, ivs, ips, out, oes, ovs, ops, nvec, total, s)
-  if (m->tail) {
-    if (m->nch == 1) NFGS_I8(1, true);
-    NFGS_I8(2, true);
+#define NFGS_I8(NCH_, TAIL_, REM_, FULL_)                                                      \
+  return launch_i8_t<EC, NCH_, TAIL_, REM_, FULL_>(m, dir, in, ies, ivs, ips, out, oes, ovs, ops, \
+                                                  nvec, total, s)
+  const bool full = m->n >= 128 * m->nch;
+  if (m->nch == 1) {
+    if (!full) {
+      if (m->rem) NFGS_I8(1, true, true, false);
+      NFGS_I8(1, true, false, false);
+    }
+    if (m->tail) {
+      if (m->rem) NFGS_I8(1, true, true, true);
+      NFGS_I8(1, true, false, true);
+    }
+    if (m->rem) NFGS_I8(1, false, true, true);
+    NFGS_I8(1, false, false, true);
   }
-  if (m->nch == 1) NFGS_I8(1, false);
-  NFGS_I8(2, false);
+  if (!full) {
+    if (m->rem) NFGS_I8(2, true, true, false);
+    NFGS_I8(2, true, false, false);
+  }
+  if (m->tail) {
+    if (m->rem) NFGS_I8(2, true, true, true);
+    NFGS_I8(2, true, false, true);
+  }
+  if (m->rem) NFGS_I8(2, false, true, true);
+  NFGS_I8(2, false, false, true);
 #undef NFGS_I8
 }
 
@@ -850,7 +987,11 @@ int launch_t(const PairedMatrix* m, int dir, const void* in, long ies, long ivs,
   const T* et = static_cast<const T*>(m->et[dir]);
   const T* ot = static_cast<const T*>(m->ot[dir]);
   if constexpr (std::is_same<TS, float>::value) {
-    if (m->hb > 0 && total < (1L << 31)) {
+    // the digit kernels address both arrays with 32-bit byte offsets
+    const long ispan = ((long)(nplanes - 1) * ips + (long)(nvec - 1) * ivs + (long)(m->n - 1) * ies + 1) * 4;
+    const long ospan = ((long)(nplanes - 1) * ops + (long)(nvec - 1) * ovs + (long)(m->n - 1) * oes + 1) * 4;
+    const bool pos = ies > 0 && ivs >= 0 && ips >= 0 && oes > 0 && ovs >= 0 && ops >= 0;
+    if (m->hb > 0 && pos && total < (1L << 31) && ispan < (1L << 32) && ospan < (1L << 32)) {
       if (ies == 1 && oes == 1)
         return launch_i8<true>(m, dir, ti, ies, ivs, ips, to, oes, ovs, ops, nvec, (int)total, s);
       if (ivs == 1 && ovs == 1)
