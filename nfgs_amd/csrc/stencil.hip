@@ -12,7 +12,11 @@ namespace {
 
 constexpr int TX = 32, TY = 8, HH = 2;
 constexpr int PW = TX + 2 * HH, PH = TY + 2 * HH, PS = PW * PH;
-constexpr int KCHUNK = 32;
+
+// J.B / |B|^2 of the alpha projection: FP32 takes the hardware reciprocal
+// (within 2 ulp of the quotient), FP64 the IEEE division.
+__device__ __forceinline__ float alpha_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double alpha_div(double a, double b) { return a / b; }
 
 template <typename T>
 struct Grid {
@@ -23,7 +27,45 @@ struct Grid {
   T i2dx, i2dy, i2dz, idz;  // 1/(2 dx), 1/(2 dy), 1/(2 dz), 1/dz
   int tgx, tgy;  // tiles along x and y (the launch is 1-D over tgx*tgy*z-chunks)
   int sx, sy;    // tiling shift: 1 when a last tile would be one node wide
+  int kc;        // planes per z-chunk
 };
+
+int cu_count_stencil() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+  }
+  return cus;
+}
+
+// Planes per z-chunk of the tiled stencils.  A chunk re-stages about four
+// planes (its Phi ring and the face below it), and the blocks of a launch
+// share the CUs evenly, so the launch takes about
+// ceil(blocks / CUs) * (chunk + 4) plane-steps on the busiest CU; the
+// chunk count minimises that.  The last chunk keeps at least two planes:
+// the top node's one-sided dBz/dz takes Bz of planes nz-1, nz-2 and nz-3
+// from the chunk's own z-march.
+int z_chunk(const nfgs_plan* p) {
+  const int nx = p->d.nx, ny = p->d.ny, nz = p->d.nz;
+  const long xy = (long)((nx + (nx % TX == 1) + TX - 1) / TX) * ((ny + (ny % TY == 1) + TY - 1) / TY);
+  const long cus = cu_count_stencil();
+  long best = -1;
+  int kc = nz;
+  for (int nch = 1; nch <= (nz + 3) / 4; ++nch) {
+    const int c = (nz + nch - 1) / nch;
+    if (nch > 1 && nz - (nch - 1) * c < 2) continue;
+    const long cost = ((xy * nch + cus - 1) / cus) * (c + 4);
+    if (best < 0 || cost < best) {
+      best = cost;
+      kc = c;
+    }
+  }
+  return kc;
+}
 
 template <typename T>
 Grid<T> make_grid(const nfgs_plan* p) {
@@ -49,6 +91,7 @@ Grid<T> make_grid(const nfgs_plan* p) {
   g.sy = g.ny % TY == 1 ? 1 : 0;
   g.tgx = (g.nx + g.sx + TX - 1) / TX;
   g.tgy = (g.ny + g.sy + TY - 1) / TY;
+  g.kc = z_chunk(p);
   return g;
 }
 
@@ -75,7 +118,7 @@ __device__ __forceinline__ Tile make_tile(const Grid<T>& g) {
   const int bx = t.id % g.tgx, r = t.id / g.tgx;
   t.x0 = bx * TX - g.sx;
   t.y0 = (r % g.tgy) * TY - g.sy;
-  t.k0 = (r / g.tgy) * KCHUNK;
+  t.k0 = (r / g.tgy) * g.kc;
   t.tx = threadIdx.x % TX;
   t.ty = threadIdx.x / TX;
   t.x = t.x0 + t.tx;
@@ -181,7 +224,7 @@ __global__ __launch_bounds__(256) void reconstruct_b_kernel(
   __shared__ T sown[3][256];  // B_bg at the own points of the current node
   const Tile t = make_tile(g);
   const Halo h = make_halo(t, g);
-  const int k0 = t.k0, k1 = min(k0 + KCHUNK, g.nz);
+  const int k0 = t.k0, k1 = min(k0 + g.kc, g.nz);
   auto pl = [&](auto* base, int k) { return base + (long)k * g.plane; };
   // Phi planes lo..lo+2 are resident while node k is built
   auto lo_of = [&](int k) { return max(0, min(k - 1, g.nz - 3)); };
@@ -285,7 +328,7 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
   __shared__ double red[NFGS_NQ * 16];
   const Tile t = make_tile(g);
   const Halo h = make_halo(t, g);
-  const int k0 = t.k0, k1 = min(k0 + KCHUNK, g.nz);
+  const int k0 = t.k0, k1 = min(k0 + g.kc, g.nz);
   const int ps = max(k0 - 1, 0), pe = min(k1, g.nz - 1);
   auto pl = [&](auto* base, int k) { return base + (long)k * g.plane; };
   const bool edge = t.y == 0 || t.y == g.ny - 1 || t.x == 0 || t.x == g.nx - 1;
@@ -361,7 +404,7 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
         T b2 = F.b[0] * F.b[0] + F.b[1] * F.b[1] + F.b[2] * F.b[2];
         b2 = b2 < floor_b2 ? floor_b2 : b2;
         const T jb = F.j[0] * F.b[0] + F.j[1] * F.b[1] + F.j[2] * F.b[2];
-        T a = jb / b2;
+        T a = alpha_div(jb, b2);
         a = a < -amax ? -amax : (a > amax ? amax : a);
         F.p[0] = a * F.b[0];
         F.p[1] = a * F.b[1];
@@ -540,7 +583,7 @@ __device__ __forceinline__ void state_pass_body(
     T* __restrict__ jp, T* __restrict__ jout, double* partials, long pstride, T floor_b2,
     T amax, const Window& w) {
   const Halo h = make_halo(t, g);
-  const int k0 = t.k0, k1 = min(k0 + KCHUNK, g.nz);
+  const int k0 = t.k0, k1 = min(k0 + g.kc, g.nz);
   const int ps = max(k0 - 1, 0), pe = min(k1, g.nz - 1);
   auto pl = [&](auto* base, int k) { return base + (long)k * g.plane; };
   auto lo_of = [&](int k) { return max(0, min(k - 1, g.nz - 3)); };
@@ -679,7 +722,7 @@ __device__ __forceinline__ void state_pass_body(
         T b2 = F.b[0] * F.b[0] + F.b[1] * F.b[1] + F.b[2] * F.b[2];
         b2 = b2 < floor_b2 ? floor_b2 : b2;
         const T jb = F.j[0] * F.b[0] + F.j[1] * F.b[1] + F.j[2] * F.b[2];
-        T a = jb / b2;
+        T a = alpha_div(jb, b2);
         a = a < -amax ? -amax : (a > amax ? amax : a);
         F.p[0] = a * F.b[0];
         F.p[1] = a * F.b[1];
@@ -831,7 +874,7 @@ __global__ __launch_bounds__(256) void sources_kernel(Grid<T> g, const T* __rest
   __shared__ double red[NFGS_NQ * 16];
   const Tile t = make_tile(g);
   const Halo h = make_halo(t, g);
-  const int k0 = t.k0, k1 = min(k0 + KCHUNK, g.nz);
+  const int k0 = t.k0, k1 = min(k0 + g.kc, g.nz);
   auto pl = [&](auto* base, int k) { return base + (long)k * g.plane; };
   const T* jx = jp;
   const T* jy = jp + g.N;
@@ -933,7 +976,7 @@ __global__ void fill_kernel(T* __restrict__ p, long n, T v) {
 dim3 tiled_grid(const nfgs_plan* p) {
   const int sx = p->d.nx % TX == 1 ? 1 : 0, sy = p->d.ny % TY == 1 ? 1 : 0;
   return dim3((unsigned)(((p->d.nx + sx + TX - 1) / TX) * ((p->d.ny + sy + TY - 1) / TY) *
-                         ((p->d.nz + KCHUNK - 1) / KCHUNK)));
+                         ((p->d.nz + z_chunk(p) - 1) / z_chunk(p))));
 }
 
 template <typename T>

@@ -162,7 +162,10 @@ def test_iterate_matches_reference_trajectory(path):
     if c["dtype"] == "float64":
         assert h["stop_reason"] == meta["stop_reason"]
         if h["best_iter"] != meta["best_iter"]:
-            assert abs(ref_sigma[h["best_iter"] - 1] - ref_sigma[meta["best_iter"] - 1]) < 1e-13
+            # On a sigma plateau (changes of ~1e-15 per step) the iteration at
+            # which the stall rule stops depends on the summation order of the
+            # monitors; the selected state's sigma must not.
+            assert abs(h["best_sigma"] - meta["best_sigma"]) < 1e-13
         fields = s.fields_to_numpy()
         for k, v in fields.items():
             assert v.shape == g["out_" + k].shape
