@@ -191,18 +191,65 @@ __device__ __forceinline__ void commit_diff(T* slot, const Fetch<T>& a, const Fe
   }
 }
 
+// Derivative helpers of the state pass.  INNER tiles (every point the pass
+// touches is at least one node from the grid edge) take the centred forms.
+// Edge tiles evaluate the centred and the one-sided second-order end form
+// for every lane and select, so their lanes do not diverge: the end form at
+// the last node is the negated first-node form with the inward step reversed
+// (the same value, rounding included).  The tiling keeps two inward nodes of
+// every end node in the staged region, so every read is in bounds; reads a
+// lane does not select are discarded by the select.
+template <typename G, typename T>
+__device__ __forceinline__ T fd_sel(G g, int i, int n, T r) {
+  const bool lo = i == 0, hi = i == n - 1;
+  const T fm = g(-1), fp = g(1), f0 = g(0), f2 = g(hi ? -2 : 2);
+  const T one = (T(-3) * f0 + T(4) * (hi ? fm : fp)) - f2;
+  const T d = (lo || hi) ? (hi ? -one : one) : (fp - fm);
+  return d * r;
+}
+template <bool INNER, typename T>
+__device__ __forceinline__ T fdr(const T* f, long s, int i, int n, T r) {
+  if (INNER) return (f[s] - f[-s]) * r;
+  return fd_sel([&](int o) { return f[o * s]; }, i, n, r);
+}
+template <typename T, bool INNER, typename G>
+__device__ __forceinline__ T fdgr(G g, int i, int n, T r) {
+  if (INNER) return (g(1) - g(-1)) * r;
+  return fd_sel(g, i, n, r);
+}
+// Mirror-Neumann 5-point lateral Laplacian; at an end node the outward
+// neighbour's weight moves to the inward one (weights 2, -2, 0).
+template <bool INNER, typename T>
+__device__ __forceinline__ T lapr(const T* p, int y, int x, int ny, int nx, long ld, T idx2,
+                                  T idy2) {
+  const T* c = p + (long)y * ld + x;
+  if (INNER) {
+    T acc = T(0);
+    acc += (c[1] - T(2) * c[0] + c[-1]) * idx2;
+    acc += (c[ld] - T(2) * c[0] + c[-ld]) * idy2;
+    return acc;
+  }
+  const T wxp = x == 0 ? T(2) : (x == nx - 1 ? T(0) : T(1));
+  const T wxm = x == nx - 1 ? T(2) : (x == 0 ? T(0) : T(1));
+  const T wyp = y == 0 ? T(2) : (y == ny - 1 ? T(0) : T(1));
+  const T wym = y == ny - 1 ? T(2) : (y == 0 ? T(0) : T(1));
+  const T ax = (wxp * c[1] - T(2) * c[0] + wxm * c[-1]) * idx2;
+  const T ay = (wyp * c[ld] - T(2) * c[0] + wym * c[-ld]) * idy2;
+  return ax + ay;
+}
+
 template <typename T>
 __device__ __forceinline__ T lds_dx(const T* s, const Tile& t, const Grid<T>& g) {
-  return fd1r(s + t.c, 1, t.x, g.nx, g.i2dx);
+  return fdr<false>(s + t.c, 1, t.x, g.nx, g.i2dx);
 }
 template <typename T>
 __device__ __forceinline__ T lds_dy(const T* s, const Tile& t, const Grid<T>& g) {
-  return fd1r(s + t.c, (long)PW, t.y, g.ny, g.i2dy);
+  return fdr<false>(s + t.c, (long)PW, t.y, g.ny, g.i2dy);
 }
 template <typename T>
 __device__ __forceinline__ T lds_lap(const T* s, const Tile& t, const Grid<T>& g) {
-  return lap_xy_at(s + t.c - ((long)t.y * PW + t.x), t.y, t.x, g.ny, g.nx, (long)PW, g.idx2,
-                   g.idy2);
+  return lapr<false>(s + t.c - ((long)t.y * PW + t.x), t.y, t.x, g.ny, g.nx, (long)PW, g.idx2,
+                     g.idy2);
 }
 
 // Register set of the loads one node of the reconstruction needs.
@@ -395,8 +442,8 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
       const T* s1 = scz[p & 1];
       const T* s0 = scz[f & 1];
       auto cfz = [&](int o) { return T(0.5) * (s1[t.c + o] + s0[t.c + o]); };
-      const T dyc = fd1gr<T>([&](int o) { return cfz(o * PW); }, t.y, g.ny, g.i2dy);
-      const T dxc = fd1gr<T>([&](int o) { return cfz(o); }, t.x, g.nx, g.i2dx);
+      const T dyc = fdgr<T, false>([&](int o) { return cfz(o * PW); }, t.y, g.ny, g.i2dy);
+      const T dxc = fdgr<T, false>([&](int o) { return cfz(o); }, t.x, g.nx, g.i2dx);
       F.j[0] = dyc - dby;
       F.j[1] = dbx - dxc;
       F.j[2] = T(0.5) * (cjz + pjz);
@@ -548,32 +595,6 @@ struct StateLds {
   T sgz[2][256];
   double red[NFGS_NQ * 16];
 };
-
-// Derivative helpers of the state pass: INNER tiles (every point the pass
-// touches is at least one node from the grid edge) take the centred forms
-// without per-lane edge branches; the arithmetic of those forms is the same.
-template <bool INNER, typename T>
-__device__ __forceinline__ T fdr(const T* f, long s, int i, int n, T r) {
-  if (INNER) return (f[s] - f[-s]) * r;
-  return fd1r(f, s, i, n, r);
-}
-template <typename T, bool INNER, typename G>
-__device__ __forceinline__ T fdgr(G g, int i, int n, T r) {
-  if (INNER) return (g(1) - g(-1)) * r;
-  return fd1gr<T>(g, i, n, r);
-}
-template <bool INNER, typename T>
-__device__ __forceinline__ T lapr(const T* p, int y, int x, int ny, int nx, long ld, T idx2,
-                                  T idy2) {
-  if (INNER) {
-    const T* c = p + (long)y * ld + x;
-    T acc = T(0);
-    acc += (c[1] - T(2) * c[0] + c[-1]) * idx2;
-    acc += (c[ld] - T(2) * c[0] + c[-ld]) * idy2;
-    return acc;
-  }
-  return lap_xy_at(p, y, x, ny, nx, ld, idx2, idy2);
-}
 
 template <typename T, bool INNER>
 __device__ __forceinline__ void state_pass_body(
