@@ -203,13 +203,18 @@ __global__ __launch_bounds__(256) void dirichlet_sweep_kernel(
 // last row (c = 0); rows past the last one are interior rows with a zero
 // right-hand side, which the last row (c = 0) does not see.
 // ---------------------------------------------------------------------------
+// Pivot reciprocals: the low modes' columns are ill-conditioned and amplify
+// pivot rounding, so FP32 refines the hardware reciprocal (about 1 ulp) with
+// one Newton step, r (2 - x r), to within rounding of the quotient; FP64
+// divides.
 template <typename T>
 __device__ __forceinline__ T recip(T x) {
   return T(1) / x;
 }
 template <>
 __device__ __forceinline__ float recip<float>(float x) {
-  return __builtin_amdgcn_rcpf(x);
+  const float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(r, fmaf(-x, r, 1.0f), r);
 }
 
 constexpr int kSegMax = 8;  // segments (wavefronts) per block
