@@ -596,7 +596,7 @@ struct StateLds {
   double red[NFGS_NQ * 16];
 };
 
-template <typename T, bool INNER>
+template <typename T, bool INNER, bool ITER>
 __device__ __forceinline__ void state_pass_body(
     StateLds<T>& L, const Tile& t, const Grid<T>& g, const T* __restrict__ phi,
     const T* __restrict__ az, const T* __restrict__ bg, const T* __restrict__ bz_obs,
@@ -606,6 +606,10 @@ __device__ __forceinline__ void state_pass_body(
   const Halo h = make_halo(t, g);
   const int k0 = t.k0, k1 = min(k0 + g.kc, g.nz);
   const int ps = max(k0 - 1, 0), pe = min(k1, g.nz - 1);
+  // ITER: the iteration's outputs (J_par and the monitor partials, no J)
+  if (ITER) jout = nullptr;
+  const bool has_jp = ITER || jp != nullptr;
+  const bool has_pa = ITER || partials != nullptr;
   auto pl = [&](auto* base, int k) { return base + (long)k * g.plane; };
   auto lo_of = [&](int k) { return max(0, min(k - 1, g.nz - 3)); };
   const bool edge = !INNER && (t.y == 0 || t.y == g.ny - 1 || t.x == 0 || t.x == g.nx - 1);
@@ -641,8 +645,11 @@ __device__ __forceinline__ void state_pass_body(
   auto step = [&](StateSet<T>& q, int p) {
     const bool act = p <= pe;
     const int sl = p & 1;
+    // ring slots of planes p, p+1 (= p-2) and p-1
+    const int r0 = p % 3, rp = r0 == 2 ? 0 : r0 + 1, rm = r0 == 0 ? 2 : r0 - 1;
     if (act) {
-      if (p > ps && lo_of(p) > lo_of(p - 1)) commit(L.sphi[(lo_of(p) + 2) % 3], q.phi);
+      // a new top plane (lo_of(p) + 2 = p + 1) enters the ring at interior p
+      if (p > ps && lo_of(p) > lo_of(p - 1)) commit(L.sphi[rp], q.phi);
       commit(L.saz[sl], q.az);
       commit(L.sgx[sl], q.gx);
       commit(L.sgy[sl], q.gy);
@@ -659,9 +666,9 @@ __device__ __forceinline__ void state_pass_body(
         if (p == 0)
           d = (T(-3) * L.sphi[0][e] + T(4) * L.sphi[1][e] - L.sphi[2][e]) * g.i2dz;
         else if (p == g.nz - 1)
-          d = (T(3) * L.sphi[p % 3][e] - T(4) * L.sphi[(p - 1) % 3][e] + L.sphi[(p - 2) % 3][e]) * g.i2dz;
+          d = (T(3) * L.sphi[r0][e] - T(4) * L.sphi[rm][e] + L.sphi[rp][e]) * g.i2dz;
         else
-          d = (L.sphi[(p + 1) % 3][e] - L.sphi[(p - 1) % 3][e]) * g.i2dz;
+          d = (L.sphi[rp][e] - L.sphi[rm][e]) * g.i2dz;
         L.sdz[e] = d;
       }
     }
@@ -690,7 +697,7 @@ __device__ __forceinline__ void state_pass_body(
               const unsigned o = (unsigned)((y * g.nx + x) * (int)sizeof(T));
               vcz = ld(bz_obs, o) - ld(bg + 2 * g.N, o);
             } else {
-              vcz = lapr<INNER>(L.sphi[p % 3] + e - (y * PW + x), y, x, g.ny, g.nx, (long)PW, g.idx2,
+              vcz = lapr<INNER>(L.sphi[r0] + e - (y * PW + x), y, x, g.ny, g.nx, (long)PW, g.idx2,
                               g.idy2);
             }
           }
@@ -751,7 +758,7 @@ __device__ __forceinline__ void state_pass_body(
       }
       const int k = f;  // node finalised now
       if (k >= k0 && k < k1) {
-        if (jp) {
+        if (has_jp) {
           if (k == 0) {
             st(pl(jp, k), h.own, edge ? T(0) : F.j[0]);
             st(pl(jp + g.N, k), h.own, edge ? T(0) : F.j[1]);
@@ -762,7 +769,7 @@ __device__ __forceinline__ void state_pass_body(
             st(pl(jp + 2 * g.N, k), h.own, T(0.5) * (fprev.p[2] + F.p[2]));
           }
         }
-        if (jout) {
+        if (!ITER && jout) {
           T c[3];
           if (k == 0) {
             c[0] = F.j[0]; c[1] = F.j[1]; c[2] = F.j[2];
@@ -777,7 +784,7 @@ __device__ __forceinline__ void state_pass_body(
           st(pl(jout + g.N, k), h.own, edge ? T(0) : c[1]);
           st(pl(jout + 2 * g.N, k), h.own, pjz);
         }
-        if (partials) {
+        if (has_pa) {
           if (k >= w.z0 && k < w.z1 && in_win) {
             T bn = msqrt(F.b[0] * F.b[0] + F.b[1] * F.b[1] + F.b[2] * F.b[2]);
             bn = bn < T(1e-10) ? T(1e-10) : bn;
@@ -821,18 +828,18 @@ __device__ __forceinline__ void state_pass_body(
   // top node: no face above it (planes kt, kt-1, kt-2 of Bz are pb, pz2, pz3)
   const int kt = g.nz - 1;
   if (k1 == g.nz && (INNER || t.valid)) {
-    if (jp) {
+    if (has_jp) {
       st(pl(jp, kt), h.own, fprev.p[0]);
       st(pl(jp + g.N, kt), h.own, fprev.p[1]);
       st(pl(jp + 2 * g.N, kt), h.own, fprev.p[2]);
     }
-    if (jout) {
+    if (!ITER && jout) {
       acc[6] += fprev.j[0] * fprev.j[0] + fprev.j[1] * fprev.j[1] + fprev.j[2] * fprev.j[2];
       st(pl(jout, kt), h.own, T(0));
       st(pl(jout + g.N, kt), h.own, T(0));
       st(pl(jout + 2 * g.N, kt), h.own, T(0));
     }
-    if (partials) {
+    if (has_pa) {
       const T dvx = fdr<INNER>(L.sbx[kt & 1] + t.c, 1, t.x, g.nx, g.i2dx);
       const T dvy = fdr<INNER>(L.sby[kt & 1] + t.c, (long)PW, t.y, g.ny, g.i2dy);
       const T dvz = (T(3) * pb[2] - T(4) * pz2 + pz3) * g.i2dz;
@@ -843,7 +850,7 @@ __device__ __forceinline__ void state_pass_body(
       acc[4] += b2;
     }
   }
-  if (!partials) return;
+  if (!has_pa) return;
   double accd[7];
 #pragma unroll
   for (int q = 0; q < 7; ++q) accd[q] = (double)acc[q];
@@ -860,7 +867,7 @@ __device__ __forceinline__ void state_pass_body(
   }
 }
 
-template <typename T>
+template <typename T, bool ITER>
 __global__ __launch_bounds__(256) void state_pass_kernel(
     Grid<T> g, const T* __restrict__ phi, const T* __restrict__ az, const T* __restrict__ bg,
     const T* __restrict__ bz_obs, const T* __restrict__ hprof, const T* __restrict__ jzt,
@@ -870,11 +877,11 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
   const Tile t = make_tile(g);
   const bool inner = t.x0 >= 2 && t.x0 + TX <= g.nx - 2 && t.y0 >= 2 && t.y0 + TY <= g.ny - 2;
   if (inner)
-    state_pass_body<T, true>(L, t, g, phi, az, bg, bz_obs, hprof, jzt, bout, jp, jout, partials,
-                             pstride, floor_b2, amax, w);
+    state_pass_body<T, true, ITER>(L, t, g, phi, az, bg, bz_obs, hprof, jzt, bout, jp, jout,
+                                   partials, pstride, floor_b2, amax, w);
   else
-    state_pass_body<T, false>(L, t, g, phi, az, bg, bz_obs, hprof, jzt, bout, jp, jout, partials,
-                              pstride, floor_b2, amax, w);
+    state_pass_body<T, false, ITER>(L, t, g, phi, az, bg, bz_obs, hprof, jzt, bout, jp, jout,
+                                    partials, pstride, floor_b2, amax, w);
 }
 
 // Register set of the loads one node of the sources needs.
@@ -925,27 +932,29 @@ __global__ __launch_bounds__(256) void sources_kernel(Grid<T> g, const T* __rest
   };
   auto step = [&](SourceSet<T>& q, int k) {
     const bool act = k < k1;
+    // ring slots of planes k, k+1 and k-1
+    const int r0 = k % 3, rp = r0 == 2 ? 0 : r0 + 1, rm = r0 == 0 ? 2 : r0 - 1;
     if (act) {
       if (k + 1 <= g.nz - 1) {
-        commit(sjx[(k + 1) % 3], q.x);
-        commit(sjy[(k + 1) % 3], q.y);
+        commit(sjx[rp], q.x);
+        commit(sjy[rp], q.y);
       }
       commit(sjz, q.z);
     }
     lds_barrier();
     fetch_set(q, k + 2);
     if (act && t.valid) {
-      const T s = lds_dy(sjx[k % 3], t, g) - lds_dx(sjy[k % 3], t, g);
+      const T s = lds_dy(sjx[r0], t, g) - lds_dx(sjy[r0], t, g);
       st(pl(sb, k), h.own, s);
       acc1 += s * s;
       if (interior) {
         T v = lds_lap(sjz, t, g);
         if (k >= 1 && k <= g.nz - 2) {
-          const T* xu = sjx[(k + 1) % 3] + t.c;
-          const T* xd = sjx[(k - 1) % 3] + t.c;
+          const T* xu = sjx[rp] + t.c;
+          const T* xd = sjx[rm] + t.c;
           const T dxz = (xu[1] - xd[1] - xu[-1] + xd[-1]) * i4dxdz;
-          const T* yu = sjy[(k + 1) % 3] + t.c;
-          const T* yd = sjy[(k - 1) % 3] + t.c;
+          const T* yu = sjy[rp] + t.c;
+          const T* yd = sjy[rm] + t.c;
           const T dyz = (yu[PW] - yd[PW] - yu[-PW] + yd[-PW]) * i4dydz;
           v = v - dxz - dyz;
         }
@@ -1030,7 +1039,8 @@ int state_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi, const
             void* b, void* jp, void* jout, double* partials, hipStream_t s) {
   const Grid<T> g = make_grid<T>(p);
   const Window w{p->d.win_z0, p->d.win_z1, p->d.win_y0, p->d.win_y1, p->d.win_x0, p->d.win_x1};
-  hipLaunchKernelGGL((state_pass_kernel<T>), tiled_grid(p), dim3(TX * TY), 0, s, g,
+  auto kern = (jp && !jout && partials) ? state_pass_kernel<T, true> : state_pass_kernel<T, false>;
+  hipLaunchKernelGGL(kern, tiled_grid(p), dim3(TX * TY), 0, s, g,
                      static_cast<const T*>(phi), static_cast<const T*>(az),
                      static_cast<const T*>(bnd->b_bg), static_cast<const T*>(bnd->bz_obs),
                      static_cast<const T*>(bnd->h_profile), static_cast<const T*>(bnd->jz_t),
