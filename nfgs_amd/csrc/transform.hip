@@ -507,6 +507,46 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef f32x4 f32x4u __attribute__((aligned(4)));
 
+// Cross-lane reads without LDS traffic: DPP inside a row of 16 lanes and
+// ds_swizzle (bit-mask mode) inside a group of 32.  Every lane of the wave
+// must be active where they are used.
+constexpr int kDppXor1 = 0xB1;   // quad_perm [1,0,3,2]: lane ^ 1
+constexpr int kDppXor2 = 0x4E;   // quad_perm [2,3,0,1]: lane ^ 2
+constexpr int kDppRor4 = 0x124;  // row_ror:4
+constexpr int kDppRor8 = 0x128;  // row_ror:8 (= lane ^ 8 inside the row)
+constexpr int kSwzXor16 = 0x1F | (0x10 << 10);  // and 0x1f, or 0, xor 0x10: lane ^ 16
+template <int C>
+__device__ __forceinline__ unsigned dpp_u32(unsigned x) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, C, 0xF, 0xF, false);
+}
+template <int C>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = dpp_u32<C>((unsigned)u), hi = dpp_u32<C>((unsigned)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ unsigned swz16_u32(unsigned x) {
+  return (unsigned)__builtin_amdgcn_ds_swizzle((int)x, kSwzXor16);
+}
+__device__ __forceinline__ double swz16_f64(double x) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = swz16_u32((unsigned)u), hi = swz16_u32((unsigned)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// largest value over the 16 lanes of a row, in every lane of the row
+__device__ __forceinline__ unsigned row_max_u32(unsigned x) {
+  x = max(x, dpp_u32<kDppXor1>(x));
+  x = max(x, dpp_u32<kDppXor2>(x));
+  x = max(x, dpp_u32<kDppRor4>(x));
+  return max(x, dpp_u32<kDppRor8>(x));
+}
+// sum over the 8 lanes of a row whose index has the parity of this lane's
+__device__ __forceinline__ double row_parity_sum_f64(double x) {
+  x += dpp_f64<kDppXor2>(x);
+  x += dpp_f64<kDppRor4>(x);
+  return x + dpp_f64<kDppRor8>(x);
+}
+
 // (plane, vector) of the first vector of a workgroup's successive tiles
 struct TileCursor {
   int p, q;
@@ -529,13 +569,11 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
   constexpr int PLANE = 16 * RSB;     // one (half, digit) plane of a tile
   constexpr int DBUF = 8 * PLANE;     // one tile: 2 halves x 4 digits
   constexpr int NR = 4 * NCH;         // main elements per loader lane
-  constexpr int NJ = 128 * NCH + 2;   // elements a lane set can hold
   __shared__ __align__(16) unsigned char dl[2 * DBUF];
   __shared__ unsigned smax[3][16];
   __shared__ double ssc[2][16];  // 2^-(fixed-point exponent), NaN for non-finite vectors
-  __shared__ float stail[2][2][16];
+  __shared__ double stail[2][2][16];  // tail elements of the E and O halves
   __shared__ double smt[2][16 * 8];
-  __shared__ double smr[REM ? NJ : 1];
   __shared__ double rpart[REM && !EC ? 2 : 1][8][2][16];
 
   const int tid = threadIdx.x, lane = tid & 63;
@@ -559,8 +597,13 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
                           : i32x4{0, 0, 0, 0};
   if (TAIL)
     for (int e = tid; e < 32 * hb; e += blockDim.x) smt[e / (16 * hb)][e % (16 * hb)] = mtail[e];
-  if (REM)
-    for (int j = tid; j < NJ; j += blockDim.x) smr[j] = j < n ? mrem[j] : 0.0;
+  // remainder-row weights of the lane's elements (zero past n), in registers
+  double rw[REM ? NR + 1 : 1];
+  if (REM) {
+#pragma unroll
+    for (int i = 0; i < NR; ++i) rw[i] = lro + 32 * i < n ? mrem[lro + 32 * i] : 0.0;
+    rw[NR] = lro < 2 && 128 * NCH + lro < n ? mrem[128 * NCH + lro] : 0.0;
+  }
   if (tid < 48) smax[tid >> 4][tid & 15] = 0u;
   // The set-up loads are complete before the pipeline starts (vmcnt(0)), so
   // the waits inside it count only tile loads.
@@ -585,6 +628,17 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
     const int p = gv / nvec, q = gv - p * nvec;
     return (long)p * ps + (long)q * vs;
   };
+  // Byte offsets of full tiles in 32-bit arithmetic: the host keeps both
+  // arrays below 4 GiB, so offsets of real elements are exact modulo 2^32.
+  // A tile's offset is a uniform part (its first vector) plus a
+  // lane-constant part plus the plane wrap of lanes past the plane's end.
+  const unsigned ies4 = (unsigned)ies * 4u, ivs4 = (unsigned)ivs * 4u, ips4 = (unsigned)ips * 4u;
+  const unsigned oes4 = (unsigned)oes * 4u, ovs4 = (unsigned)ovs * 4u, ops4 = (unsigned)ops * 4u;
+  const unsigned iwrap4 = (unsigned)iwrap * 4u, owrap4 = (unsigned)owrap * 4u;
+  auto tile_base = [&](const TileCursor& c, unsigned ps4, unsigned vs4) {
+    return (unsigned)c.p * ps4 + (unsigned)c.q * vs4;
+  };
+  const unsigned lin = (unsigned)lv * ivs4 + (unsigned)lro * ies4;  // load lane part
   const int etail = min(128 * NCH + lro, n - 1);  // tail element (clamped)
   // Loads of the j-th tile of this workgroup.  Lanes past the last vector
   // read vector 0 and elements past n re-read element n-1 (masked when
@@ -594,8 +648,7 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
     // byte offset of element ro of the lane's vector (arrays below 4 GiB)
     unsigned o;
     if (FULL && full_tile(j)) {
-      o = (unsigned)(((long)cl.p * ips + (long)(cl.q + lv) * ivs + (cl.q + lv >= nvec ? iwrap : 0L) +
-                      (long)lro * ies) * 4);
+      o = tile_base(cl, ips4, ivs4) + lin + (cl.q + lv >= nvec ? iwrap4 : 0u);
     } else {
       int gv = (tile0 + j * G) * 16 + lv;
       gv = (j < cnt && gv < nvec_total) ? gv : 0;
@@ -620,14 +673,16 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
     for (int i = 0; i < NR; ++i)
       if (valid(i)) mx = max(mx, __float_as_uint(r[i]) & 0x7fffffffu);
     if (has_tail_elem()) mx = max(mx, __float_as_uint(rt) & 0x7fffffffu);
+    // ELEM_CONTIG: a vector spans two rows, each reduced by DPP; otherwise
+    // rows hold offsets g4 = 0..3 of 16 vectors, paired by the swizzle.  One
+    // lane per row (pair) merges into the tile's maxima.
     if (EC) {
-#pragma unroll
-      for (int off = 16; off > 0; off >>= 1) mx = max(mx, (unsigned)__shfl_xor((int)mx, off, 32));
+      mx = row_max_u32(mx);
+      if ((lro & 15) == 0) atomicMax(&smax[j % 3][lv], mx);
     } else {
-      mx = max(mx, (unsigned)__shfl_xor((int)mx, 16));
-      mx = max(mx, (unsigned)__shfl_xor((int)mx, 32));
+      mx = max(mx, swz16_u32(mx));
+      if ((g4 & 1) == 0) atomicMax(&smax[j % 3][lv], mx);
     }
-    if (EC ? lro == 0 : g4 == 0) atomicMax(&smax[j % 3][lv], mx);
   };
 
   // ---- digits of tile j into LDS (and the remainder row) ----
@@ -659,28 +714,29 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
       dst[(2 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23l, p01l, 0x07060302u);
       dst[(3 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23l, p01l, 0x05040100u);
     }
-    if (TAIL && lro < 2) stail[buf][lro][lv] = has_tail_elem() ? rt : 0.0f;
+    if (TAIL && lro < 2) stail[buf][lro][lv] = has_tail_elem() ? (double)rt : 0.0;
     if (REM) {
       // remainder row: FP64 dot product of the raw elements, per parity
+      // (elements past n re-read element n-1 of the vector and weigh zero)
       double acc = 0.0;
 #pragma unroll
-      for (int i = 0; i < NR; ++i)
-        acc = fma(smr[lro + 32 * i], (double)(valid(i) ? r[i] : 0.0f), acc);
-      if (has_tail_elem()) acc = fma(smr[128 * NCH + lro], (double)rt, acc);
+      for (int i = 0; i < NR; ++i) acc = fma(rw[i], (double)r[i], acc);
+      acc = fma(rw[NR], (double)rt, acc);
       if (EC) {
-        // same-parity lanes of the half-wavefront, then E (even) with O (odd)
-#pragma unroll
-        for (int off = 16; off > 1; off >>= 1) acc += __shfl_xor(acc, off, 32);
-        const double other = __shfl_xor(acc, 1, 32);
+        // same-parity lanes of each row, the two rows of the half-wavefront,
+        // then E (even) with O (odd)
+        acc = row_parity_sum_f64(acc);
+        acc += swz16_f64(acc);
+        const double other = dpp_f64<kDppXor1>(acc);
         const int gv = (tile0 + j * G) * 16 + lv;
         if (lro == 0 && gv < nvec_total) {
-          const long vo = full_tile(j) ? (long)cd.p * ops + (long)(cd.q + lv) * ovs +
-                                             (cd.q + lv >= nvec ? owrap : 0L)
-                                       : voff(gv, ops, ovs);
+          const unsigned ob = full_tile(j) ? tile_base(cd, ops4, ovs4) + (unsigned)lv * ovs4 +
+                                                 (cd.q + lv >= nvec ? owrap4 : 0u)
+                                           : (unsigned)(voff(gv, ops, ovs) * 4);
           const float nanf = __int_as_float(0x7fc00000);
-          st(out, (unsigned)((vo + krem) * 4), bad ? nanf : (float)(acc + other));
+          st(out, ob + (unsigned)krem * 4u, bad ? nanf : (float)(acc + other));
           if (n - 1 - krem != krem)
-            st(out, (unsigned)((vo + n - 1 - krem) * 4), bad ? nanf : (float)(acc - other));
+            st(out, ob + (unsigned)(n - 1 - krem) * 4u, bad ? nanf : (float)(acc - other));
         }
       } else {
         acc += __shfl_xor(acc, 32);  // offsets g4 and g4 + 2 share a parity
@@ -702,7 +758,7 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
     const int buf = j & 1;
     const int t = tile0 + j * G;
     const bool ft = full_tile(j);
-    const long obase = (long)cp.p * ops + (long)cp.q * ovs;  // first vector, when ft
+    const unsigned obase = tile_base(cp, ops4, ovs4);  // first vector, when ft
     if (REM && !EC && wave == 0 && lane < 16) {
       const int gv = t * 16 + lane;
       if (gv < nvec_total) {
@@ -712,10 +768,10 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
           od += rpart[buf][w][1][lane];
         }
         if (ssc[buf][lane] != ssc[buf][lane]) e = qnan;  // non-finite vector
-        const long vo = ft ? obase + lane + (cp.q + lane >= nvec ? owrap : 0L) : voff(gv, ops, 1);
-        st(out, (unsigned)((vo + (long)krem * oes) * 4), (float)(e + od));
-        if (n - 1 - krem != krem)
-          st(out, (unsigned)((vo + (long)(n - 1 - krem) * oes) * 4), (float)(e - od));
+        const unsigned ob = ft ? obase + (unsigned)lane * ovs4 + (cp.q + lane >= nvec ? owrap4 : 0u)
+                               : (unsigned)(voff(gv, ops, ovs) * 4);
+        st(out, ob + (unsigned)krem * oes4, (float)(e + od));
+        if (n - 1 - krem != krem) st(out, ob + (unsigned)(n - 1 - krem) * oes4, (float)(e - od));
       }
     }
     if (mw) {
@@ -758,26 +814,26 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
             const int k = k0 + r;
             double e = combine(acc[0], r) * sc, od = combine(acc[1], r) * sc;
             if (TAIL) {
-              e = fma(smt[0][k], (double)stail[buf][0][v], e);
-              od = fma(smt[1][k], (double)stail[buf][1][v], od);
+              e = fma(smt[0][k], stail[buf][0][v], e);
+              od = fma(smt[1][k], stail[buf][1][v], od);
             }
             vp[r] = (float)(e + od);
             vm[r] = (float)(e - od);
           }
-          const long vo = ft ? obase + (long)v * ovs + (cp.q + v >= nvec ? owrap : 0L)
-                             : voff(gv, ops, ovs);
-          float* o = out + vo;
+          // byte offset of the vector's element 0 (elements contiguous)
+          const unsigned ob = ft ? obase + (unsigned)v * ovs4 + (cp.q + v >= nvec ? owrap4 : 0u)
+                                 : (unsigned)(voff(gv, ops, ovs) * 4);
           if (k0 + 3 < h && n - 4 - k0 > k0 + 3) {
-            st(reinterpret_cast<f32x4u*>(out), (unsigned)((vo + k0) * 4), vp);
-            st(reinterpret_cast<f32x4u*>(out), (unsigned)((vo + n - 4 - k0) * 4),
+            st(reinterpret_cast<f32x4u*>(out), ob + (unsigned)k0 * 4u, vp);
+            st(reinterpret_cast<f32x4u*>(out), ob + (unsigned)(n - 4 - k0) * 4u,
                f32x4{vm[3], vm[2], vm[1], vm[0]});
           } else {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int k = k0 + r;
               if (k < h) {
-                o[k] = vp[r];
-                if (n - 1 - k != k) o[n - 1 - k] = vm[r];
+                st(out, ob + (unsigned)k * 4u, vp[r]);
+                if (n - 1 - k != k) st(out, ob + (unsigned)(n - 1 - k) * 4u, vm[r]);
               }
             }
           }
@@ -792,8 +848,8 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
             const double sc = ssc[buf][v0 + r];
             double e = combine(acc[0], r) * sc, od = combine(acc[1], r) * sc;
             if (TAIL) {
-              e = fma(smt[0][k], (double)stail[buf][0][v0 + r], e);
-              od = fma(smt[1][k], (double)stail[buf][1][v0 + r], od);
+              e = fma(smt[0][k], stail[buf][0][v0 + r], e);
+              od = fma(smt[1][k], stail[buf][1][v0 + r], od);
             }
             vp[r] = (float)(e + od);
             vm[r] = (float)(e - od);
@@ -802,9 +858,10 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
           // the four vectors sit in one plane unless a boundary falls inside them
           const bool one = ft && (cp.q + v0 >= nvec || cp.q + v0 + 3 < nvec);
           if (one) {
-            const long vo = obase + v0 + (cp.q + v0 >= nvec ? owrap : 0L);
-            st(reinterpret_cast<f32x4u*>(out), (unsigned)((vo + (long)k * oes) * 4), vp);
-            if (k2 != k) st(reinterpret_cast<f32x4u*>(out), (unsigned)((vo + (long)k2 * oes) * 4), vm);
+            // vectors contiguous: the lane's four vectors are one 16-byte run
+            const unsigned ob = obase + (unsigned)v0 * 4u + (cp.q + v0 >= nvec ? owrap4 : 0u);
+            st(reinterpret_cast<f32x4u*>(out), ob + (unsigned)k * oes4, vp);
+            if (k2 != k) st(reinterpret_cast<f32x4u*>(out), ob + (unsigned)k2 * oes4, vm);
           } else {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -823,14 +880,19 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
   };
 
   // One pipeline step: products of tile i, digits of i+1, maxima of i+2,
-  // loads of i+4.  Tile j's raw elements live in register set j % 4.
+  // loads of i+4.  Tile j's raw elements live in register set j % 4.  The
+  // parts of a step depend on each other only through the barrier, so half of
+  // the waves take the products last: a SIMD's two waves then issue matrix
+  // products and digit work at the same time instead of in turn.
+  const bool late = (wave >> 2) & 1;
   auto step = [&](int i, float (&rd)[NR], float& td, float (&rx)[NR], float& tx,
                   float (&rl)[NR], float& tl) {
-    if (i >= 0 && i < cnt) products(i);
+    if (!late && i >= 0 && i < cnt) products(i);
     if (i + 1 >= 0 && i + 1 < cnt) digits(i + 1, rd, td);
     if (i + 2 < cnt) vmax(i + 2, rx, tx);
     if (i >= 0 && tid < 16) smax[i % 3][tid] = 0u;  // tile i+3 gathers here next step
     load(i + 4, rl, tl);
+    if (late && i >= 0 && i < cnt) products(i);
     lds_barrier();
   };
 
