@@ -16,19 +16,33 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 namespace {
 
 // Exact-argument type-I matrices: the integer phase is reduced before the
-// trigonometric call so every entry is correctly rounded in double.
-double type1_entry(int kind, int n, int k, int j) {
-  const double pi = 3.14159265358979323846;
-  if (kind == NFGS_DCT1) {
-    const long period = 2L * (n - 1);
-    const long ph = ((long)j * k) % period;
-    const double w = (j == 0 || j == n - 1) ? 1.0 : 2.0;
-    return w * std::cos(pi * (double)ph / (double)(n - 1));
+// trigonometric call so every entry is correctly rounded in double.  Entries
+// depend on the phase only, so the trigonometric values of one period are
+// tabulated once per matrix.
+class Type1 {
+ public:
+  Type1(int kind, int n) : dct_(kind == NFGS_DCT1), n_(n) {
+    const double pi = 3.14159265358979323846;
+    period_ = dct_ ? 2L * (n - 1) : 2L * (n + 1);
+    tab_.resize((size_t)period_);
+    for (long ph = 0; ph < period_; ++ph)
+      tab_[(size_t)ph] = dct_ ? std::cos(pi * (double)ph / (double)(n - 1))
+                              : 2.0 * std::sin(pi * (double)ph / (double)(n + 1));
   }
-  const long period = 2L * (n + 1);
-  const long ph = ((long)(j + 1) * (k + 1)) % period;
-  return 2.0 * std::sin(pi * (double)ph / (double)(n + 1));
-}
+  double operator()(int k, int j) const {
+    if (dct_) {
+      const double w = (j == 0 || j == n_ - 1) ? 1.0 : 2.0;
+      return w * tab_[(size_t)(((long)j * k) % period_)];
+    }
+    return tab_[(size_t)(((long)(j + 1) * (k + 1)) % period_)];
+  }
+
+ private:
+  bool dct_;
+  int n_;
+  long period_;
+  std::vector<double> tab_;
+};
 
 template <typename T>
 int upload(const std::vector<double>& host, void** dev) {
@@ -50,7 +64,7 @@ int upload_dtype(int dtype, const std::vector<double>& host, void** dev) {
 // of output row k = 16b + l%16 (the order in which the kernel packs the
 // vector digits).  Paired rows past 64*nch (at most one) form the FP64 tail;
 // when h = 16*hb + 1 the last output row is kept whole (mrem) in FP64.
-int build_digits(int kind, int n, PairedMatrix* m) {
+int build_digits(int kind, int n, const Type1& T1, PairedMatrix* m) {
   const int h = m->h, ne = m->rows;
   const int nch = std::max(1, (ne + 62) / 64);
   const int rem = (h > 16 && h % 16 == 1) ? 1 : 0;
@@ -65,7 +79,7 @@ int build_digits(int kind, int n, PairedMatrix* m) {
     const double sc = dir == NFGS_FORWARD ? 1.0 : inv_scale;
     auto entry = [&](int half, int k, int jp) -> double {
       const int j = 2 * jp + half;
-      return (k < h && j < n) ? sc * type1_entry(kind, n, k, j) : 0.0;
+      return (k < h && j < n) ? sc * T1(k, j) : 0.0;
     };
     double mx = 0.0;
     for (int half = 0; half < 2; ++half)
@@ -76,6 +90,15 @@ int build_digits(int kind, int n, PairedMatrix* m) {
     int sh = 30 - e;
     if (std::ldexp(mx, sh + 1) <= 1073741824.0) ++sh;
     m->shift[dir] = sh;
+    // biased digit words of every (half, k, j') the blocks hold
+    const int kr = 16 * hb, jr = 64 * nch;
+    std::vector<uint32_t> words((size_t)2 * kr * jr);
+    for (int half = 0; half < 2; ++half)
+      for (int k = 0; k < kr; ++k)
+        for (int jp = 0; jp < jr; ++jp) {
+          const long mi = std::lrint(std::ldexp(entry(half, k, jp), sh));
+          words[((size_t)half * kr + k) * jr + jp] = ((uint32_t)(int32_t)mi + 0x808080u) ^ 0x808080u;
+        }
     const size_t nd = (size_t)2 * 4 * nch * hb * 64 * 16;
     std::vector<int8_t> dg(nd);
     size_t idx = 0;
@@ -88,8 +111,7 @@ int build_digits(int kind, int n, PairedMatrix* m) {
                 const int w = byte >> 2, i = byte & 3;
                 const int k = 16 * b + (l & 15);
                 const int jp = 4 * (l >> 4) + w + 16 * (4 * c + i);
-                const long mi = std::lrint(std::ldexp(entry(half, k, jp), sh));
-                const uint32_t d = ((uint32_t)(int32_t)mi + 0x808080u) ^ 0x808080u;
+                const uint32_t d = words[((size_t)half * kr + k) * jr + jp];
                 dg[idx++] = (int8_t)(uint8_t)(d >> (8 * (3 - p)));
               }
     NFGS_HIP_CHECK(hipMalloc(&m->dig[dir], nd));
@@ -102,7 +124,7 @@ int build_digits(int kind, int n, PairedMatrix* m) {
     if (rc) return rc;
     std::vector<double> mr((size_t)n, 0.0);
     if (rem)
-      for (int j = 0; j < n; ++j) mr[j] = sc * type1_entry(kind, n, 16 * hb, j);
+      for (int j = 0; j < n; ++j) mr[j] = sc * T1(16 * hb, j);
     rc = upload<double>(mr, &m->mrem[dir]);
     if (rc) return rc;
   }
@@ -116,6 +138,7 @@ int build_paired(int dtype, int kind, int n, PairedMatrix* m) {
   m->rows_pad = ((m->rows + 15) / 16) * 16;
   m->cols_pad = ((m->h + 15) / 16) * 16;
   const double inv_scale = kind == NFGS_DCT1 ? 1.0 / (2.0 * (n - 1)) : 1.0 / (2.0 * (n + 1));
+  const Type1 T1(kind, n);
   for (int dir = 0; dir < 2; ++dir) {
     const double sc = dir == NFGS_FORWARD ? 1.0 : inv_scale;
     std::vector<double> et((size_t)m->rows_pad * m->cols_pad, 0.0);
@@ -123,8 +146,8 @@ int build_paired(int dtype, int kind, int n, PairedMatrix* m) {
     for (int jp = 0; jp < m->rows; ++jp) {
       for (int k = 0; k < m->h; ++k) {
         const int je = 2 * jp, jo = 2 * jp + 1;
-        if (je < n) et[(size_t)jp * m->cols_pad + k] = sc * type1_entry(kind, n, k, je);
-        if (jo < n) ot[(size_t)jp * m->cols_pad + k] = sc * type1_entry(kind, n, k, jo);
+        if (je < n) et[(size_t)jp * m->cols_pad + k] = sc * T1(k, je);
+        if (jo < n) ot[(size_t)jp * m->cols_pad + k] = sc * T1(k, jo);
       }
     }
     // FP64 for every plan dtype: FP64 plans (and FP32 vectors too long for
@@ -134,7 +157,7 @@ int build_paired(int dtype, int kind, int n, PairedMatrix* m) {
     rc = upload<double>(ot, &m->ot[dir]);
     if (rc) return rc;
   }
-  return dtype == NFGS_F32 ? build_digits(kind, n, m) : NFGS_OK;
+  return dtype == NFGS_F32 ? build_digits(kind, n, T1, m) : NFGS_OK;
 }
 
 void free_paired(PairedMatrix* m) {
