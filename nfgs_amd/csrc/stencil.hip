@@ -979,15 +979,28 @@ __global__ __launch_bounds__(256) void sources_kernel(Grid<T> g, const T* __rest
   }
 }
 
-__global__ void reduce_kernel(const double* __restrict__ partials, long pstride,
-                              double* __restrict__ metrics) {
+// Monitor sums of the per-block partials.  The first kRedBatch partials of a
+// thread per quantity are loaded together (one memory round trip for grids up
+// to kRedBatch * 1024 blocks); the sum order is fixed.
+constexpr int kRedBatch = 4;
+static_assert(kRedBatch == 4, "the batch is summed pairwise below");
+__global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__ partials,
+                                                      long pstride, double* __restrict__ metrics) {
   __shared__ double red[NFGS_NQ * 16];
   double acc[NFGS_NQ];
+  const long nt = blockDim.x;
 #pragma unroll
   for (int q = 0; q < NFGS_NQ; ++q) {
     const long cnt = q == NFGS_Q_ZERO_MODE ? 1 : pstride;
-    double s = 0.0;
-    for (long b = threadIdx.x; b < cnt; b += blockDim.x) s += partials[q * pstride + b];
+    const double* src = partials + q * pstride;
+    double v[kRedBatch];
+#pragma unroll
+    for (int i = 0; i < kRedBatch; ++i) {
+      const long b = threadIdx.x + i * nt;
+      v[i] = b < cnt ? src[b] : 0.0;
+    }
+    double s = (v[0] + v[1]) + (v[2] + v[3]);
+    for (long b = threadIdx.x + kRedBatch * nt; b < cnt; b += nt) s += src[b];
     acc[q] = s;
   }
   block_sum<NFGS_NQ>(acc, red);

@@ -507,46 +507,6 @@ __global__ __launch_bounds__(512) void paired_transform_v2(
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef f32x4 f32x4u __attribute__((aligned(4)));
 
-// Cross-lane reads without LDS traffic: DPP inside a row of 16 lanes and
-// ds_swizzle (bit-mask mode) inside a group of 32.  Every lane of the wave
-// must be active where they are used.
-constexpr int kDppXor1 = 0xB1;   // quad_perm [1,0,3,2]: lane ^ 1
-constexpr int kDppXor2 = 0x4E;   // quad_perm [2,3,0,1]: lane ^ 2
-constexpr int kDppRor4 = 0x124;  // row_ror:4
-constexpr int kDppRor8 = 0x128;  // row_ror:8 (= lane ^ 8 inside the row)
-constexpr int kSwzXor16 = 0x1F | (0x10 << 10);  // and 0x1f, or 0, xor 0x10: lane ^ 16
-template <int C>
-__device__ __forceinline__ unsigned dpp_u32(unsigned x) {
-  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, C, 0xF, 0xF, false);
-}
-template <int C>
-__device__ __forceinline__ double dpp_f64(double x) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
-  const unsigned lo = dpp_u32<C>((unsigned)u), hi = dpp_u32<C>((unsigned)(u >> 32));
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-__device__ __forceinline__ unsigned swz16_u32(unsigned x) {
-  return (unsigned)__builtin_amdgcn_ds_swizzle((int)x, kSwzXor16);
-}
-__device__ __forceinline__ double swz16_f64(double x) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
-  const unsigned lo = swz16_u32((unsigned)u), hi = swz16_u32((unsigned)(u >> 32));
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-// largest value over the 16 lanes of a row, in every lane of the row
-__device__ __forceinline__ unsigned row_max_u32(unsigned x) {
-  x = max(x, dpp_u32<kDppXor1>(x));
-  x = max(x, dpp_u32<kDppXor2>(x));
-  x = max(x, dpp_u32<kDppRor4>(x));
-  return max(x, dpp_u32<kDppRor8>(x));
-}
-// sum over the 8 lanes of a row whose index has the parity of this lane's
-__device__ __forceinline__ double row_parity_sum_f64(double x) {
-  x += dpp_f64<kDppXor2>(x);
-  x += dpp_f64<kDppRor4>(x);
-  return x + dpp_f64<kDppRor8>(x);
-}
-
 // (plane, vector) of the first vector of a workgroup's successive tiles
 struct TileCursor {
   int p, q;
