@@ -518,6 +518,12 @@ struct TileCursor {
       ++p;
     }
   }
+  // next() when go holds, by selects (no branch)
+  __device__ __forceinline__ void next_if(bool go, int dp, int dq, int nvec) {
+    const int nq = q + dq, w = nq >= nvec ? 1 : 0;
+    p = go ? p + dp + w : p;
+    q = go ? nq - w * nvec : q;
+  }
 };
 
 template <bool EC, int NCH, bool TAIL, bool REM, bool FULL>
@@ -535,6 +541,8 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
   __shared__ double stail[2][2][16];  // tail elements of the E and O halves
   __shared__ double smt[2][16 * 8];
   __shared__ double rpart[REM && !EC ? 2 : 1][8][2][16];
+  __shared__ double sjunk[64];  // target of LDS writes by lanes that own no slot
+  __shared__ double smr[REM ? NR + 1 : 1][32];
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -557,13 +565,14 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
                           : i32x4{0, 0, 0, 0};
   if (TAIL)
     for (int e = tid; e < 32 * hb; e += blockDim.x) smt[e / (16 * hb)][e % (16 * hb)] = mtail[e];
-  // remainder-row weights of the lane's elements (zero past n), in registers
-  double rw[REM ? NR + 1 : 1];
-  if (REM) {
-#pragma unroll
-    for (int i = 0; i < NR; ++i) rw[i] = lro + 32 * i < n ? mrem[lro + 32 * i] : 0.0;
-    rw[NR] = lro < 2 && 128 * NCH + lro < n ? mrem[128 * NCH + lro] : 0.0;
-  }
+  // remainder-row weights by element (zero past n): smr[i][lro] weighs element
+  // lro + 32 i, smr[NR][lro] the tail element 128*NCH + lro
+  if (REM)
+    for (int e = tid; e < (NR + 1) * 32; e += blockDim.x) {
+      const int i = e >> 5, ro = e & 31;
+      const int el = i < NR ? ro + 32 * i : 128 * NCH + ro;
+      smr[i][ro] = (i < NR || ro < 2) && el < n ? mrem[el] : 0.0;
+    }
   if (tid < 48) smax[tid >> 4][tid & 15] = 0u;
   // The set-up loads are complete before the pipeline starts (vmcnt(0)), so
   // the waits inside it count only tile loads.
@@ -581,7 +590,7 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
   TileCursor cl = c0, cd = c0, cp = c0;
   // a full tile whose vectors cross at most one plane boundary takes the
   // uniform-base addressing
-  auto full_tile = [&](int j) { return nvec >= 16 && (tile0 + j * G) * 16 + 15 < nvec_total; };
+  auto full_tile = [&](int j) { return (FULL || nvec >= 16) && (tile0 + j * G) * 16 + 15 < nvec_total; };
   const long iwrap = ips - (long)nvec * ivs, owrap = ops - (long)nvec * ovs;
 
   auto voff = [&](int gv, long ps, long vs) {
@@ -599,6 +608,15 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
     return (unsigned)c.p * ps4 + (unsigned)c.q * vs4;
   };
   const unsigned lin = (unsigned)lv * ivs4 + (unsigned)lro * ies4;  // load lane part
+  // Output byte offset of vector v of the tile at cursor c (global tile t).
+  // When a plane holds at least 16 vectors a tile spans at most two planes,
+  // so the cursor form holds for every tile; lanes past the last vector get
+  // an offset they never store to.
+  const bool wide = FULL || nvec >= 16;  // FULL kernels run on planes of >= 16 vectors
+  auto out_off = [&](const TileCursor& c, int t, int v) -> unsigned {
+    if (wide) return tile_base(c, ops4, ovs4) + (unsigned)v * ovs4 + (c.q + v >= nvec ? owrap4 : 0u);
+    return (unsigned)(voff(t * 16 + v, ops, ovs) * 4);
+  };
   const int etail = min(128 * NCH + lro, n - 1);  // tail element (clamped)
   // Loads of the j-th tile of this workgroup.  Lanes past the last vector
   // read vector 0 and elements past n re-read element n-1 (masked when
@@ -607,8 +625,11 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
   auto load = [&](int j, float (&r)[NR], float& rt) {
     // byte offset of element ro of the lane's vector (arrays below 4 GiB)
     unsigned o;
-    if (FULL && full_tile(j)) {
-      o = tile_base(cl, ips4, ivs4) + lin + (cl.q + lv >= nvec ? iwrap4 : 0u);
+    if (FULL) {
+      // planes of >= 16 vectors: the cursor form holds for every tile
+      const bool ok = j < cnt && (tile0 + j * G) * 16 + lv < nvec_total;
+      o = ok ? tile_base(cl, ips4, ivs4) + lin + (cl.q + lv >= nvec ? iwrap4 : 0u)
+             : (unsigned)lro * ies4;
     } else {
       int gv = (tile0 + j * G) * 16 + lv;
       gv = (j < cnt && gv < nvec_total) ? gv : 0;
@@ -645,15 +666,17 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
     }
   };
 
-  // ---- digits of tile j into LDS (and the remainder row) ----
-  auto digits = [&](int j, const float (&r)[NR], float rt) {
-    const int buf = j & 1;
-    const unsigned mb = smax[j % 3][lv];
-    const bool bad = mb >= 0x7f800000u;
-    const int s = (mb == 0u || bad) ? 0 : 156 - (int)(mb >> 23);  // max|x| * 2^s < 2^30
-    if (EC ? lro == 0 : (wave == 0 && g4 == 0))
-      ssc[buf][lv] = bad ? qnan : ldexp(1.0, 24 - shift - s);
-    unsigned* dst = reinterpret_cast<unsigned*>(dl + buf * DBUF + lhalf * 4 * PLANE + lv * RSB) + lm;
+  // ---- digits of tile j, in registers: the words of the four digit planes,
+  // the scale exponent and the remainder-row sums (no memory writes) ----
+  struct Digits {
+    unsigned w[4 * NCH];
+    int s;
+    bool bad;
+  };
+  auto digit_words = [&](int j, const float (&r)[NR], float rt, Digits& D) {
+    const unsigned mb = smax[(j + 3) % 3][lv];
+    D.bad = mb >= 0x7f800000u;
+    D.s = (mb == 0u || D.bad) ? 0 : 156 - (int)(mb >> 23);  // max|x| * 2^s < 2^30
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
       unsigned d[4];
@@ -661,7 +684,7 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
       for (int i = 0; i < 4; ++i) {
         const int ri = 4 * c + i;
         const float x = valid(ri) ? r[ri] : 0.0f;
-        const int X = (int)ldexpf(x, s);
+        const int X = (int)ldexpf(x, D.s);
         d[i] = ((unsigned)X + 0x808080u) ^ 0x808080u;  // bytes = signed digits
       }
       const unsigned p01l = __builtin_amdgcn_perm(d[1], d[0], 0x05010400u);
@@ -669,41 +692,54 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
       const unsigned p23l = __builtin_amdgcn_perm(d[3], d[2], 0x05010400u);
       const unsigned p23h = __builtin_amdgcn_perm(d[3], d[2], 0x07030602u);
       // digit plane p holds byte 3-p of every element
-      dst[(0 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23h, p01h, 0x07060302u);
-      dst[(1 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23h, p01h, 0x05040100u);
-      dst[(2 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23l, p01l, 0x07060302u);
-      dst[(3 * PLANE + c * 64) / 4] = __builtin_amdgcn_perm(p23l, p01l, 0x05040100u);
+      D.w[4 * c + 0] = __builtin_amdgcn_perm(p23h, p01h, 0x07060302u);
+      D.w[4 * c + 1] = __builtin_amdgcn_perm(p23h, p01h, 0x05040100u);
+      D.w[4 * c + 2] = __builtin_amdgcn_perm(p23l, p01l, 0x07060302u);
+      D.w[4 * c + 3] = __builtin_amdgcn_perm(p23l, p01l, 0x05040100u);
     }
-    if (TAIL && lro < 2) stail[buf][lro][lv] = has_tail_elem() ? (double)rt : 0.0;
-    if (REM) {
-      // remainder row: FP64 dot product of the raw elements, per parity
-      // (elements past n re-read element n-1 of the vector and weigh zero)
-      double acc = 0.0;
+  };
+
+  // ---- digits of tile j into LDS with its scale and tail (branch-free:
+  // lanes that own no slot write to sjunk) ----
+  auto digit_store = [&](int j, const Digits& D, float rt) {
+    const int buf = j & 1;
+    const bool sw = EC ? lro == 0 : (wave == 0 && g4 == 0);
+    *(sw ? &ssc[buf][lv] : &sjunk[lane]) = D.bad ? qnan : ldexp(1.0, 24 - shift - D.s);
+    unsigned* dst = reinterpret_cast<unsigned*>(dl + buf * DBUF + lhalf * 4 * PLANE + lv * RSB) + lm;
 #pragma unroll
-      for (int i = 0; i < NR; ++i) acc = fma(rw[i], (double)r[i], acc);
-      acc = fma(rw[NR], (double)rt, acc);
-      if (EC) {
-        // same-parity lanes of each row, the two rows of the half-wavefront,
-        // then E (even) with O (odd)
-        acc = row_parity_sum_f64(acc);
-        acc += swz16_f64(acc);
-        const double other = dpp_f64<kDppXor1>(acc);
-        const int gv = (tile0 + j * G) * 16 + lv;
-        if (lro == 0 && gv < nvec_total) {
-          const unsigned ob = full_tile(j) ? tile_base(cd, ops4, ovs4) + (unsigned)lv * ovs4 +
-                                                 (cd.q + lv >= nvec ? owrap4 : 0u)
-                                           : (unsigned)(voff(gv, ops, ovs) * 4);
-          const float nanf = __int_as_float(0x7fc00000);
-          st(out, ob + (unsigned)krem * 4u, bad ? nanf : (float)(acc + other));
-          if (n - 1 - krem != krem)
-            st(out, ob + (unsigned)(n - 1 - krem) * 4u, bad ? nanf : (float)(acc - other));
-        }
-      } else {
-        acc += __shfl_xor(acc, 32);  // offsets g4 and g4 + 2 share a parity
-        if (g4 < 2) rpart[buf][wave][g4][lv] = acc;
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+      for (int p = 0; p < 4; ++p) dst[(p * PLANE + c * 64) / 4] = D.w[4 * c + p];
+    if (TAIL) *(lro < 2 ? &stail[buf][lro & 1][lv] : &sjunk[lane]) = has_tail_elem() ? (double)rt : 0.0;
+  };
+  // ---- remainder row of tile j: an FP64 dot product of the raw elements
+  // per parity (elements past n re-read element n-1 of the vector and weigh
+  // zero), stored (ELEM_CONTIG) or left as partials ----
+  auto digit_rem = [&](int j, const Digits& D, const float (&r)[NR], float rt, bool real) {
+    if (!REM) return;
+    const int buf = j & 1;
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < NR; ++i) acc = fma(smr[i][lro], (double)r[i], acc);
+    acc = fma(smr[NR][lro], (double)rt, acc);
+    if (EC) {
+      // same-parity lanes of each row, the two rows of the half-wavefront,
+      // then E (even) with O (odd)
+      acc = row_parity_sum_f64(acc);
+      acc += swz16_f64(acc);
+      const double re = acc, ro = dpp_f64<kDppXor1>(acc);
+      const int gv = (tile0 + j * G) * 16 + lv;
+      if (real && lro == 0 && gv < nvec_total) {
+        const unsigned ob = out_off(cd, tile0 + j * G, lv);
+        const float nanf = __int_as_float(0x7fc00000);
+        st(out, ob + (unsigned)krem * 4u, D.bad ? nanf : (float)(re + ro));
+        if (n - 1 - krem != krem)
+          st(out, ob + (unsigned)(n - 1 - krem) * 4u, D.bad ? nanf : (float)(re - ro));
       }
+    } else {
+      acc += __shfl_xor(acc, 32);  // offsets g4 and g4 + 2 share a parity
+      *(g4 < 2 ? &rpart[buf][wave][g4 & 1][lv] : &sjunk[lane]) = acc;
     }
-    cd.next(dp, dq, nvec);
   };
 
   // FP64 value of the four accumulators, times the vector's scale
@@ -713,126 +749,126 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
     return fma((double)t01, 65536.0, (double)t23);
   };
 
-  // ---- products of tile j and out[k] = E + O, out[n-1-k] = E - O ----
-  auto products = [&](int j) {
+  // ---- products of tile j (waves without rows multiply zero digits) ----
+  auto mfma_tile = [&](int j, i32x4 (&acc)[2][4]) {
+    const int buf = j & 1;
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[hf][q] = i32x4{0, 0, 0, 0};
+    const unsigned char* xb = dl + buf * DBUF + l16 * RSB + 16 * g4;
+    constexpr int PI[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};  // matrix digit
+    constexpr int PJ[10] = {3, 2, 1, 0, 2, 1, 0, 1, 0, 0};  // data digit
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf) {
+        i32x4 xd[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          xd[p] = *reinterpret_cast<const i32x4*>(xb + (hf * 4 + p) * PLANE + c * 64);
+#pragma unroll
+        for (int u = 0; u < 10; ++u) {
+          const int L = PI[u] + PJ[u];
+          acc[hf][L] = EC ? __builtin_amdgcn_mfma_i32_16x16x64_i8(md[hf][PI[u]][c], xd[PJ[u]],
+                                                                 acc[hf][L], 0, 0, 0)
+                          : __builtin_amdgcn_mfma_i32_16x16x64_i8(xd[PJ[u]], md[hf][PI[u]][c],
+                                                                 acc[hf][L], 0, 0, 0);
+        }
+      }
+  };
+
+  // ---- out[k] = E + O and out[n-1-k] = E - O of tile j ----
+  // Accumulator register r of lane l holds row 4*(l/16) + r, column l%16:
+  // ELEM_CONTIG rows are output rows (4 adjacent k per lane), otherwise rows
+  // are vectors (4 adjacent vectors per lane).  The values are formed by
+  // every lane; stores are predicated (real: j is one of this block's tiles).
+  auto epilogue = [&](int j, const i32x4 (&acc)[2][4], bool real) {
     const int buf = j & 1;
     const int t = tile0 + j * G;
-    const bool ft = full_tile(j);
-    const unsigned obase = tile_base(cp, ops4, ovs4);  // first vector, when ft
-    if (REM && !EC && wave == 0 && lane < 16) {
-      const int gv = t * 16 + lane;
-      if (gv < nvec_total) {
-        double e = 0.0, od = 0.0;
-        for (int w = 0; w < 8; ++w) {
-          e += rpart[buf][w][0][lane];
-          od += rpart[buf][w][1][lane];
+    f32x4 vp, vm;
+    if (EC) {
+      const int v = l16, gv = t * 16 + v;
+      const double sc = ssc[buf][v];
+      const int k0 = 16 * wave + 4 * g4;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = k0 + r;
+        double e = combine(acc[0], r) * sc, od = combine(acc[1], r) * sc;
+        if (TAIL) {
+          e = fma(smt[0][k], stail[buf][0][v], e);
+          od = fma(smt[1][k], stail[buf][1][v], od);
         }
-        if (ssc[buf][lane] != ssc[buf][lane]) e = qnan;  // non-finite vector
-        const unsigned ob = ft ? obase + (unsigned)lane * ovs4 + (cp.q + lane >= nvec ? owrap4 : 0u)
-                               : (unsigned)(voff(gv, ops, ovs) * 4);
-        st(out, ob + (unsigned)krem * oes4, (float)(e + od));
-        if (n - 1 - krem != krem) st(out, ob + (unsigned)(n - 1 - krem) * oes4, (float)(e - od));
+        vp[r] = (float)(e + od);
+        vm[r] = (float)(e - od);
       }
-    }
-    if (mw) {
-      i32x4 acc[2][4];
-#pragma unroll
-      for (int hf = 0; hf < 2; ++hf)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[hf][q] = i32x4{0, 0, 0, 0};
-      const unsigned char* xb = dl + buf * DBUF + l16 * RSB + 16 * g4;
-      constexpr int PI[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};  // matrix digit
-      constexpr int PJ[10] = {3, 2, 1, 0, 2, 1, 0, 1, 0, 0};  // data digit
-#pragma unroll
-      for (int c = 0; c < NCH; ++c)
-#pragma unroll
-        for (int hf = 0; hf < 2; ++hf) {
-          i32x4 xd[4];
-#pragma unroll
-          for (int p = 0; p < 4; ++p)
-            xd[p] = *reinterpret_cast<const i32x4*>(xb + (hf * 4 + p) * PLANE + c * 64);
-#pragma unroll
-          for (int u = 0; u < 10; ++u) {
-            const int L = PI[u] + PJ[u];
-            acc[hf][L] = EC ? __builtin_amdgcn_mfma_i32_16x16x64_i8(md[hf][PI[u]][c], xd[PJ[u]],
-                                                                   acc[hf][L], 0, 0, 0)
-                            : __builtin_amdgcn_mfma_i32_16x16x64_i8(xd[PJ[u]], md[hf][PI[u]][c],
-                                                                   acc[hf][L], 0, 0, 0);
-          }
-        }
-      // Accumulator register r of lane l holds row 4*(l/16) + r, column l%16:
-      // ELEM_CONTIG rows are output rows (4 adjacent k per lane), otherwise
-      // rows are vectors (4 adjacent vectors per lane).
-      if (EC) {
-        const int v = l16, gv = t * 16 + v;
-        if (gv < nvec_total) {
-          const double sc = ssc[buf][v];
-          const int k0 = 16 * wave + 4 * g4;
-          f32x4 vp, vm;
+      if (real && mw && gv < nvec_total) {
+        // byte offset of the vector's element 0 (elements contiguous)
+        const unsigned ob = out_off(cp, t, v);
+        if (k0 + 3 < h && n - 4 - k0 > k0 + 3) {
+          st(reinterpret_cast<f32x4u*>(out), ob + (unsigned)k0 * 4u, vp);
+          st(reinterpret_cast<f32x4u*>(out), ob + (unsigned)(n - 4 - k0) * 4u,
+             f32x4{vm[3], vm[2], vm[1], vm[0]});
+        } else {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int k = k0 + r;
-            double e = combine(acc[0], r) * sc, od = combine(acc[1], r) * sc;
-            if (TAIL) {
-              e = fma(smt[0][k], stail[buf][0][v], e);
-              od = fma(smt[1][k], stail[buf][1][v], od);
-            }
-            vp[r] = (float)(e + od);
-            vm[r] = (float)(e - od);
-          }
-          // byte offset of the vector's element 0 (elements contiguous)
-          const unsigned ob = ft ? obase + (unsigned)v * ovs4 + (cp.q + v >= nvec ? owrap4 : 0u)
-                                 : (unsigned)(voff(gv, ops, ovs) * 4);
-          if (k0 + 3 < h && n - 4 - k0 > k0 + 3) {
-            st(reinterpret_cast<f32x4u*>(out), ob + (unsigned)k0 * 4u, vp);
-            st(reinterpret_cast<f32x4u*>(out), ob + (unsigned)(n - 4 - k0) * 4u,
-               f32x4{vm[3], vm[2], vm[1], vm[0]});
-          } else {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int k = k0 + r;
-              if (k < h) {
-                st(out, ob + (unsigned)k * 4u, vp[r]);
-                if (n - 1 - k != k) st(out, ob + (unsigned)(n - 1 - k) * 4u, vm[r]);
-              }
+            if (k < h) {
+              st(out, ob + (unsigned)k * 4u, vp[r]);
+              if (n - 1 - k != k) st(out, ob + (unsigned)(n - 1 - k) * 4u, vm[r]);
             }
           }
         }
-      } else {
-        const int k = 16 * wave + l16;
-        if (k < h) {
-          const int v0 = 4 * g4, gv0 = t * 16 + v0;
-          f32x4 vp, vm;
+      }
+    } else {
+      const int k = 16 * wave + l16;
+      const int v0 = 4 * g4, gv0 = t * 16 + v0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double sc = ssc[buf][v0 + r];
+        double e = combine(acc[0], r) * sc, od = combine(acc[1], r) * sc;
+        if (TAIL) {
+          e = fma(smt[0][k], stail[buf][0][v0 + r], e);
+          od = fma(smt[1][k], stail[buf][1][v0 + r], od);
+        }
+        vp[r] = (float)(e + od);
+        vm[r] = (float)(e - od);
+      }
+      const int k2 = n - 1 - k;
+      if (real && mw && k < h) {
+        // the four vectors sit in one plane unless a boundary falls inside them
+        const bool one = full_tile(j) && (cp.q + v0 >= nvec || cp.q + v0 + 3 < nvec);
+        if (one) {
+          // vectors contiguous: the lane's four vectors are one 16-byte run
+          const unsigned ob = tile_base(cp, ops4, ovs4) + (unsigned)v0 * 4u +
+                              (cp.q + v0 >= nvec ? owrap4 : 0u);
+          st(reinterpret_cast<f32x4u*>(out), ob + (unsigned)k * oes4, vp);
+          if (k2 != k) st(reinterpret_cast<f32x4u*>(out), ob + (unsigned)k2 * oes4, vm);
+        } else {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const double sc = ssc[buf][v0 + r];
-            double e = combine(acc[0], r) * sc, od = combine(acc[1], r) * sc;
-            if (TAIL) {
-              e = fma(smt[0][k], stail[buf][0][v0 + r], e);
-              od = fma(smt[1][k], stail[buf][1][v0 + r], od);
-            }
-            vp[r] = (float)(e + od);
-            vm[r] = (float)(e - od);
-          }
-          const int k2 = n - 1 - k;
-          // the four vectors sit in one plane unless a boundary falls inside them
-          const bool one = ft && (cp.q + v0 >= nvec || cp.q + v0 + 3 < nvec);
-          if (one) {
-            // vectors contiguous: the lane's four vectors are one 16-byte run
-            const unsigned ob = obase + (unsigned)v0 * 4u + (cp.q + v0 >= nvec ? owrap4 : 0u);
-            st(reinterpret_cast<f32x4u*>(out), ob + (unsigned)k * oes4, vp);
-            if (k2 != k) st(reinterpret_cast<f32x4u*>(out), ob + (unsigned)k2 * oes4, vm);
-          } else {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int gv = gv0 + r;
-              if (gv < nvec_total) {
-                float* o = out + voff(gv, ops, 1);
-                o[(long)k * oes] = vp[r];
-                if (k2 != k) o[(long)k2 * oes] = vm[r];
-              }
+            const int gv = gv0 + r;
+            if (gv < nvec_total) {
+              const unsigned ob = out_off(cp, t, v0 + r);
+              st(out, ob + (unsigned)k * oes4, vp[r]);
+              if (k2 != k) st(out, ob + (unsigned)k2 * oes4, vm[r]);
             }
           }
+        }
+      }
+      if (REM && wave == 0 && lane < 16) {
+        // remainder row of the tile from the eight waves' partials
+        const int gv = t * 16 + lane;
+        if (real && gv < nvec_total) {
+          double e = 0.0, od = 0.0;
+          for (int w = 0; w < 8; ++w) {
+            e += rpart[buf][w][0][lane];
+            od += rpart[buf][w][1][lane];
+          }
+          if (ssc[buf][lane] != ssc[buf][lane]) e = qnan;  // non-finite vector
+          const unsigned ob = out_off(cp, t, lane);
+          st(out, ob + (unsigned)krem * oes4, (float)(e + od));
+          if (n - 1 - krem != krem) st(out, ob + (unsigned)(n - 1 - krem) * oes4, (float)(e - od));
         }
       }
     }
@@ -840,32 +876,73 @@ __global__ __launch_bounds__(512) void paired_transform_i8(
   };
 
   // One pipeline step: products of tile i, digits of i+1, maxima of i+2,
-  // loads of i+4.  Tile j's raw elements live in register set j % 4.  The
-  // parts of a step depend on each other only through the barrier, so half of
-  // the waves take the products last: a SIMD's two waves then issue matrix
-  // products and digit work at the same time instead of in turn.
+  // loads of i+4.  Tile j's raw elements live in register set j % 4.  In an
+  // ELEM_CONTIG steady step (every part on a real tile) the digit arithmetic
+  // of tile i+1,
+  // the matrix products of tile i, the digit writes and the epilogue
+  // arithmetic form one branch-free region, so the digit work issues between
+  // the products.  The parts of a step depend on each other only through the
+  // barrier, so half of the waves take the products last: a SIMD's two
+  // waves then issue matrix products and digit work at the same time.
   const bool late = (wave >> 2) & 1;
-  auto step = [&](int i, float (&rd)[NR], float& td, float (&rx)[NR], float& tx,
+  auto work_steady = [&](int i, float (&rd)[NR], float td) {
+    Digits D;
+    digit_words(i + 1, rd, td, D);
+    i32x4 acc[2][4];
+    mfma_tile(i, acc);
+    digit_store(i + 1, D, td);
+    epilogue(i, acc, true);
+    digit_rem(i + 1, D, rd, td, true);
+    cd.next(dp, dq, nvec);
+  };
+  auto work_guarded = [&](int i, float (&rd)[NR], float td) {
+    if (i >= 0 && i < cnt) {
+      i32x4 acc[2][4];
+      mfma_tile(i, acc);
+      epilogue(i, acc, true);
+    }
+    Digits D;
+    digit_words(i + 1, rd, td, D);
+    digit_store(i + 1, D, td);
+    digit_rem(i + 1, D, rd, td, i + 1 >= 0 && i + 1 < cnt);
+    cd.next_if(i + 1 >= 0, dp, dq, nvec);
+  };
+  auto step = [&](bool steady, int i, float (&rd)[NR], float& td, float (&rx)[NR], float& tx,
                   float (&rl)[NR], float& tl) {
-    if (!late && i >= 0 && i < cnt) products(i);
-    if (i + 1 >= 0 && i + 1 < cnt) digits(i + 1, rd, td);
-    if (i + 2 < cnt) vmax(i + 2, rx, tx);
-    if (i >= 0 && tid < 16) smax[i % 3][tid] = 0u;  // tile i+3 gathers here next step
+    if (!late) {
+      if (steady) work_steady(i, rd, td);
+      else work_guarded(i, rd, td);
+    }
+    vmax(i + 2, rx, tx);
+    if (tid < 16) smax[(i + 3) % 3][tid] = 0u;  // tile i+3 gathers here next step
     load(i + 4, rl, tl);
-    if (late && i >= 0 && i < cnt) products(i);
+    if (late) {
+      if (steady) work_steady(i, rd, td);
+      else work_guarded(i, rd, td);
+    }
     lds_barrier();
   };
 
-  float r0[NR], r1[NR], r2[NR], r3[NR];
+  // set 3 is read (as stale data) by the first step before its first load
+  float r0[NR] = {}, r1[NR] = {}, r2[NR] = {}, r3[NR] = {};
   float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
   load(0, r0, t0);
   load(1, r1, t1);
   lds_barrier();
   for (int i = -2; i < cnt; i += 4) {
-    step(i, r3, t3, r0, t0, r2, t2);
-    step(i + 1, r0, t0, r1, t1, r3, t3);
-    step(i + 2, r1, t1, r2, t2, r0, t0);
-    step(i + 3, r2, t2, r3, t3, r1, t1);
+    // (vectors across lanes hold more registers per lane in the epilogue:
+    // there the digit work follows the products)
+    if (EC && i >= 0 && i + 4 < cnt) {
+      step(true, i, r3, t3, r0, t0, r2, t2);
+      step(true, i + 1, r0, t0, r1, t1, r3, t3);
+      step(true, i + 2, r1, t1, r2, t2, r0, t0);
+      step(true, i + 3, r2, t2, r3, t3, r1, t1);
+    } else {
+      step(false, i, r3, t3, r0, t0, r2, t2);
+      step(false, i + 1, r0, t0, r1, t1, r3, t3);
+      step(false, i + 2, r1, t1, r2, t2, r0, t0);
+      step(false, i + 3, r2, t2, r3, t3, r1, t1);
+    }
   }
 }
 
@@ -956,16 +1033,16 @@ int launch_i8_t(const PairedMatrix* m, int dir, const float* in, long ies, long 
   return NFGS_OK;
 }
 
-// Vectors of at least 128*nch elements use the unmasked kernels with the
-// exact tail/remainder variant; shorter ones share a masked kernel that
-// always adds the (then zero) tail row.
+// Vectors of at least 128*nch elements, on planes of at least 16 vectors,
+// use the unmasked kernels with the exact tail/remainder variant; others
+// share a masked kernel that always adds the (then zero) tail row.
 template <bool EC>
 int launch_i8(const PairedMatrix* m, int dir, const float* in, long ies, long ivs, long ips,
               float* out, long oes, long ovs, long ops, int nvec, int total, hipStream_t s) {
 #define NFGS_I8(NCH_, TAIL_, REM_, FULL_)                                                      \
   return launch_i8_t<EC, NCH_, TAIL_, REM_, FULL_>(m, dir, in, ies, ivs, ips, out, oes, ovs, ops, \
                                                   nvec, total, s)
-  const bool full = m->n >= 128 * m->nch;
+  const bool full = m->n >= 128 * m->nch && nvec >= 16;
   if (m->nch == 1) {
     if (!full) {
       if (m->rem) NFGS_I8(1, true, true, false);
