@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void robin_sweep_kernel(
     const T* __restrict__ kk, const T* __restrict__ bh, const T* __restrict__ qh, int nz,
     long plane, T inv_dz2, T two_inv_dz2, T dz, T dz2, T two_dz, T rho, T one_m_rho, T kb,
     T fu_rho, T rho_over_dz, double* zm_partial, double norm) {
-  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long m = (long)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (m >= plane) return;
   if (m == 0) {
     // The (0,0) mode carries no correction; it is the zero-mode diagnostic.
@@ -154,7 +154,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void dirichlet_sweep_kernel(
     const T* __restrict__ src, T* __restrict__ out, T* __restrict__ cs, const T* __restrict__ lamd,
     const T* __restrict__ bh, int nz, long plane, T inv_dz2, T two_inv_dz2) {
-  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long m = (long)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (m >= plane) return;
   const T l = lamd[m];
   const T a = inv_dz2, c = inv_dz2;
@@ -377,7 +377,9 @@ __global__ __launch_bounds__(64 * kSegMax) void robin_seg_kernel(
   // addresses) scalar
   const int lane = threadIdx.x & 63, ns = blockDim.x >> 6;
   const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const long m = (long)blockIdx.x * 64 + lane;
+  // neighbouring mode blocks share the cache lines at their ends (a plane's
+  // rows are not line aligned): keep them in one XCD's L2
+  const long m = (long)xcd_tile(blockIdx.x, gridDim.x) * 64 + lane;
   const bool live = m < plane;
   const unsigned mo = (unsigned)((live ? m : 0) * (long)sizeof(T));  // byte offset in a plane
   const long pb = plane * (long)sizeof(T);                           // plane bytes
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(64 * kSegMax) void dirichlet_seg_kernel(
   __shared__ SegLds<T> L;
   const int lane = threadIdx.x & 63, ns = blockDim.x >> 6;
   const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const long m = (long)blockIdx.x * 64 + lane;
+  const long m = (long)xcd_tile(blockIdx.x, gridDim.x) * 64 + lane;
   const bool live = m < plane;
   const unsigned mo = (unsigned)((live ? m : 0) * (long)sizeof(T));
   const long pb = plane * (long)sizeof(T);
