@@ -49,6 +49,7 @@ def parse():
     p.add_argument("--cpu-iters", type=int, default=2)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 error reference")
+    p.add_argument("--no-warm", action="store_true", help="skip the warm-start solves")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run a 256 MiB device copy for counter calibration")
     return p.parse_args()
@@ -68,8 +69,9 @@ ITER_KW = dict(init_mode="gs_cfcf", tol=1e-5, mu=1, patience=40, sigma_improveme
                stop_mode="sigma_best", verbose=False)
 
 
-def solve(torch, NFGSSolverGPU, planes, n, rho, dtype, max_iter):
-    """One timed solve with the boundary already resident on the device."""
+def solve(torch, NFGSSolverGPU, planes, n, rho, dtype, max_iter, start=None):
+    """One timed solve with the boundary already resident on the device.
+    start=(B, Az): init_mode="field" from that state instead of gs_cfcf."""
     bx, by, bz = (p.to(dtype) for p in planes)
     s = NFGSSolverGPU(n, n, n, 1.0, 1.0, 1.0, device="cuda", dtype=dtype, **solver_kwargs(rho))
     torch.cuda.synchronize()
@@ -77,7 +79,10 @@ def solve(torch, NFGSSolverGPU, planes, n, rho, dtype, max_iter):
     s.set_bottom_vector_boundary(bx, by, bz, zero_net_current=False, smooth_passes=0, ijzbcor=3)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    h = s.iterate(max_iter=max_iter, **ITER_KW)
+    kw = dict(ITER_KW)
+    if start is not None:
+        kw.update(init_mode="field", init_field=start[0], init_az=start[1])
+    h = s.iterate(max_iter=max_iter, **kw)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     return s, h, t1 - t0, t2 - t0
@@ -312,6 +317,35 @@ def main():
         r["traffic"] = measured_traffic(r["kernel"], [s.Nz, s.Ny, s.Nx])
     log(f"dominant kernel {dom}: {roofline}")
 
+    # warm start (init_mode="field"): the next frame of the same region (poles
+    # drifted by one cell, alpha x1.05) started from this frame's selected
+    # state, against its own plain gs_cfcf solve
+    warm = None
+    if not a.no_warm:
+        log("next frame: plain and warm-started solves, fp32 rho=0.5")
+        nb = multipole_boundary(n, seed=a.seed, drift=1.0, alpha_scale=1.05)
+        planes2 = tuple(torch.tensor(p, dtype=torch.float64, device=dev) for p in nb[:3])
+        B0, Az0, _ = s.warm_start_state()
+        sp, hp, _, tp = solve(torch, NFGSSolverGPU, planes2, n, 0.5, torch.float32, a.max_iter)
+        sw, hw, _, tw = solve(torch, NFGSSolverGPU, planes2, n, 0.5, torch.float32, a.max_iter,
+                              start=(B0, Az0))
+        fp, fw = sp.fields_to_numpy(), sw.fields_to_numpy()
+        bdiff = float(np.sqrt(sum(np.sum((fw[k].astype(np.float64) - fp[k]) ** 2)
+                                  for k in ("Bx", "By", "Bz")) /
+                              sum(np.sum(fp[k].astype(np.float64) ** 2) for k in ("Bx", "By", "Bz"))))
+        warm = dict(frame="seed %d, poles drifted 1 cell, alpha x1.05" % a.seed,
+                    plain_sec_to_converge=round(tp, 4), plain_iterations=hp["final_iter"],
+                    plain_best_iter=hp["best_iter"], plain_best_sigma=hp["best_sigma"],
+                    warm_sec_to_converge=round(tw, 4), warm_iterations=hw["final_iter"],
+                    warm_best_iter=hw["best_iter"], warm_best_sigma=hw["best_sigma"],
+                    warm_start_projection_error=hw["init_diagnostics"].get(
+                        "field_relative_difference"),
+                    speedup=round(tp / tw, 3), B_rel_diff_warm_vs_plain=bdiff)
+        log(f"  plain {tp:.3f} s ({hp['final_iter']} it, best {hp['best_sigma']:.5e}); "
+            f"warm {tw:.3f} s ({hw['final_iter']} it, best {hw['best_sigma']:.5e}); "
+            f"B diff {bdiff:.3e}")
+        del sp, sw, B0, Az0
+
     # error of the FP32 solve against an FP64 solve of the same boundary
     err = None
     if not a.no_fp64:
@@ -360,6 +394,7 @@ def main():
         "roofline": roofline,
         "stencil_roofline": stencil_roof,
         "kernels": kernels,
+        "warm_start": warm,
         "err_vs_fp64": err,
         "cpu_baseline": cpu,
     }

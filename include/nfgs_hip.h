@@ -235,6 +235,25 @@ int nfgs_setup_gaussian(const nfgs_plan* plan, const void* bx, const void* by, c
                         void* dphi_target, void* d0, void* phi_above_hat, void* h_target,
                         double* diag, void* workspace, void* stream);
 
+/* ---- initial state from a caller's field (init_mode="field") ----------- */
+
+/* Workspace bytes needed by nfgs_project_field (three volumes). */
+long nfgs_project_workspace_bytes(const nfgs_plan* plan);
+
+/* Projects a field into the NFGS scalars, as the cfcf handoff does with the
+ * Stable solver's B and Az (nfgs_solver.py:6080-6147; the Neumann solve
+ * :1120-1130 and :1146-1180): phi[0] = Phi_above_bottom, phi[1:] solves
+ * lap_xy phi = B_z - B_bg,z per plane with every plane's constant mode set to
+ * zero (its value per height, :4466-4468, is written to mean_bcz: nz device
+ * doubles); az = az_in with Az_bottom on the bottom plane and zero on the top
+ * plane and the lateral walls.  b is 3*N, az_in, phi and az are N.  The
+ * projected state's B and J come from nfgs_state_pass.  Not in the
+ * reference as an entry point: the reference builds its handoff state inside
+ * _build_stable_cfcf_initial_state. */
+int nfgs_project_field(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* b,
+                       const void* az_in, void* phi, void* az, double* mean_bcz, void* workspace,
+                       void* stream);
+
 /* ---- post-selection diagnostics ---------------------------------------- */
 
 /* out[0] = sum (a-b)^2, out[1] = sum b^2 over n elements (FP64 device

@@ -76,6 +76,8 @@ def _declare(lib):
         "nfgs_gaussian_workspace_bytes": (C.c_long, [vp]),
         "nfgs_setup_gaussian": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                           vp]),
+        "nfgs_project_workspace_bytes": (C.c_long, [vp]),
+        "nfgs_project_field": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp]),
         "nfgs_sq_diff": (C.c_int, [vp, vp, vp, C.c_long, vp, vp]),
         "nfgs_source_diagnostics_workspace_bytes": (C.c_long, [vp]),
         "nfgs_source_diagnostics": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp]),

@@ -54,11 +54,13 @@ def dipole_boundary(n: int = 64, seed: int = 0, alpha: float = 0.02, sigma: floa
 
 def multipole_boundary(nx: int = 256, ny: int | None = None, seed: int = 0,
                        npoles: tuple = (4, 8), amp: tuple = (0.5, 2.0), sigma: tuple = (8.0, 20.0),
-                       alpha_max: float = 0.03):
+                       alpha_max: float = 0.03, drift: float = 0.0, alpha_scale: float = 1.0):
     """4-8 flux-balanced Gaussian poles and a random-alpha LFFF horizontal field.
 
     Returns (Bx, By, Bz, alpha).  Negative poles are rescaled so that the net
-    flux sum(a_i sigma_i^2) vanishes.
+    flux sum(a_i sigma_i^2) vanishes.  drift > 0 moves every pole by that many
+    cells in a seeded direction and alpha_scale scales alpha: a later frame of
+    the same region (drift=0, alpha_scale=1 give the base frame).
     """
     ny = nx if ny is None else ny
     rng = np.random.default_rng(seed)
@@ -74,7 +76,11 @@ def multipole_boundary(nx: int = 256, ny: int | None = None, seed: int = 0,
     flux = a * s * s
     pos, neg = flux[flux > 0].sum(), -flux[flux < 0].sum()
     a = np.where(a < 0, a * pos / neg, a)
+    alpha = float(rng.uniform(-alpha_max, alpha_max)) * alpha_scale
+    if drift:
+        th = np.random.default_rng(seed + 7919).uniform(0.0, 2.0 * np.pi, n)
+        cx = cx + drift * np.cos(th)
+        cy = cy + drift * np.sin(th)
     bz = sum(ai * _gauss(ny, nx, xi, yi, si) for ai, xi, yi, si in zip(a, cx, cy, s))
-    alpha = float(rng.uniform(-alpha_max, alpha_max))
     bx, by = lfff_horizontal(bz, alpha)
     return bx, by, bz, alpha

@@ -306,10 +306,77 @@ int plane_lap_t(const nfgs_plan* p, const T* in, double scale, int zero_edges, T
   }, s);
 }
 
+// Projection of a caller's field into the NFGS scalars: the handoff of
+// nfgs_solver.py:6080-6147 with (B, Az) supplied by the caller.  Phi_c[0] is
+// Phi_above_bottom and Phi_c[1:] solves the endpoint-Neumann lap_xy Phi_c =
+// B_z - B_bg,z plane by plane (:1120-1130, :1146-1180: the constant mode of
+// every plane is set to zero; its value per height, :4466-4468, goes to
+// mean_bcz).  Az keeps the solver's fixed planes: Az_bottom at the bottom,
+// zero on the top plane and on the lateral walls (the planes the iteration
+// never rewrites).  Workspace: 3 volumes.
+template <typename T>
+int project_field_t(const nfgs_plan* p, const nfgs_boundary* bnd, const T* b, const T* az_in,
+                    T* phi, T* az, double* mean_bcz, void* workspace, hipStream_t s) {
+  Ctx<T> c{p, s, p->d.nx, p->d.ny, p->d.nz, p->plane};
+  const int nx = c.nx, ny = c.ny, nz = c.nz;
+  const long pl = c.pl, N = p->N;
+  const double norm = 4.0 * (nx - 1) * (ny - 1);
+  const T* lam = static_cast<const T*>(p->lam);
+  const T* bgz = static_cast<const T*>(bnd->b_bg) + 2 * N;
+  const T* bz = b + 2 * N;
+  const T* pb = static_cast<const T*>(bnd->phi_bottom);
+  const T* ab = static_cast<const T*>(bnd->az_bottom);
+  T* v0 = static_cast<T*>(workspace);
+  T* v1 = v0 + N;
+  T* v2 = v1 + N;
+  NFGS_TRY(launch_map(N, [=] __device__(long i) { v0[i] = bz[i] - bgz[i]; }, s));
+  NFGS_TRY(xy_pair(c, NFGS_FORWARD, v0, v1, v2, nz));
+  NFGS_TRY(launch_map(nz, [=] __device__(long q) { mean_bcz[q] = (double)v1[q * pl] / norm; }, s));
+  NFGS_TRY(launch_map(N - pl, [=] __device__(long i) {
+    const long j = i % pl;
+    v1[pl + i] = j == 0 ? T(0) : v1[pl + i] / lam[j];
+  }, s));
+  NFGS_TRY(xy_pair(c, NFGS_INVERSE, v1 + pl, phi + pl, v2, nz - 1));
+  return launch_map(N, [=] __device__(long i) {
+    const long q = i / pl, j = i - q * pl;
+    if (q == 0) {
+      phi[i] = pb[j];
+      az[i] = ab[j];
+      return;
+    }
+    const int y = (int)(j / nx), x = (int)(j - (long)y * nx);
+    const bool wall = q == nz - 1 || y == 0 || y == ny - 1 || x == 0 || x == nx - 1;
+    az[i] = wall ? T(0) : az_in[i];
+  }, s);
+}
+
 }  // namespace
 }  // namespace nfgs
 
 using namespace nfgs;
+
+extern "C" long nfgs_project_workspace_bytes(const nfgs_plan* p) {
+  return p ? 3 * p->N * (long)p->esize : 0;
+}
+
+extern "C" int nfgs_project_field(const nfgs_plan* p, const nfgs_boundary* bnd, const void* b,
+                                  const void* az_in, void* phi, void* az, double* mean_bcz,
+                                  void* workspace, void* stream) {
+  NFGS_REQUIRE(p && bnd && b && az_in && phi && az && mean_bcz && workspace, NFGS_ERR_ARG,
+               "project_field: null argument");
+  NFGS_REQUIRE(bnd->b_bg && bnd->phi_bottom && bnd->az_bottom, NFGS_ERR_ARG,
+               "project_field: the boundary has no B_background / bottom planes");
+  NFGS_REQUIRE(phi != b && az != b && phi != az, NFGS_ERR_ARG,
+               "project_field: outputs must not alias the field or each other");
+  hipStream_t s = as_stream(stream);
+  if (p->d.dtype == NFGS_F32)
+    return project_field_t<float>(p, bnd, static_cast<const float*>(b),
+                                  static_cast<const float*>(az_in), static_cast<float*>(phi),
+                                  static_cast<float*>(az), mean_bcz, workspace, s);
+  return project_field_t<double>(p, bnd, static_cast<const double*>(b),
+                                 static_cast<const double*>(az_in), static_cast<double*>(phi),
+                                 static_cast<double*>(az), mean_bcz, workspace, s);
+}
 
 extern "C" int nfgs_setup_jz(const nfgs_plan* p, const void* bx, const void* by, void* jz, int hny,
                              int hnx, int bfy, int bfx, int ijzbcor, int legacy, void* stream) {

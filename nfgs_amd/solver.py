@@ -680,22 +680,120 @@ class NFGSSolverGPU:
                                           self._stream), "reduce")
         return B, J
 
-    def compute_initial_conditions(self, *, init_mode="gs_cfcf", **_unused):
-        """gs_cfcf start (nfgs_solver.py:6613-6645); returns (B, J, Phi_c, Az, None)."""
+    def compute_initial_conditions(self, *, init_mode="gs_cfcf", init_field=None, init_az=None,
+                                   init_current=None, **_unused):
+        """gs_cfcf start (nfgs_solver.py:6613-6645); returns (B, J, Phi_c, Az, None).
+
+        init_mode="field" (an extension) starts from a caller's field instead:
+        B = init_field and Az = init_az (for example the selected state of an
+        earlier solve, `warm_start_state()`), projected into the NFGS scalars
+        the way the cfcf handoff projects the Stable solver's state
+        (nfgs_solver.py:6080-6147).  init_current, when given, is compared
+        with the projected state's current.
+        """
+        mode = self._init_mode(init_mode)
+        self._require_boundary()
+        if mode == "field":
+            B, J = self._init_from_field(init_field, init_az, init_current, want_fields=True)
+        else:
+            self._advance(0, zero_source=True)
+            sigma, _, _, ratio = self._read_metrics()
+            self._check_zero_mode(ratio)
+            B, J = self._state_fields(0)
+            self._init_diagnostics = self._gs_cfcf_diagnostics()
+        phi, az = self._slots[0]
+        return B, J, phi.clone(), az.clone(), None
+
+    def _init_mode(self, init_mode) -> str:
         mode = _norm(init_mode)
         if mode == "yi_soft":
             mode = "soft_cfcf"
-        if mode not in ("gs_cfcf", "cfcf", "curl_free_current", "soft_cfcf"):
-            raise ValueError("NFGS supports init_mode='gs_cfcf', 'cfcf', or 'soft_cfcf'")
-        self._require_boundary()
-        if mode != "gs_cfcf":
+        if mode not in ("gs_cfcf", "cfcf", "curl_free_current", "soft_cfcf", "field"):
+            raise ValueError("NFGS supports init_mode='gs_cfcf', 'cfcf', or 'soft_cfcf' "
+                             "(and 'field' for a caller's starting field)")
+        if mode not in ("gs_cfcf", "field"):
             _unsupported(f"init_mode={init_mode!r}")
-        self._advance(0, zero_source=True)
-        sigma, _, _, ratio = self._read_metrics()
-        self._check_zero_mode(ratio)
-        B, J = self._state_fields(0)
+        return mode
+
+    def _gs_cfcf_diagnostics(self) -> dict:
+        return {"init_mode": "gs_cfcf", "mfr_count": 0,
+                "gaussian_separation": self.gaussian_separation_diagnostics,
+                "coupled_k0": self._k0_diagnostics}
+
+    def _volume_input(self, a, name, ncomp):
+        shape = (ncomp, self.Nz, self.Ny, self.Nx) if ncomp else (self.Nz, self.Ny, self.Nx)
+        if a is None:
+            raise ValueError(f"init_mode='field' needs {name} of shape {shape}")
+        t = torch.as_tensor(np.asarray(a) if not torch.is_tensor(a) else a)
+        if tuple(t.shape) != shape:
+            raise ValueError(f"{name} must have shape {shape}; got {tuple(t.shape)}")
+        return t.to(device=self.device, dtype=self.dtype).contiguous()
+
+    def _init_from_field(self, init_field, init_az, init_current, want_fields=False):
+        """Slot 0 from a caller's (B, Az): the handoff projection
+        (nfgs_solver.py:6080-6147) through nfgs_project_field, then the state
+        pass of the projected state (its J_par for the first step, its
+        monitors and, for the error measures, its B and J)."""
+        Nz, Ny, Nx = self.Nz, self.Ny, self.Nx
+        N = Nz * Ny * Nx
+        B_in = self._volume_input(init_field, "init_field", 3)
+        az_in = self._volume_input(init_az, "init_az", 0)
+        J_in = None if init_current is None else self._volume_input(init_current, "init_current", 3)
+        lib, plan, s = self._lib, self._plan, self._stream
         phi, az = self._slots[0]
-        return B, J, phi.clone(), az.clone(), None
+        mean = torch.zeros(Nz, device=self.device, dtype=torch.float64)
+        ws = torch.empty(int(lib.nfgs_project_workspace_bytes(plan)), device=self.device,
+                         dtype=torch.uint8)
+        L.check(lib.nfgs_project_field(plan, L.C.byref(self._bnd), L.ptr(B_in), L.ptr(az_in),
+                                       L.ptr(phi), L.ptr(az), L.ptr(mean), L.ptr(ws), s),
+                "field projection")
+        del ws
+        B = self._empty(3, Nz, Ny, Nx)
+        J = self._empty(3, Nz, Ny, Nx)
+        L.check(lib.nfgs_state_pass(plan, L.C.byref(self._bnd), L.ptr(phi), L.ptr(az), L.ptr(B),
+                                    L.ptr(self._ws_t["jp"]), L.ptr(J), L.ptr(self._partials), s),
+                "state pass")
+        L.check(lib.nfgs_reduce(plan, L.ptr(self._partials), L.ptr(self._metrics), s), "reduce")
+        d = torch.zeros(4, device=self.device, dtype=torch.float64)
+        L.check(lib.nfgs_sq_diff(plan, L.ptr(B), L.ptr(B_in), 3 * N, L.ptr(d), s), "field error")
+        if J_in is not None:
+            L.check(lib.nfgs_sq_diff(plan, L.ptr(J), L.ptr(J_in), 3 * N, L.ptr(d) + 16, s),
+                    "current error")
+        dh = d.cpu().numpy()
+        n3 = 3.0 * N
+        field_error = math.sqrt(dh[0] / n3)
+        field_relative = field_error / max(math.sqrt(dh[1] / n3), 1.0e-30)
+        diag = {"init_mode": "field", "mfr_count": 0, "field_handoff": True,
+                "field_handoff_projected": True,
+                "field_rms_difference": field_error,
+                "field_relative_difference": field_relative,
+                "mean_bcz_by_height": [float(v) for v in mean.cpu().numpy()]}
+        projection_error = field_relative
+        if J_in is not None:
+            current_error = math.sqrt(dh[2] / n3)
+            current_relative = current_error / max(math.sqrt(dh[3] / n3), 1.0e-30)
+            diag.update(current_rms_difference=current_error,
+                        current_relative_difference=current_relative)
+            projection_error = max(projection_error, current_relative)
+        if (self.opts.handoff_projection_policy == "error"
+                and projection_error > self.opts.handoff_projection_tolerance):
+            raise ValueError("The starting field is outside the NFGS representable subspace: "
+                             f"field_relative={field_relative:.6e}, "
+                             f"tolerance={self.opts.handoff_projection_tolerance:.6e}.")
+        diag["gaussian_separation"] = self.gaussian_separation_diagnostics
+        diag["coupled_k0"] = self._k0_diagnostics
+        self._init_diagnostics = diag
+        if want_fields:
+            return B, J
+        del B, J
+        return None, None
+
+    def warm_start_state(self):
+        """(B, Az, J) of the selected state on the calculation grid, for
+        init_mode='field' of another solve (an extension; cloned tensors)."""
+        if getattr(self, "_B", None) is None:
+            raise RuntimeError("No selected state yet. Call iterate() first.")
+        return self._B.clone(), self._Az.clone(), self._J.clone()
 
     def iteration_step(self, B, J, Phi_c, Az, B_c=None, mu=1.0, polarity_mask: str = "both"):
         """One map of the state (nfgs_solver.py:6647-6935) at mu = 1.
@@ -739,12 +837,14 @@ class NFGSSolverGPU:
                 mu_min=0.02, mu_max=1.0, mu_backoff=0.5, mu_growth=1.1, growth_streak=5,
                 divergence_factor=2.0, max_rollbacks=12, stop_mode: str = "sigma_best",
                 fixed_point_update_tol=None, fixed_point_pde_tol=1.0e-8,
-                fixed_point_observable_tol=None, fixed_point_patience=10):
+                fixed_point_observable_tol=None, fixed_point_patience=10,
+                init_field=None, init_az=None, init_current=None):
         mode = _norm(init_mode)
         if mode == "yi_soft":
             mode = "soft_cfcf"
-        if mode not in ("gs_cfcf", "cfcf", "curl_free_current", "soft_cfcf"):
-            raise ValueError("NFGS supports init_mode='gs_cfcf', 'cfcf', or 'soft_cfcf'")
+        if mode not in ("gs_cfcf", "cfcf", "curl_free_current", "soft_cfcf", "field"):
+            raise ValueError("NFGS supports init_mode='gs_cfcf', 'cfcf', or 'soft_cfcf' "
+                             "(and 'field' for a caller's starting field)")
         stop_mode = _norm(stop_mode)
         if stop_mode == "fixed_point":
             raise ValueError("stop_mode='fixed_point' requires record_iteration_diagnostics=True")
@@ -764,13 +864,17 @@ class NFGSSolverGPU:
         if float(mu) != 1.0 or adaptive_mu:
             _unsupported("mu != 1 or adaptive_mu")
         self._require_boundary()
-        if mode != "gs_cfcf":
-            _unsupported(f"init_mode={init_mode!r}")
+        mode = self._init_mode(init_mode)
 
-        # initial state in slot 0 (gs_cfcf) and its monitors
-        self._advance(0, zero_source=True)
-        sigma0, divB0, E0, ratio0 = self._read_metrics()
-        self._check_zero_mode(ratio0)
+        # initial state in slot 0 (gs_cfcf, or a caller's field) and its monitors
+        if mode == "field":
+            self._init_from_field(init_field, init_az, init_current)
+            sigma0, divB0, E0, _ = self._read_metrics()  # no source zero mode at the start
+        else:
+            self._advance(0, zero_source=True)
+            sigma0, divB0, E0, ratio0 = self._read_metrics()
+            self._check_zero_mode(ratio0)
+            self._init_diagnostics = self._gs_cfcf_diagnostics()
         if verbose:
             am = self.alpha_max()
             am_text = f"{am:.3e}" if am is not None else "off"
@@ -894,9 +998,7 @@ class NFGSSolverGPU:
                                     "patience": int(fixed_point_patience)},
             sigma_improvement_tol=eps if use_imp else None, auto_rbsl_improvement_grace=0,
             last_sigma_improvement=last_imp,
-            init_diagnostics={"init_mode": "gs_cfcf", "mfr_count": 0,
-                              "gaussian_separation": self.gaussian_separation_diagnostics,
-                              "coupled_k0": self._k0_diagnostics},
+            init_diagnostics=dict(self._init_diagnostics),
             source_partition_mode=self.source_partition_mode,
             bottom_current_grid_mode=self._bottom_current_grid_mode,
             vertical_current_operator=self.vertical_current_operator,

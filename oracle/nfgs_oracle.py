@@ -850,6 +850,25 @@ class OracleNFGS:
         B, J = self.reconstruct(phi, az, lap_xy(phi, self.dx, self.dy))
         return B, J, phi, az
 
+    def project_field(self, B, Az_in):
+        """Caller-field start: the cfcf handoff of nfgs_solver.py:6080-6147 with
+        (B, Az) given.  Phi_c = skip-bottom endpoint-Neumann Poisson of
+        Bc_z = B_z - B_bg,z (:1120-1130, :1146-1180), the constant mode of every
+        plane by height (:4466-4468), Az with the solver's fixed planes; the
+        projected state's B and J by reconstruct().  Returns
+        (Phi_c, Az, mean_bcz_by_height, B_projected, J_projected)."""
+        bcz = B[2] - self.B_background[2]
+        mean = dct2(bcz)[:, 0, 0] / (4.0 * (self.Nx - 1) * (self.Ny - 1))
+        phi = np.empty_like(bcz)
+        phi[0] = self.Phi_above_bottom
+        phi[1:] = self.poisson2d_neumann(bcz[1:])
+        az = np.array(Az_in, dtype=self.dtype, copy=True)
+        az[0] = self.Az_bottom
+        az[-1] = 0.0
+        self._az_lateral_zero(az[1:])
+        Bp, Jp = self.reconstruct(phi, az, lap_xy(phi, self.dx, self.dy))
+        return phi, az, mean, Bp, Jp
+
     def sources(self, B, J):
         """Parallel current, bottom row, and both Poisson sources (nfgs_solver.py:6694-6745)."""
         jp = self.parallel_current(J, B)
@@ -874,8 +893,12 @@ class OracleNFGS:
     # Driver (nfgs_solver.py:3522-3850, :6937-7009)
     # ------------------------------------------------------------------
     def iterate(self, max_iter=200, tol=1e-5, patience=1, sigma_improvement_tol=1e-5,
-                callback=None):
-        B, J, phi, az = self.initial_state()
+                callback=None, start=None):
+        """start=(B, Az): begin from project_field(B, Az) instead of gs_cfcf."""
+        if start is None:
+            B, J, phi, az = self.initial_state()
+        else:
+            phi, az, _, B, J = self.project_field(*start)
         s0, d0, e0 = self.sigma(B, J), self.divB(B), self.energy(B)
         h = {k: [] for k in ("sigma", "divB", "energy", "mu", "rollback", "sigma_relative_change",
                              "energy_relative_change", "fixed_point_criteria_met")}
