@@ -359,6 +359,28 @@ def main():
         err["fp64_sec_to_converge"] = t64
         err["fp64_best_iter"] = h64["best_iter"]
         del s64
+        # the reference's precision comparison in its own terms: agreement
+        # metrics of the FP32 fields against the FP64 ones and sigma_J of the
+        # common FD2 curl, over the observed field of view (GPU scoring; the
+        # oracle's NumPy scoring timed beside it unless --no-cpu)
+        from nfgs_amd import scores as S
+        box = (slice(None),) * 3
+        t0 = time.perf_counter()
+        m = S.compute_reference_metrics(f64, f32, box)
+        sj32, sj64 = S.sigma_j(f32, 1.0, box), S.sigma_j(f64, 1.0, box)
+        t_gpu = time.perf_counter() - t0
+        cs = {k: m[k] for k in ("C_vec", "Eprime_n", "Eprime_m", "epsilon")}
+        cs.update(sigma_J_common_fd2_fp32=sj32, sigma_J_common_fd2_fp64=sj64,
+                  scoring_gpu_s=round(t_gpu, 4))
+        if not a.no_cpu:
+            from oracle import nfgs_oracle as O
+            t0 = time.perf_counter()
+            O.reference_metrics(f64, f32, box)
+            O.common_sigma_j(f32, 1.0, box)
+            O.common_sigma_j(f64, 1.0, box)
+            cs["scoring_cpu_oracle_s"] = round(time.perf_counter() - t0, 3)
+        err["common_scores"] = cs
+        log(f"  common scores FP32 vs FP64: {cs}")
 
     cpu = None
     if not a.no_cpu:

@@ -286,6 +286,38 @@ int nfgs_source_diagnostics(const nfgs_plan* plan, const nfgs_boundary* bnd, con
                             const void* d0, const void* dphi_target, const void* bz_above,
                             void* workspace, double* out, void* stream);
 
+/* ---- common scoring of output fields (FP64, any grid, no plan) --------- */
+
+/* Second-order curl and divergence of b (3*nz*ny*nx, component-first z-y-x):
+ * centred differences inside, the second-order one-sided forms on the end
+ * planes, forward differences on an axis of 2 samples, 0 on an axis of 1
+ * (nfgs/diagnostics/common_fd2.py:23-61).  j (3 volumes) or div (1 volume)
+ * may be NULL. */
+int nfgs_fd2_curl_div(const double* b, int nz, int ny, int nx, double dx, double dy, double dz,
+                      double* j, double* div, void* stream);
+
+/* Sums over the box [z0,z1) x [y0,y1) x [x0,x1) for the agreement metrics of
+ * util/metrics.py:128-176 (b_ref against b) and sigma_J (j against b,
+ * common_fd2.py:64-83 and util/metrics.py:66-92 with |B| floored at b2min).
+ * b_ref or j may be NULL (their sums are then 0).  out: NFGS_SC_COUNT device
+ * doubles; workspace: nfgs_scores_workspace_bytes() bytes of device memory. */
+enum {
+  NFGS_SC_DOT = 0,   /* sum b.B                      */
+  NFGS_SC_REF2,      /* sum |B|^2                    */
+  NFGS_SC_SOL2,      /* sum |b|^2                    */
+  NFGS_SC_COS,       /* sum b.B / (|B||b| + 1e-12)   */
+  NFGS_SC_DIFF,      /* sum |b - B|                  */
+  NFGS_SC_REFN,      /* sum |B|                      */
+  NFGS_SC_DIFF_REL,  /* sum |b - B| / (|B| + 1e-12)  */
+  NFGS_SC_JXB,       /* sum |j x b| / max(|b|, b2min) */
+  NFGS_SC_JN,        /* sum |j|                      */
+  NFGS_SC_COUNT
+};
+long nfgs_scores_workspace_bytes(void);
+int nfgs_field_scores(const double* b_ref, const double* b, const double* j, int nz, int ny,
+                      int nx, int z0, int z1, int y0, int y1, int x0, int x1, double b2min,
+                      double* workspace, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

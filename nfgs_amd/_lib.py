@@ -24,6 +24,9 @@ GD = dict(BX_MEAN=0, BY_MEAN=1, BZ_MEAN=2, D0_MEAN=3, D0_SQ=4, RECON_SQ=5, RECON
           HDIV_SQ=13, HCURL_SQ=14, BZ_SUM=15, BZ_ABS_SUM=16, BX_SUM=17, BY_SUM=18)
 GDIAG_COUNT = 19
 SD_COUNT = 10
+# nfgs_field_scores sums (NFGS_SC_*)
+(SC_DOT, SC_REF2, SC_SOL2, SC_COS, SC_DIFF, SC_REFN, SC_DIFF_REL, SC_JXB, SC_JN) = range(9)
+SC_COUNT = 9
 
 vp = C.c_void_p
 
@@ -79,6 +82,12 @@ def _declare(lib):
         "nfgs_project_workspace_bytes": (C.c_long, [vp]),
         "nfgs_project_field": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp]),
         "nfgs_sq_diff": (C.c_int, [vp, vp, vp, C.c_long, vp, vp]),
+        "nfgs_fd2_curl_div": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                        C.c_double, vp, vp, vp]),
+        "nfgs_scores_workspace_bytes": (C.c_long, []),
+        "nfgs_field_scores": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, vp, vp,
+                                        vp]),
         "nfgs_source_diagnostics_workspace_bytes": (C.c_long, [vp]),
         "nfgs_source_diagnostics": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp]),
     }

@@ -1,0 +1,149 @@
+// Method-independent scoring of output fields: the common second-order curl
+// and divergence (nfgs/diagnostics/common_fd2.py:23-61), sigma_J on a box
+// (common_fd2.py:64-83, util/metrics.py:66-92) and the reference-field
+// agreement metrics (util/metrics.py:128-176).  FP64 on (3, nz, ny, nx)
+// component-first arrays of any size, independent of a plan, as the
+// reference evaluates them ("common_fd2_float64_full_volume_then_crop").
+
+#include <algorithm>
+
+#include "nfgs_common.h"
+
+namespace nfgs {
+namespace {
+
+// d/d(axis) at index i of n samples with stride s: centred inside, the
+// second-order one-sided forms at the ends; n = 2 takes the forward
+// difference at both nodes and n = 1 gives 0 (common_fd2.py:23-37).
+__device__ __forceinline__ double fd2(const double* f, long s, int i, int n, double h) {
+  if (n < 3) return n == 1 ? 0.0 : (f[(1 - i) * s] - f[-i * s]) / h;
+  if (i == 0) return (-3.0 * f[0] + 4.0 * f[s] - f[2 * s]) / (2.0 * h);
+  if (i == n - 1) return (3.0 * f[0] - 4.0 * f[-s] + f[-2 * s]) / (2.0 * h);
+  return (f[s] - f[-s]) / (2.0 * h);
+}
+
+__global__ __launch_bounds__(256) void curl_div_kernel(const double* __restrict__ b, int nz, int ny,
+                                                       int nx, double dx, double dy, double dz,
+                                                       double* __restrict__ j,
+                                                       double* __restrict__ div) {
+  const long N = (long)nz * ny * nx, pl = (long)ny * nx;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int z = (int)(i / pl);
+  const long r = i - z * pl;
+  const int y = (int)(r / nx), x = (int)(r - (long)y * nx);
+  const double *bx = b + i, *by = b + N + i, *bz = b + 2 * N + i;
+  const double dbz_dy = fd2(bz, nx, y, ny, dy), dby_dz = fd2(by, pl, z, nz, dz);
+  const double dbx_dz = fd2(bx, pl, z, nz, dz), dbz_dx = fd2(bz, 1, x, nx, dx);
+  const double dby_dx = fd2(by, 1, x, nx, dx), dbx_dy = fd2(bx, nx, y, ny, dy);
+  if (j) {
+    j[i] = dbz_dy - dby_dz;
+    j[N + i] = dbx_dz - dbz_dx;
+    j[2 * N + i] = dby_dx - dbx_dy;
+  }
+  if (div) div[i] = fd2(bx, 1, x, nx, dx) + fd2(by, nx, y, ny, dy) + fd2(bz, pl, z, nz, dz);
+}
+
+// Per-block sums over the box [z0,z1) x [y0,y1) x [x0,x1) (NFGS_SC_* order).
+constexpr int kScores = NFGS_SC_COUNT;
+__global__ __launch_bounds__(256) void scores_kernel(const double* __restrict__ bref,
+                                                     const double* __restrict__ b,
+                                                     const double* __restrict__ j, int nz, int ny,
+                                                     int nx, int z0, int y0, int x0, int bz_n,
+                                                     int by_n, int bx_n, double b2min,
+                                                     double* __restrict__ part) {
+  __shared__ double red[kScores * 16];
+  const long N = (long)nz * ny * nx, pl = (long)ny * nx;
+  const long M = (long)bz_n * by_n * bx_n, bpl = (long)by_n * bx_n;
+  double acc[kScores];
+#pragma unroll
+  for (int q = 0; q < kScores; ++q) acc[q] = 0.0;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < M;
+       k += (long)gridDim.x * blockDim.x) {
+    const int z = (int)(k / bpl);
+    const long r = k - z * bpl;
+    const int y = (int)(r / bx_n), x = (int)(r - (long)y * bx_n);
+    const long i = (long)(z0 + z) * pl + (long)(y0 + y) * nx + (x0 + x);
+    const double s0 = b[i], s1 = b[N + i], s2 = b[2 * N + i];
+    const double bn = sqrt(s0 * s0 + s1 * s1 + s2 * s2);
+    if (bref) {
+      const double r0 = bref[i], r1 = bref[N + i], r2 = bref[2 * N + i];
+      const double rn = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+      const double dot = r0 * s0 + r1 * s1 + r2 * s2;
+      const double e0 = s0 - r0, e1 = s1 - r1, e2 = s2 - r2;
+      const double en = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+      acc[NFGS_SC_DOT] += dot;
+      acc[NFGS_SC_REF2] += r0 * r0 + r1 * r1 + r2 * r2;
+      acc[NFGS_SC_SOL2] += s0 * s0 + s1 * s1 + s2 * s2;
+      acc[NFGS_SC_COS] += dot / (rn * bn + 1e-12);
+      acc[NFGS_SC_DIFF] += en;
+      acc[NFGS_SC_REFN] += rn;
+      acc[NFGS_SC_DIFF_REL] += en / (rn + 1e-12);
+    }
+    if (j) {
+      const double j0 = j[i], j1 = j[N + i], j2 = j[2 * N + i];
+      const double c0 = j1 * s2 - j2 * s1, c1 = j2 * s0 - j0 * s2, c2 = j0 * s1 - j1 * s0;
+      acc[NFGS_SC_JXB] += sqrt(c0 * c0 + c1 * c1 + c2 * c2) / fmax(bn, b2min);
+      acc[NFGS_SC_JN] += sqrt(j0 * j0 + j1 * j1 + j2 * j2);
+    }
+  }
+  block_sum<kScores>(acc, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < kScores; ++q) part[(long)q * gridDim.x + blockIdx.x] = acc[q];
+}
+
+__global__ __launch_bounds__(256) void scores_final_kernel(const double* __restrict__ part,
+                                                           int nb, double* __restrict__ out) {
+  __shared__ double red[kScores * 16];
+  double acc[kScores];
+#pragma unroll
+  for (int q = 0; q < kScores; ++q) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) s += part[(long)q * nb + k];
+    acc[q] = s;
+  }
+  block_sum<kScores>(acc, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < kScores; ++q) out[q] = acc[q];
+}
+
+constexpr int kScoreBlocks = 1024;
+
+}  // namespace
+}  // namespace nfgs
+
+using namespace nfgs;
+
+extern "C" int nfgs_fd2_curl_div(const double* b, int nz, int ny, int nx, double dx, double dy,
+                                 double dz, double* j, double* div, void* stream) {
+  NFGS_REQUIRE(b && (j || div) && nz >= 1 && ny >= 1 && nx >= 1, NFGS_ERR_ARG,
+               "fd2_curl_div: bad argument");
+  NFGS_REQUIRE(dx > 0 && dy > 0 && dz > 0, NFGS_ERR_ARG, "spacing must be positive");
+  const long N = (long)nz * ny * nx;
+  hipLaunchKernelGGL(curl_div_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                     as_stream(stream), b, nz, ny, nx, dx, dy, dz, j, div);
+  NFGS_LAUNCH_CHECK();
+  return NFGS_OK;
+}
+
+extern "C" long nfgs_scores_workspace_bytes(void) {
+  return (long)kScores * kScoreBlocks * (long)sizeof(double);
+}
+
+extern "C" int nfgs_field_scores(const double* b_ref, const double* b, const double* j, int nz,
+                                 int ny, int nx, int z0, int z1, int y0, int y1, int x0, int x1,
+                                 double b2min, double* workspace, double* out, void* stream) {
+  NFGS_REQUIRE(b && out && workspace && (b_ref || j), NFGS_ERR_ARG, "field_scores: bad argument");
+  NFGS_REQUIRE(0 <= z0 && z0 < z1 && z1 <= nz && 0 <= y0 && y0 < y1 && y1 <= ny && 0 <= x0 &&
+                   x0 < x1 && x1 <= nx,
+               NFGS_ERR_ARG, "field_scores: the box is empty or outside the field");
+  const long M = (long)(z1 - z0) * (y1 - y0) * (x1 - x0);
+  const int nb = (int)std::min<long>(kScoreBlocks, (M + 255) / 256);
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(scores_kernel, dim3(nb), dim3(256), 0, s, b_ref, b, j, nz, ny, nx, z0, y0,
+                     x0, z1 - z0, y1 - y0, x1 - x0, b2min, workspace);
+  NFGS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(scores_final_kernel, dim3(1), dim3(256), 0, s, workspace, nb, out);
+  NFGS_LAUNCH_CHECK();
+  return NFGS_OK;
+}

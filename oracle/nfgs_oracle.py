@@ -1025,3 +1025,75 @@ class OracleNFGS:
             "Jz_projected_out_rms": tor["raw_vs_t_implied_jz_rms"],
             "B_harm_over_Bobs_rms": self.gaussian_diagnostics["bottom_horizontal_reconstruction_ratio"],
         }
+
+
+# ----------------------------------------------------------------------
+# Common scoring of output fields (FP64): the second-order curl and
+# divergence of nfgs/diagnostics/common_fd2.py:23-61, sigma_J on a box
+# (common_fd2.py:64-83, util/metrics.py:66-92) and the reference-field
+# agreement metrics (util/metrics.py:55-63, :128-176).
+# ----------------------------------------------------------------------
+def fd2_derivative(f, h, axis):
+    """Centred inside, second-order one-sided on the end samples; an axis of 2
+    samples takes the forward difference at both, an axis of 1 gives 0."""
+    g = np.moveaxis(np.asarray(f, dtype=np.float64), axis, 0)
+    d = np.zeros_like(g)
+    n = g.shape[0]
+    if n == 2:
+        d[0] = d[1] = (g[1] - g[0]) / h
+    elif n >= 3:
+        d[1:-1] = (g[2:] - g[:-2]) / (2.0 * h)
+        d[0] = (4.0 * g[1] - 3.0 * g[0] - g[2]) / (2.0 * h)
+        d[-1] = (3.0 * g[-1] - 4.0 * g[-2] + g[-3]) / (2.0 * h)
+    return np.moveaxis(d, 0, axis)
+
+
+def fd2_curl(fields, spacing):
+    """Curl of z-y-x arrays; spacing is (dx, dy, dz) or one number."""
+    dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
+    bx, by, bz = fields["Bx"], fields["By"], fields["Bz"]
+    return {"Jx": fd2_derivative(bz, dy, 1) - fd2_derivative(by, dz, 0),
+            "Jy": fd2_derivative(bx, dz, 0) - fd2_derivative(bz, dx, 2),
+            "Jz": fd2_derivative(by, dx, 2) - fd2_derivative(bx, dy, 1)}
+
+
+def fd2_divergence(fields, spacing):
+    dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
+    return (fd2_derivative(fields["Bx"], dx, 2) + fd2_derivative(fields["By"], dy, 1)
+            + fd2_derivative(fields["Bz"], dz, 0))
+
+
+def box_cwsin(B, J, b2min=1e-10):
+    """sum |J x B| / max(|B|, b2min) over sum |J| on component-first boxes."""
+    B = np.asarray(B, dtype=np.float64)
+    J = np.asarray(J, dtype=np.float64)
+    c = np.stack([J[1] * B[2] - J[2] * B[1], J[2] * B[0] - J[0] * B[2],
+                  J[0] * B[1] - J[1] * B[0]])
+    bn = np.sqrt(np.sum(B * B, axis=0))
+    num = np.sum(np.sqrt(np.sum(c * c, axis=0)) / np.maximum(bn, b2min))
+    return float(num / max(float(np.sum(np.sqrt(np.sum(J * J, axis=0)))), 1e-30))
+
+
+def common_sigma_j(fields, spacing, slices):
+    j = fd2_curl(fields, spacing)
+    sz, sy, sx = slices
+    B = np.stack([fields[k] for k in ("Bx", "By", "Bz")])[:, sz, sy, sx]
+    J = np.stack([j[k] for k in ("Jx", "Jy", "Jz")])[:, sz, sy, sx]
+    return box_cwsin(B, J)
+
+
+def reference_metrics(ref_fields, fields, slices):
+    sz, sy, sx = slices
+    R = np.stack([ref_fields[k] for k in ("Bx", "By", "Bz")]).astype(np.float64)[:, sz, sy, sx]
+    S = np.stack([fields[k] for k in ("Bx", "By", "Bz")]).astype(np.float64)[:, sz, sy, sx]
+    J = np.stack([fields[k] for k in ("Jx", "Jy", "Jz")]).astype(np.float64)[:, sz, sy, sx]
+    rn = np.sqrt(np.sum(R * R, axis=0))
+    sn = np.sqrt(np.sum(S * S, axis=0))
+    dot = np.sum(R * S, axis=0)
+    en = np.sqrt(np.sum((S - R) ** 2, axis=0))
+    c_vec = float(np.sum(dot)) / (float(np.sqrt(np.sum(S * S) * np.sum(R * R))) + 1e-12)
+    e_n = float(np.sum(en) / (np.sum(rn) + 1e-12))
+    e_m = float(np.mean(en / (rn + 1e-12)))
+    return {"C_vec": c_vec, "C_CS": float(np.mean(dot / (rn * sn + 1e-12))), "En": e_n, "Em": e_m,
+            "Eprime_n": 1.0 - e_n, "Eprime_m": 1.0 - e_m,
+            "epsilon": float(np.sum(S * S) / (np.sum(R * R) + 1e-12)), "sigma_J": box_cwsin(S, J)}

@@ -73,10 +73,12 @@ def test_struct_layouts_match_the_header(tmp_path):
     # enum values used by the Python layer
     prog2 = tmp_path / "enums.c"
     prog2.write_text("\n".join(['#include <stdio.h>', f'#include "{HEADER}"', "int main(void){",
-                                'printf("%d %d %d %d %d %d\\n", NFGS_NQ, NFGS_GDIAG_COUNT, '
-                                'NFGS_Q_ZERO_MODE, NFGS_GD_BY_SUM, NFGS_ERR_STATE, '
-                                'NFGS_SD_COUNT);', "return 0;}"]))
+                                'printf("%d %d %d %d %d %d %d %d %d\\n", NFGS_NQ, '
+                                'NFGS_GDIAG_COUNT, NFGS_Q_ZERO_MODE, NFGS_GD_BY_SUM, '
+                                'NFGS_ERR_STATE, NFGS_SD_COUNT, NFGS_SC_COUNT, NFGS_SC_JXB, '
+                                'NFGS_SC_DIFF_REL);', "return 0;}"]))
     subprocess.run(["gcc", str(prog2), "-o", str(tmp_path / "enums")], check=True)
     vals = subprocess.run([str(tmp_path / "enums")], capture_output=True, text=True).stdout.split()
     assert list(map(int, vals)) == [L.NQ, L.GDIAG_COUNT, L.Q_ZERO_MODE, L.GD["BY_SUM"],
-                                    L.NFGS_ERR_STATE, L.SD_COUNT]
+                                    L.NFGS_ERR_STATE, L.SD_COUNT, L.SC_COUNT, L.SC_JXB,
+                                    L.SC_DIFF_REL]
