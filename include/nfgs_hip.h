@@ -254,6 +254,20 @@ int nfgs_project_field(const nfgs_plan* plan, const nfgs_boundary* bnd, const vo
                        const void* az_in, void* phi, void* az, double* mean_bcz, void* workspace,
                        void* stream);
 
+/* ---- the outward Gaussian field on z = 0 -------------------------------- */
+
+/* Workspace bytes needed by nfgs_gaussian_above_field (five planes). */
+long nfgs_above_field_workspace_bytes(const nfgs_plan* plan);
+
+/* Replaces NFGSSolver.gaussian_above_bottom_field (nfgs_solver.py:4921-4943):
+ * out (3 planes, component-first x, y, z) = (-d/dx, -d/dy) of
+ * dPhi/dz = idct2(k * phi_above_hat) with the alpha-consistent spectral
+ * gradient (:4385-4406: DST_x x DCT_y and DCT_x x DST_y, zero on the walls
+ * each sine transform leaves out), and B_z = bz_above.  phi_above_hat and
+ * bz_above are the planes nfgs_setup_gaussian wrote. */
+int nfgs_gaussian_above_field(const nfgs_plan* plan, const void* phi_above_hat,
+                              const void* bz_above, void* out, void* workspace, void* stream);
+
 /* ---- post-selection diagnostics ---------------------------------------- */
 
 /* out[0] = sum (a-b)^2, out[1] = sum b^2 over n elements (FP64 device

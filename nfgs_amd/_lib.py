@@ -81,6 +81,8 @@ def _declare(lib):
                                           vp]),
         "nfgs_project_workspace_bytes": (C.c_long, [vp]),
         "nfgs_project_field": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp]),
+        "nfgs_above_field_workspace_bytes": (C.c_long, [vp]),
+        "nfgs_gaussian_above_field": (C.c_int, [vp, vp, vp, vp, vp, vp]),
         "nfgs_sq_diff": (C.c_int, [vp, vp, vp, C.c_long, vp, vp]),
         "nfgs_fd2_curl_div": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                         C.c_double, vp, vp, vp]),

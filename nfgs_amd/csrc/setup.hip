@@ -350,10 +350,53 @@ int project_field_t(const nfgs_plan* p, const nfgs_boundary* bnd, const T* b, co
   }, s);
 }
 
+// The above-bottom (outward) Gaussian field on z = 0, (3, ny, nx)
+// component-first: B_h = -grad_h(dPhi/dz) with dPhi/dz = idct2(k Phi^>_hat)
+// and the alpha-consistent spectral gradient, B_z = Bz_above
+// (nfgs_solver.py:4921-4943, gradient :4385-4406).  Workspace: 5 planes.
+template <typename T>
+int above_field_t(const nfgs_plan* p, const T* pah, const T* bz_above, T* out, void* workspace,
+                  hipStream_t s) {
+  Ctx<T> c{p, s, p->d.nx, p->d.ny, p->d.nz, p->plane};
+  const long pl = c.pl;
+  const T* kk = static_cast<const T*>(p->kk);
+  T* w = static_cast<T*>(workspace);
+  T *dph = w, *dphi = w + pl, *gh = w + 2 * pl, *t1 = w + 3 * pl, *t2 = w + 4 * pl;
+  T* oz = out + 2 * pl;
+  NFGS_TRY(launch_map(pl, [=] __device__(long i) {
+    dph[i] = kk[i] * pah[i];
+    oz[i] = bz_above[i];
+  }, s));
+  NFGS_TRY(xy_pair(c, NFGS_INVERSE, dph, dphi, t1, 1));
+  NFGS_TRY(xy_pair(c, NFGS_FORWARD, dphi, gh, t1, 1));
+  return neg_spectral_grad(c, gh, out, out + pl, t1, t2, 1);
+}
+
 }  // namespace
 }  // namespace nfgs
 
 using namespace nfgs;
+
+extern "C" long nfgs_above_field_workspace_bytes(const nfgs_plan* p) {
+  return p ? 5 * p->plane * (long)p->esize : 0;
+}
+
+extern "C" int nfgs_gaussian_above_field(const nfgs_plan* p, const void* phi_above_hat,
+                                         const void* bz_above, void* out, void* workspace,
+                                         void* stream) {
+  NFGS_REQUIRE(p && phi_above_hat && bz_above && out && workspace, NFGS_ERR_ARG,
+               "gaussian_above_field: null argument");
+  NFGS_REQUIRE(out != phi_above_hat && out != bz_above, NFGS_ERR_ARG,
+               "gaussian_above_field: the output must not alias an input");
+  hipStream_t s = as_stream(stream);
+  if (p->d.dtype == NFGS_F32)
+    return above_field_t<float>(p, static_cast<const float*>(phi_above_hat),
+                                static_cast<const float*>(bz_above), static_cast<float*>(out),
+                                workspace, s);
+  return above_field_t<double>(p, static_cast<const double*>(phi_above_hat),
+                               static_cast<const double*>(bz_above), static_cast<double*>(out),
+                               workspace, s);
+}
 
 extern "C" long nfgs_project_workspace_bytes(const nfgs_plan* p) {
   return p ? 3 * p->N * (long)p->esize : 0;

@@ -795,6 +795,22 @@ class NFGSSolverGPU:
             raise RuntimeError("No selected state yet. Call iterate() first.")
         return self._B.clone(), self._Az.clone(), self._J.clone()
 
+    def gaussian_above_bottom_field(self) -> torch.Tensor:
+        """The outward Gaussian field B^> on z = 0, shape (3, Ny, Nx)
+        (nfgs_solver.py:4921-4943): B_h = -grad_h(dPhi/dz) with
+        dPhi/dz = idct2(k Phi^>_hat) and the spectral gradient, B_z = Bz_above.
+        Computed by nfgs_gaussian_above_field on the device."""
+        if self.B_background is None or getattr(self, "_phi_above_hat", None) is None:
+            raise RuntimeError("NFGS Gaussian separation is not configured")
+        lib = self._lib
+        out = self._empty(3, self.Ny, self.Nx)
+        ws = torch.empty(int(lib.nfgs_above_field_workspace_bytes(self._plan)),
+                         device=self.device, dtype=torch.uint8)
+        L.check(lib.nfgs_gaussian_above_field(self._plan, L.ptr(self._phi_above_hat),
+                                              L.ptr(self.Bz_above), L.ptr(out), L.ptr(ws),
+                                              self._stream), "Gaussian above field")
+        return out
+
     def iteration_step(self, B, J, Phi_c, Az, B_c=None, mu=1.0, polarity_mask: str = "both"):
         """One map of the state (nfgs_solver.py:6647-6935) at mu = 1.
 

@@ -655,6 +655,13 @@ class OracleNFGS:
             "harmonic_residual_curl_rms": rms(curl),
         }
 
+    def gaussian_above_bottom_field(self):
+        """The outward Gaussian field on z = 0, (3, Ny, Nx) (nfgs_solver.py:4921-4943):
+        B_h = -spectral grad_h of idct2(k Phi^>_hat), B_z = Bz_above."""
+        dphi = idct2(self.k * self.phi_above_hat)
+        gx, gy = self.spectral_dx_2d(dphi), self.spectral_dy_2d(dphi)
+        return np.stack([-gx, -gy, self.Bz_above]).astype(self.dtype)
+
     def _zero_mode_policy(self, ratio, tol, policy, message):
         """nfgs_solver.py:4477-4487."""
         if ratio <= tol:
