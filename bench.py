@@ -372,6 +372,12 @@ def main():
         cs = {k: m[k] for k in ("C_vec", "Eprime_n", "Eprime_m", "epsilon")}
         cs.update(sigma_J_common_fd2_fp32=sj32, sigma_J_common_fd2_fp64=sj64,
                   scoring_gpu_s=round(t_gpu, 4))
+        # bottom-plane re-separation: E^rel of the outward Gaussian part
+        from nfgs_amd import cci
+        ny_o, nx_o = f64["Bz"].shape[1:]
+        helper = cci.make_bgt_helper_solver(nx=nx_o, ny=ny_o, dx=1.0, dy=1.0, dz=1.0)
+        cs["Erel_Bgt_bottom"] = cci.bottom_bgt_relative_error(helper, f64, f32)
+        del helper
         if not a.no_cpu:
             from oracle import nfgs_oracle as O
             t0 = time.perf_counter()
