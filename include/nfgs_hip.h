@@ -311,10 +311,13 @@ int nfgs_fd2_curl_div(const double* b, int nz, int ny, int nx, double dx, double
                       double* j, double* div, void* stream);
 
 /* Sums over the box [z0,z1) x [y0,y1) x [x0,x1) for the agreement metrics of
- * util/metrics.py:128-176 (b_ref against b) and sigma_J (j against b,
- * common_fd2.py:64-83 and util/metrics.py:66-92 with |B| floored at b2min).
- * b_ref or j may be NULL (their sums are then 0).  out: NFGS_SC_COUNT device
- * doubles; workspace: nfgs_scores_workspace_bytes() bytes of device memory. */
+ * util/metrics.py:128-176 (b_ref against b), sigma_J (j against b,
+ * common_fd2.py:64-83 and util/metrics.py:66-92 with |B| floored at b2min)
+ * and the normalised divergence of the controlled evaluation (div and |b|,
+ * nfgs/_backend/scripts/recompute_td_referee_diagnostics.py:344-400).
+ * b_ref, j or div may be NULL (their sums are then 0).  out: NFGS_SC_COUNT
+ * device doubles; workspace: nfgs_scores_workspace_bytes() bytes of device
+ * memory. */
 enum {
   NFGS_SC_DOT = 0,   /* sum b.B                      */
   NFGS_SC_REF2,      /* sum |B|^2                    */
@@ -325,12 +328,14 @@ enum {
   NFGS_SC_DIFF_REL,  /* sum |b - B| / (|B| + 1e-12)  */
   NFGS_SC_JXB,       /* sum |j x b| / max(|b|, b2min) */
   NFGS_SC_JN,        /* sum |j|                      */
+  NFGS_SC_BN,        /* sum |b|                      */
+  NFGS_SC_DIVABS,    /* sum |div|                    */
   NFGS_SC_COUNT
 };
 long nfgs_scores_workspace_bytes(void);
-int nfgs_field_scores(const double* b_ref, const double* b, const double* j, int nz, int ny,
-                      int nx, int z0, int z1, int y0, int y1, int x0, int x1, double b2min,
-                      double* workspace, double* out, void* stream);
+int nfgs_field_scores(const double* b_ref, const double* b, const double* j, const double* div,
+                      int nz, int ny, int nx, int z0, int z1, int y0, int y1, int x0, int x1,
+                      double b2min, double* workspace, double* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -25,8 +25,9 @@ GD = dict(BX_MEAN=0, BY_MEAN=1, BZ_MEAN=2, D0_MEAN=3, D0_SQ=4, RECON_SQ=5, RECON
 GDIAG_COUNT = 19
 SD_COUNT = 10
 # nfgs_field_scores sums (NFGS_SC_*)
-(SC_DOT, SC_REF2, SC_SOL2, SC_COS, SC_DIFF, SC_REFN, SC_DIFF_REL, SC_JXB, SC_JN) = range(9)
-SC_COUNT = 9
+(SC_DOT, SC_REF2, SC_SOL2, SC_COS, SC_DIFF, SC_REFN, SC_DIFF_REL, SC_JXB, SC_JN, SC_BN,
+ SC_DIVABS) = range(11)
+SC_COUNT = 11
 
 vp = C.c_void_p
 
@@ -87,7 +88,7 @@ def _declare(lib):
         "nfgs_fd2_curl_div": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                         C.c_double, vp, vp, vp]),
         "nfgs_scores_workspace_bytes": (C.c_long, []),
-        "nfgs_field_scores": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+        "nfgs_field_scores": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, vp, vp,
                                         vp]),
         "nfgs_source_diagnostics_workspace_bytes": (C.c_long, [vp]),

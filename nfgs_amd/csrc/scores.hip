@@ -1,7 +1,8 @@
 // Method-independent scoring of output fields: the common second-order curl
 // and divergence (nfgs/diagnostics/common_fd2.py:23-61), sigma_J on a box
-// (common_fd2.py:64-83, util/metrics.py:66-92) and the reference-field
-// agreement metrics (util/metrics.py:128-176).  FP64 on (3, nz, ny, nx)
+// (common_fd2.py:64-83, util/metrics.py:66-92), the reference-field
+// agreement metrics (util/metrics.py:128-176) and the normalised divergence
+// of the controlled evaluation (recompute_td_referee_diagnostics.py:344-400).  FP64 on (3, nz, ny, nx)
 // component-first arrays of any size, independent of a plan, as the
 // reference evaluates them ("common_fd2_float64_full_volume_then_crop").
 
@@ -48,7 +49,8 @@ __global__ __launch_bounds__(256) void curl_div_kernel(const double* __restrict_
 constexpr int kScores = NFGS_SC_COUNT;
 __global__ __launch_bounds__(256) void scores_kernel(const double* __restrict__ bref,
                                                      const double* __restrict__ b,
-                                                     const double* __restrict__ j, int nz, int ny,
+                                                     const double* __restrict__ j,
+                                                     const double* __restrict__ dv, int nz, int ny,
                                                      int nx, int z0, int y0, int x0, int bz_n,
                                                      int by_n, int bx_n, double b2min,
                                                      double* __restrict__ part) {
@@ -66,6 +68,8 @@ __global__ __launch_bounds__(256) void scores_kernel(const double* __restrict__ 
     const long i = (long)(z0 + z) * pl + (long)(y0 + y) * nx + (x0 + x);
     const double s0 = b[i], s1 = b[N + i], s2 = b[2 * N + i];
     const double bn = sqrt(s0 * s0 + s1 * s1 + s2 * s2);
+    acc[NFGS_SC_BN] += bn;
+    if (dv) acc[NFGS_SC_DIVABS] += fabs(dv[i]);
     if (bref) {
       const double r0 = bref[i], r1 = bref[N + i], r2 = bref[2 * N + i];
       const double rn = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
@@ -130,18 +134,19 @@ extern "C" long nfgs_scores_workspace_bytes(void) {
   return (long)kScores * kScoreBlocks * (long)sizeof(double);
 }
 
-extern "C" int nfgs_field_scores(const double* b_ref, const double* b, const double* j, int nz,
-                                 int ny, int nx, int z0, int z1, int y0, int y1, int x0, int x1,
-                                 double b2min, double* workspace, double* out, void* stream) {
-  NFGS_REQUIRE(b && out && workspace && (b_ref || j), NFGS_ERR_ARG, "field_scores: bad argument");
+extern "C" int nfgs_field_scores(const double* b_ref, const double* b, const double* j,
+                                 const double* div, int nz, int ny, int nx, int z0, int z1, int y0,
+                                 int y1, int x0, int x1, double b2min, double* workspace,
+                                 double* out, void* stream) {
+  NFGS_REQUIRE(b && out && workspace, NFGS_ERR_ARG, "field_scores: bad argument");
   NFGS_REQUIRE(0 <= z0 && z0 < z1 && z1 <= nz && 0 <= y0 && y0 < y1 && y1 <= ny && 0 <= x0 &&
                    x0 < x1 && x1 <= nx,
                NFGS_ERR_ARG, "field_scores: the box is empty or outside the field");
   const long M = (long)(z1 - z0) * (y1 - y0) * (x1 - x0);
   const int nb = (int)std::min<long>(kScoreBlocks, (M + 255) / 256);
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(scores_kernel, dim3(nb), dim3(256), 0, s, b_ref, b, j, nz, ny, nx, z0, y0,
-                     x0, z1 - z0, y1 - y0, x1 - x0, b2min, workspace);
+  hipLaunchKernelGGL(scores_kernel, dim3(nb), dim3(256), 0, s, b_ref, b, j, div, nz, ny, nx, z0,
+                     y0, x0, z1 - z0, y1 - y0, x1 - x0, b2min, workspace);
   NFGS_LAUNCH_CHECK();
   hipLaunchKernelGGL(scores_final_kernel, dim3(1), dim3(256), 0, s, workspace, nb, out);
   NFGS_LAUNCH_CHECK();

@@ -2,8 +2,10 @@
 
 Same names, arguments and results as the reference's scoring helpers:
 `curl`, `divergence` and `sigma_j` of nfgs/diagnostics/common_fd2.py:41-83,
-and `roi_cwsin`, `vector_correlation` and `compute_reference_metrics` of
-nfgs/_backend/util/metrics.py:55-176.  Fields are dicts of (nz, ny, nx)
+`roi_cwsin`, `vector_correlation` and `compute_reference_metrics` of
+nfgs/_backend/util/metrics.py:55-176, and `compute_metrics` of
+nfgs/_backend/scripts/recompute_td_referee_diagnostics.py:344-400 (the
+controlled evaluation's metrics).  Fields are dicts of (nz, ny, nx)
 arrays (NumPy or torch); the arithmetic runs in the HIP kernels of
 libnfgs_hip.so (nfgs_fd2_curl_div, nfgs_field_scores) in FP64, as the
 reference evaluates its common diagnostics.
@@ -34,6 +36,14 @@ def _vec(parts):
         raise ValueError(f"components must be (nz, ny, nx) arrays of one shape; got "
                          f"{[tuple(t.shape) for t in ts]}")
     return torch.stack([t.to(device=dev, dtype=torch.float64) for t in ts]).contiguous()
+
+
+def _vol(a):
+    """(nz, ny, nx) FP64 device tensor from one array."""
+    t = torch.as_tensor(np.asarray(a) if not torch.is_tensor(a) else a)
+    if t.dim() != 3:
+        raise ValueError(f"expected an (nz, ny, nx) array; got {tuple(t.shape)}")
+    return t.to(device=_device(), dtype=torch.float64).contiguous()
 
 
 def _spacing_xyz(spacing):
@@ -67,15 +77,15 @@ def _curl_div(B, spacing, want_j=True, want_div=False):
     return j, d
 
 
-def _sums(bref, b, j, slices, b2min):
+def _sums(bref, b, j, slices, b2min, div=None):
     lib = L.load()
     _, nz, ny, nx = b.shape
     box = _box(slices, (nz, ny, nx))
     ws = torch.empty(int(lib.nfgs_scores_workspace_bytes()) // 8, device=b.device,
                      dtype=torch.float64)
     out = torch.zeros(L.SC_COUNT, device=b.device, dtype=torch.float64)
-    L.check(lib.nfgs_field_scores(L.ptr(bref) if bref is not None else None, L.ptr(b),
-                                  L.ptr(j) if j is not None else None, nz, ny, nx, *box,
+    opt = [L.ptr(t) if t is not None else None for t in (bref, j, div)]
+    L.check(lib.nfgs_field_scores(opt[0], L.ptr(b), opt[1], opt[2], nz, ny, nx, *box,
                                   float(b2min), L.ptr(ws), L.ptr(out),
                                   torch.cuda.current_stream().cuda_stream), "field scores")
     s = out.cpu().numpy()
@@ -142,3 +152,29 @@ def compute_reference_metrics(ref_fields, fields, metric_slices):
             "Eprime_n": 1.0 - e_n, "Eprime_m": 1.0 - e_m,
             "epsilon": float(s[L.SC_SOL2] / (s[L.SC_REF2] + 1e-12)),
             "sigma_J": float(s[L.SC_JXB] / max(float(s[L.SC_JN]), 1e-30))}
+
+
+def compute_metrics(reference, candidate, current, divergence, slices, spacing) -> dict:
+    """Agreement metrics, sigma_J and the normalised divergence of a candidate
+    on a z-y-x box (recompute_td_referee_diagnostics.py:344-400); `current`
+    holds Jx, Jy, Jz, `divergence` is one volume and `spacing` the effective
+    grid step of the divergence normalisation."""
+    R = _vec([reference[k] for k in _B])
+    S = _vec([candidate[k] for k in _B])
+    J = _vec([current[k] for k in _J])
+    D = _vol(divergence)
+    if R.shape != S.shape or J.shape != S.shape or tuple(D.shape) != tuple(S.shape[1:]):
+        raise ValueError("reference, candidate, current and divergence must have one shape")
+    s, count = _sums(R, S, J, slices, 1.0e-10, div=D)
+    n = max(count, 1.0)
+    div_mean = float(s[L.SC_DIVABS]) / n
+    bn_mean = float(s[L.SC_BN]) / n
+    return {
+        "C_vec": float(s[L.SC_DOT]) / (float(np.sqrt(s[L.SC_REF2] * s[L.SC_SOL2])) + 1.0e-12),
+        "C_CS": float(s[L.SC_COS]) / n,
+        "Eprime_n": 1.0 - float(s[L.SC_DIFF]) / (float(s[L.SC_REFN]) + 1.0e-12),
+        "Eprime_m": 1.0 - float(s[L.SC_DIFF_REL]) / n,
+        "epsilon": float(s[L.SC_SOL2]) / (float(s[L.SC_REF2]) + 1.0e-12),
+        "sigma_J": float(s[L.SC_JXB]) / max(float(s[L.SC_JN]), 1.0e-30),
+        "divB_normalized": div_mean / (6.0 * bn_mean / float(spacing) + 1.0e-12),
+    }

@@ -20,6 +20,14 @@ Parity pins (what checks each part; see tests/test_oracle_*.py):
                          (linear field curl 3.0; sign-preserving balance)
   * everything composite golden fixtures written by tests/golden/make_golden.py
                          (reference imported on CPU, FP64 and FP32)
+  * project_field        round trips on representable states, the dropped
+                         per-plane constant mode (test_oracle_handoff.py)
+  * common scoring       the reference's known-answer polynomial field and
+                         closed forms of the metrics (test_oracle_scores.py)
+  * outward field        potential fields of sources above / below the
+                         bottom plane (test_oracle_above_field.py)
+  * td_metrics           the divergence term on fields of known divergence
+                         (test_controlled_contract.py)
 No function is left "parity unpinned".
 """
 from __future__ import annotations
@@ -1104,3 +1112,19 @@ def reference_metrics(ref_fields, fields, slices):
     return {"C_vec": c_vec, "C_CS": float(np.mean(dot / (rn * sn + 1e-12))), "En": e_n, "Em": e_m,
             "Eprime_n": 1.0 - e_n, "Eprime_m": 1.0 - e_m,
             "epsilon": float(np.sum(S * S) / (np.sum(R * R) + 1e-12)), "sigma_J": box_cwsin(S, J)}
+
+
+def td_metrics(reference, candidate, current, divergence, slices, spacing):
+    """The controlled evaluation's metrics
+    (nfgs/_backend/scripts/recompute_td_referee_diagnostics.py:344-400): the
+    agreement metrics and sigma_J of reference_metrics plus the divergence
+    normalised by 6 mean|B| / spacing."""
+    sz, sy, sx = slices
+    fields = dict(candidate)
+    fields.update({k: current[k] for k in ("Jx", "Jy", "Jz")})
+    m = reference_metrics(reference, fields, slices)
+    S = np.stack([candidate[k] for k in ("Bx", "By", "Bz")]).astype(np.float64)[:, sz, sy, sx]
+    bn_mean = float(np.mean(np.sqrt(np.sum(S * S, axis=0))))
+    div_mean = float(np.mean(np.abs(np.asarray(divergence, dtype=np.float64)[sz, sy, sx])))
+    return {k: m[k] for k in ("C_vec", "C_CS", "Eprime_n", "Eprime_m", "epsilon", "sigma_J")} | {
+        "divB_normalized": div_mean / (6.0 * bn_mean / float(spacing) + 1e-12)}
