@@ -108,7 +108,8 @@ struct Tile {
   int id;  // logical tile id (x fastest, then y, then z-chunk)
   int x0, y0, k0, tx, ty, x, y;
   bool valid;
-  int c;  // LDS index of this thread's point
+  bool narrow;  // a last-column tile holding at most TX/4 grid columns
+  int c;        // LDS index of this thread's point
 };
 
 template <typename T>
@@ -119,8 +120,13 @@ __device__ __forceinline__ Tile make_tile(const Grid<T>& g) {
   t.x0 = bx * TX - g.sx;
   t.y0 = (r % g.tgy) * TY - g.sy;
   t.k0 = (r / g.tgy) * g.kc;
-  t.tx = threadIdx.x % TX;
-  t.ty = threadIdx.x / TX;
+  // A narrow tile (the grid's last few columns, e.g. 2 of 32 on a 258-wide
+  // grid) numbers its points column-major, so its valid points fill the first
+  // wave and the other waves, whose lanes all lie outside the grid, skip the
+  // per-point work.
+  t.narrow = g.nx - t.x0 <= TX / 4;
+  t.tx = t.narrow ? (int)threadIdx.x / TY : (int)threadIdx.x % TX;
+  t.ty = t.narrow ? (int)threadIdx.x % TY : (int)threadIdx.x / TX;
   t.x = t.x0 + t.tx;
   t.y = t.y0 + t.ty;
   t.valid = t.x >= 0 && t.y >= 0 && t.x < g.nx && t.y < g.ny;
@@ -680,7 +686,9 @@ __device__ __forceinline__ void state_pass_body(
       for (int i = 0; i < R1PT; ++i) {
         const int r = threadIdx.x + 256 * i;
         if (r < R1N) {
-          const int ly = 1 + r / R1W, lx = 1 + r % R1W;
+          // narrow tiles: column-major, so the in-grid ring points share a wave
+          const bool cm = !INNER && t.narrow;
+          const int ly = 1 + (cm ? r % R1H : r / R1W), lx = 1 + (cm ? r / R1H : r % R1W);
           const int e = ly * PW + lx;
           const int y = t.y0 + ly - HH, x = t.x0 + lx - HH;
           T vbx = T(0), vby = T(0), vcz = T(0);

@@ -97,3 +97,40 @@ def test_state_pass_equals_reconstruct_then_face(dtype, shape):
         assert torch.equal(met2[q], out[True][3][q])
     finally:
         lib.nfgs_plan_destroy(plan)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nxo", [34, 65])
+def test_narrow_last_tile_column_against_oracle(dtype, nxo):
+    """Grids whose last tile column holds two grid columns (34 = 32 + 2; 65 =
+    1 mod 32, shifted tiling): those tiles number their points column-major.
+    A short unpadded solve against the oracle's."""
+    from nfgs_amd import NFGSSolverGPU
+    from oracle.nfgs_oracle import OracleNFGS
+    from tests.golden_cases import rel_l2
+    print(f"[gpu-case] narrow tiles nx={nxo} {np.dtype(dtype).name}", flush=True)
+    sys.stdout.flush()
+    nyo, nz = 12, 9
+    rng = np.random.default_rng(nxo)
+    y, x = np.mgrid[0:nyo, 0:nxo]
+    bz = (np.exp(-((x - 0.3 * nxo) ** 2 + (y - 5) ** 2) / 18.0)
+          - np.exp(-((x - 0.7 * nxo) ** 2 + (y - 6) ** 2) / 14.0))
+    bx = 0.2 * np.sin(x / 5.0) + 0.05 * rng.standard_normal((nyo, nxo))
+    by = 0.2 * np.cos(y / 4.0) + 0.05 * rng.standard_normal((nyo, nxo))
+    o = OracleNFGS(nxo, nyo, nz, 1.0, 1.0, 1.0, dtype=dtype, bottom_divergence_robin_weight=0.5,
+                   pad_calculation_domain=False, source_zero_mode_policy="project",
+                   d0_zero_mode_policy="project")
+    o.set_bottom_vector_boundary(bx, by, bz)
+    ho = o.iterate(max_iter=4, tol=1e-30, patience=50, sigma_improvement_tol=None)
+    s = NFGSSolverGPU(nxo, nyo, nz, 1.0, 1.0, 1.0, device="cuda",
+                      dtype=torch.float64 if dtype == np.float64 else torch.float32,
+                      pad_calculation_domain=False, bottom_divergence_robin_weight=0.5,
+                      d0_zero_mode_policy="project", source_zero_mode_policy="project")
+    s.set_bottom_vector_boundary(bx, by, bz, zero_net_current=False, smooth_passes=0, ijzbcor=3)
+    h = s.iterate(max_iter=4, tol=1e-30, patience=50, sigma_improvement_tol=None, verbose=False)
+    tol = 1e-10 if dtype == np.float64 else 2e-4
+    assert rel_l2(np.array(h["sigma"]), np.array(ho["sigma"])) < tol
+    assert rel_l2(np.array(h["energy"]), np.array(ho["energy"])) < tol
+    f, fo = s.fields_to_numpy(), o.fields_to_numpy()
+    for k in ("Bx", "By", "Bz"):
+        assert rel_l2(f[k], fo[k]) < (1e-10 if dtype == np.float64 else 1e-4), k
