@@ -178,6 +178,17 @@ int nfgs_reduce(const nfgs_plan* plan, const double* partials, double* metrics, 
 int nfgs_advance(const nfgs_plan* plan, const nfgs_boundary* bnd, const nfgs_workspace* ws,
                  void* phi_out, void* az_out, void* b_out, int zero_source, void* stream);
 
+/* nfgs_advance with the poloidal update selected (poloidal_update_mode,
+ * nfgs_solver.py:5727-5733): NFGS_POLOIDAL_DIRECT solves Phi_c directly
+ * (:5627-5672, what nfgs_advance does); NFGS_POLOIDAL_TWO_STAGE solves B_c,z
+ * with the same rows and recovers Phi_c_hat = B_c,z_hat / lambda plane by
+ * plane (:5674-5725; four more transform passes).  The two agree up to
+ * rounding.  s_bcz also carries the two lambda-scaled bottom planes. */
+enum { NFGS_POLOIDAL_DIRECT = 0, NFGS_POLOIDAL_TWO_STAGE = 1 };
+int nfgs_advance_ex(const nfgs_plan* plan, const nfgs_boundary* bnd, const nfgs_workspace* ws,
+                    void* phi_out, void* az_out, void* b_out, int zero_source, int poloidal_mode,
+                    void* stream);
+
 /* --- boundary setup (set_bottom_vector_boundary, once per boundary) --- */
 
 /* Fortran readin+botbal bottom Jz (nfgs_solver.py:1498-1567 legacy helper

@@ -28,6 +28,8 @@ SD_COUNT = 10
 (SC_DOT, SC_REF2, SC_SOL2, SC_COS, SC_DIFF, SC_REFN, SC_DIFF_REL, SC_JXB, SC_JN, SC_BN,
  SC_DIVABS) = range(11)
 SC_COUNT = 11
+# nfgs_advance_ex poloidal update modes (NFGS_POLOIDAL_*)
+POLOIDAL_DIRECT, POLOIDAL_TWO_STAGE = 0, 1
 
 vp = C.c_void_p
 
@@ -73,6 +75,8 @@ def _declare(lib):
         "nfgs_sources": (C.c_int, [vp, vp, vp, vp, vp, vp]),
         "nfgs_reduce": (C.c_int, [vp, vp, vp, vp]),
         "nfgs_advance": (C.c_int, [vp, P(Boundary), P(Workspace), vp, vp, vp, C.c_int, vp]),
+        "nfgs_advance_ex": (C.c_int, [vp, P(Boundary), P(Workspace), vp, vp, vp, C.c_int, C.c_int,
+                                      vp]),
         "nfgs_setup_jz": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, vp]),
         "nfgs_setup_poisson2d_dirichlet": (C.c_int, [vp, vp, C.c_double, vp, vp, vp]),

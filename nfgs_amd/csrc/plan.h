@@ -51,4 +51,13 @@ int fill_zero_f64(double* ptr, long n, hipStream_t s);
 int launch_transform(const nfgs_plan* p, const PairedMatrix* m, int direction, const void* in,
                      long ies, long ivs, long ips, void* out, long oes, long ovs, long ops,
                      int nvec, int nplanes, hipStream_t s);
+// Modal Robin z-solve with explicit bottom tables: (d_zz + lambda) u = S/lambda
+// (div_src) or = S, u(0) = bh, the rho-mixed first row with target qh.
+int robin_modal(const nfgs_plan* p, const void* bh, const void* qh, const void* src, void* out,
+                void* scr, double* partials, bool div_src, hipStream_t s);
+// Two-stage Phi_c update helpers: out[0:plane] = lambda * phi_bottom_hat and
+// out[plane:2 plane] = lambda * q_hat (0 without q_hat); hat[q][j] /= lambda[j]
+// on nplanes planes, the constant mode set to 0.
+int two_stage_tables(const nfgs_plan* p, const nfgs_boundary* bnd, void* out, hipStream_t s);
+int divide_by_lambda(const nfgs_plan* p, void* hat, int nplanes, hipStream_t s);
 }  // namespace nfgs

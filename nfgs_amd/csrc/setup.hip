@@ -372,7 +372,40 @@ int above_field_t(const nfgs_plan* p, const T* pah, const T* bz_above, T* out, v
   return neg_spectral_grad(c, gh, out, out + pl, t1, t2, 1);
 }
 
+template <typename T>
+int two_stage_tables_t(const nfgs_plan* p, const nfgs_boundary* bnd, T* out, hipStream_t s) {
+  const long pl = p->plane;
+  const T* lam = static_cast<const T*>(p->lam);
+  const T* bh = static_cast<const T*>(bnd->phi_bottom_hat);
+  const T* qh = static_cast<const T*>(bnd->q_hat);
+  return launch_map(pl, [=] __device__(long i) {
+    out[i] = lam[i] * bh[i];
+    out[pl + i] = qh ? lam[i] * qh[i] : T(0);
+  }, s);
+}
+
+template <typename T>
+int divide_by_lambda_t(const nfgs_plan* p, T* hat, int nplanes, hipStream_t s) {
+  const long pl = p->plane;
+  const T* lam = static_cast<const T*>(p->lam);
+  return launch_map(pl * nplanes, [=] __device__(long i) {
+    const long j = i % pl;
+    hat[i] = j == 0 ? T(0) : hat[i] / lam[j];
+  }, s);
+}
+
 }  // namespace
+
+int two_stage_tables(const nfgs_plan* p, const nfgs_boundary* bnd, void* out, hipStream_t s) {
+  if (p->d.dtype == NFGS_F32) return two_stage_tables_t(p, bnd, static_cast<float*>(out), s);
+  return two_stage_tables_t(p, bnd, static_cast<double*>(out), s);
+}
+
+int divide_by_lambda(const nfgs_plan* p, void* hat, int nplanes, hipStream_t s) {
+  if (p->d.dtype == NFGS_F32) return divide_by_lambda_t(p, static_cast<float*>(hat), nplanes, s);
+  return divide_by_lambda_t(p, static_cast<double*>(hat), nplanes, s);
+}
+
 }  // namespace nfgs
 
 using namespace nfgs;

@@ -19,11 +19,13 @@ inline const void* at(const nfgs_plan* p, const void* base, long elems) {
 
 }  // namespace
 
-extern "C" int nfgs_advance(const nfgs_plan* p, const nfgs_boundary* bnd,
-                            const nfgs_workspace* ws, void* phi_out, void* az_out, void* b_out,
-                            int zero_source, void* stream) {
+extern "C" int nfgs_advance_ex(const nfgs_plan* p, const nfgs_boundary* bnd,
+                               const nfgs_workspace* ws, void* phi_out, void* az_out, void* b_out,
+                               int zero_source, int poloidal_mode, void* stream) {
   using namespace nfgs;
   NFGS_REQUIRE(p && bnd && ws && phi_out && az_out, NFGS_ERR_ARG, "advance: null argument");
+  NFGS_REQUIRE(poloidal_mode == NFGS_POLOIDAL_DIRECT || poloidal_mode == NFGS_POLOIDAL_TWO_STAGE,
+               NFGS_ERR_ARG, "advance: poloidal_mode must be NFGS_POLOIDAL_DIRECT or _TWO_STAGE");
   NFGS_REQUIRE(ws->jp && ws->s_bcz && ws->s_jz && ws->hat0 && ws->hat1 && ws->cscr &&
                    ws->partials && ws->metrics,
                NFGS_ERR_ARG, "advance: workspace buffer missing");
@@ -46,7 +48,28 @@ extern "C" int nfgs_advance(const nfgs_plan* p, const nfgs_boundary* bnd,
                             ny, nz, s));
   NFGS_TRY(launch_transform(p, &p->dct_y, NFGS_FORWARD, ws->hat0, nx, 1, pl, ws->hat1, nx, 1, pl,
                             nx, nz, s));
-  NFGS_TRY(nfgs_robin_solve(p, bnd, ws->hat1, ws->hat0, ws->cscr, ws->partials, stream));
+  if (poloidal_mode == NFGS_POLOIDAL_DIRECT) {
+    NFGS_TRY(nfgs_robin_solve(p, bnd, ws->hat1, ws->hat0, ws->cscr, ws->partials, stream));
+  } else {
+    // Two-stage (nfgs_solver.py:5674-5725): B_c,z from the Robin solve with
+    // the undivided source and lambda-scaled bottom data (two planes of
+    // s_bcz, free once the forward transform has read it), back to space and
+    // forward again, then Phi_c_hat = B_c,z_hat / lambda on planes 1..nz-1.
+    NFGS_REQUIRE(bnd->phi_bottom_hat && (p->d.rho == 0.0 || bnd->q_hat), NFGS_ERR_STATE,
+                 "advance: boundary tables are not set");
+    NFGS_TRY(two_stage_tables(p, bnd, ws->s_bcz, s));
+    NFGS_TRY(robin_modal(p, ws->s_bcz, at(p, ws->s_bcz, pl), ws->hat1, ws->hat0, ws->cscr,
+                         ws->partials, false, s));
+    NFGS_TRY(launch_transform(p, &p->dct_x, NFGS_INVERSE, ws->hat0, 1, nx, pl, ws->hat1, 1, nx, pl,
+                              ny, nz, s));
+    NFGS_TRY(launch_transform(p, &p->dct_y, NFGS_INVERSE, ws->hat1, nx, 1, pl, ws->hat0, nx, 1,
+                              pl, nx, nz, s));
+    NFGS_TRY(launch_transform(p, &p->dct_x, NFGS_FORWARD, ws->hat0, 1, nx, pl, ws->hat1, 1, nx, pl,
+                              ny, nz, s));
+    NFGS_TRY(launch_transform(p, &p->dct_y, NFGS_FORWARD, ws->hat1, nx, 1, pl, ws->hat0, nx, 1,
+                              pl, nx, nz, s));
+    NFGS_TRY(divide_by_lambda(p, at(p, ws->hat0, pl), nz - 1, s));
+  }
   NFGS_TRY(launch_transform(p, &p->dct_x, NFGS_INVERSE, at(p, ws->hat0, pl), 1, nx, pl,
                             at(p, ws->hat1, pl), 1, nx, pl, ny, nz - 1, s));
   NFGS_TRY(launch_transform(p, &p->dct_y, NFGS_INVERSE, at(p, ws->hat1, pl), nx, 1, pl,
@@ -68,4 +91,11 @@ extern "C" int nfgs_advance(const nfgs_plan* p, const nfgs_boundary* bnd,
   NFGS_TRY(nfgs_state_pass(p, bnd, phi_out, az_out, b_out, ws->jp, nullptr, ws->partials, stream));
   NFGS_TRY(nfgs_reduce(p, ws->partials, ws->metrics, stream));
   return NFGS_OK;
+}
+
+extern "C" int nfgs_advance(const nfgs_plan* p, const nfgs_boundary* bnd,
+                            const nfgs_workspace* ws, void* phi_out, void* az_out, void* b_out,
+                            int zero_source, void* stream) {
+  return nfgs_advance_ex(p, bnd, ws, phi_out, az_out, b_out, zero_source, NFGS_POLOIDAL_DIRECT,
+                         stream);
 }

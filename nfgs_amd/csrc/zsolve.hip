@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void robin_sweep_kernel(
     const T* __restrict__ src, T* __restrict__ out, T* __restrict__ cs, const T* __restrict__ lam,
     const T* __restrict__ kk, const T* __restrict__ bh, const T* __restrict__ qh, int nz,
     long plane, T inv_dz2, T two_inv_dz2, T dz, T dz2, T two_dz, T rho, T one_m_rho, T kb,
-    T fu_rho, T rho_over_dz, double* zm_partial, double norm) {
+    T fu_rho, T rho_over_dz, double* zm_partial, double norm, bool div_src) {
   const long m = (long)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (m >= plane) return;
   if (m == 0) {
@@ -103,12 +103,13 @@ __global__ __launch_bounds__(256) void robin_sweep_kernel(
     return;
   }
   const T l = lam[m];
+  const T ls = div_src ? l : T(1);  // the source is S / lambda, or S itself
   const T lo = inv_dz2, up = inv_dz2;
   const T diag = l - two_inv_dz2;
   const T b = bh[m];
   T cp, dp;
   {
-    const T s1 = src[plane + m] / l;
+    const T s1 = src[plane + m] / ls;
     T fd, fu, r0;
     if (rho > T(0)) {
       fd = one_m_rho * l - two_inv_dz2;
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(256) void robin_sweep_kernel(
   }
   const int nu = nz - 1;
   forward_rows(src, plane, m, 1, nu - 2, [&](int r, T sv) {
-    const T s = sv / l;
+    const T s = sv / ls;
     const T den = diag - lo * cp;
     cp = up / den;
     dp = (s - lo * dp) / den;
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(256) void robin_sweep_kernel(
   });
   // Top Robin row with u_{N-2} eliminated through the PDE at plane N-1.
   {
-    const T s = src[(long)(nz - 2) * plane + m] / l;
+    const T s = src[(long)(nz - 2) * plane + m] / ls;
     const T rl = (-l) * dz2 - T(2);
     const T rd = T(2) + two_dz * kk[m];
     const T rr = T(0) - s * dz2;
@@ -371,7 +372,7 @@ __global__ __launch_bounds__(64 * kSegMax) void robin_seg_kernel(
     const T* __restrict__ src, T* __restrict__ out, const T* __restrict__ lam,
     const T* __restrict__ kk, const T* __restrict__ bh, const T* __restrict__ qh, int nz,
     long plane, T inv_dz2, T two_inv_dz2, T dz, T dz2, T two_dz, T rho, T one_m_rho, T kb,
-    T fu_rho, T rho_over_dz, double* zm_partial, double norm) {
+    T fu_rho, T rho_over_dz, double* zm_partial, double norm, bool div_src) {
   __shared__ SegLds<T> L;
   // the segment index is uniform per wavefront: keep it (and the row
   // addresses) scalar
@@ -419,7 +420,7 @@ __global__ __launch_bounds__(64 * kSegMax) void robin_seg_kernel(
     L.zm[k] = acc;
   }
   const T l = ld(lam, mo);
-  const T il = T(1) / l;
+  const T il = div_src ? T(1) / l : T(1);  // the source is S / lambda, or S itself
   const T b0 = ld(bh, mo);
   SegRows<T> R;
   R.inv = inv_dz2;
@@ -513,8 +514,8 @@ __global__ __launch_bounds__(64 * kSegMax) void dirichlet_seg_kernel(
 }
 
 template <typename T>
-int robin_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* src, void* out, void* scr,
-            double* partials, hipStream_t s) {
+int robin_t(const nfgs_plan* p, const void* bh, const void* qh, const void* src, void* out,
+            void* scr, double* partials, bool div_src, hipStream_t s) {
   const nfgs_grid_desc& d = p->d;
   const double dz = d.dz, dz2 = dz * dz, rho = d.rho;
   const double norm = 4.0 * (d.nx - 1) * (d.ny - 1);
@@ -525,11 +526,11 @@ int robin_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* src, void*
     hipLaunchKernelGGL((robin_seg_kernel<T, SEG>), dim3((unsigned)((p->plane + 63) / 64)),
                        dim3(64 * ns), 0, s, static_cast<const T*>(src), static_cast<T*>(out),
                        static_cast<const T*>(p->lam), static_cast<const T*>(p->kk),
-                       static_cast<const T*>(bnd->phi_bottom_hat),
-                       static_cast<const T*>(bnd->q_hat), d.nz, p->plane, T(1.0 / dz2),
+                       static_cast<const T*>(bh), static_cast<const T*>(qh), d.nz, p->plane,
+                       T(1.0 / dz2),
                        T(2.0 / dz2), T(dz), T(dz2), T(2.0 * dz), T(rho), T(1.0 - rho),
                        T((1.0 + 0.5 * rho) / dz2), T((1.0 - 0.5 * rho) / dz2), T(rho / dz), zm,
-                       norm);
+                       norm, div_src);
     NFGS_LAUNCH_CHECK();
     return NFGS_OK;
   }
@@ -537,10 +538,10 @@ int robin_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* src, void*
   hipLaunchKernelGGL((robin_sweep_kernel<T>), dim3(nb), dim3(256), 0, s,
                      static_cast<const T*>(src), static_cast<T*>(out), static_cast<T*>(scr),
                      static_cast<const T*>(p->lam), static_cast<const T*>(p->kk),
-                     static_cast<const T*>(bnd->phi_bottom_hat), static_cast<const T*>(bnd->q_hat),
-                     d.nz, p->plane, T(1.0 / dz2), T(2.0 / dz2), T(dz), T(dz2), T(2.0 * dz),
-                     T(rho), T(1.0 - rho), T((1.0 + 0.5 * rho) / dz2), T((1.0 - 0.5 * rho) / dz2),
-                     T(rho / dz), zm, norm);
+                     static_cast<const T*>(bh), static_cast<const T*>(qh), d.nz, p->plane,
+                     T(1.0 / dz2), T(2.0 / dz2), T(dz), T(dz2), T(2.0 * dz), T(rho), T(1.0 - rho),
+                     T((1.0 + 0.5 * rho) / dz2), T((1.0 - 0.5 * rho) / dz2), T(rho / dz), zm,
+                     norm, div_src);
   NFGS_LAUNCH_CHECK();
   return NFGS_OK;
 }
@@ -569,6 +570,13 @@ int dirichlet_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* src, v
 }
 
 }  // namespace
+
+int robin_modal(const nfgs_plan* p, const void* bh, const void* qh, const void* src, void* out,
+                void* scr, double* partials, bool div_src, hipStream_t s) {
+  if (p->d.dtype == NFGS_F32) return robin_t<float>(p, bh, qh, src, out, scr, partials, div_src, s);
+  return robin_t<double>(p, bh, qh, src, out, scr, partials, div_src, s);
+}
+
 }  // namespace nfgs
 
 extern "C" int nfgs_robin_solve(const nfgs_plan* p, const nfgs_boundary* bnd, const void* src_hat,
@@ -579,11 +587,8 @@ extern "C" int nfgs_robin_solve(const nfgs_plan* p, const nfgs_boundary* bnd, co
                "robin_solve: boundary tables are not set");
   NFGS_REQUIRE(src_hat != out_hat && out_hat != scratch, NFGS_ERR_ARG,
                "robin_solve: buffers must be distinct");
-  if (p->d.dtype == NFGS_F32)
-    return nfgs::robin_t<float>(p, bnd, src_hat, out_hat, scratch, partials,
-                                nfgs::as_stream(stream));
-  return nfgs::robin_t<double>(p, bnd, src_hat, out_hat, scratch, partials,
-                               nfgs::as_stream(stream));
+  return nfgs::robin_modal(p, bnd->phi_bottom_hat, bnd->q_hat, src_hat, out_hat, scratch, partials,
+                           true, nfgs::as_stream(stream));
 }
 
 extern "C" int nfgs_dirichlet_solve(const nfgs_plan* p, const nfgs_boundary* bnd,

@@ -12,7 +12,7 @@ Supported profile (the reference defaults and production settings):
 {'uniform_open', 'require_balanced'}, ``horizontal_zero_mode_policy`` in
 {'frozen_linear_harmonic', 'omit'}, ``gaussian_horizontal_operator=
 'spectral'``, ``vertical_current_operator='staggered_face'``,
-``poloidal_update_mode='direct'``, projection weight 0, any
+``poloidal_update_mode`` in {'direct', 'two_stage'}, projection weight 0, any
 ``bottom_divergence_robin_weight`` in [0, 1], ``init_mode='gs_cfcf'``,
 ``mu=1``, ``polarity_mask='both'``, ``stop_mode='sigma_best'``.  Other
 values the reference accepts raise ``NotImplementedError``; values it
@@ -205,8 +205,6 @@ class _Options:
             _unsupported("gaussian_horizontal_operator='fd'")
         if self.vertical_current_operator != "staggered_face":
             _unsupported("vertical_current_operator='collocated_centered'")
-        if self.poloidal_update_mode != "direct":
-            _unsupported("poloidal_update_mode='two_stage'")
         if self.projection_weight != 0.0:
             _unsupported("bottom_divergence_projection_weight > 0")
         if self.bottom_parallel_current_mode != "observed_full":
@@ -625,10 +623,12 @@ class NFGSSolverGPU:
         """One iteration into `slot`; B of the new state lands in _Bwork only
         when want_b (the iteration itself never needs B in memory)."""
         phi, az = self._slots[slot]
-        L.check(self._lib.nfgs_advance(self._plan, L.C.byref(self._bnd), L.C.byref(self._ws),
-                                       L.ptr(phi), L.ptr(az),
-                                       L.ptr(self._Bwork) if want_b else None, int(zero_source),
-                                       self._stream), "iteration")
+        mode = L.POLOIDAL_TWO_STAGE if self.opts.poloidal_update_mode == "two_stage" \
+            else L.POLOIDAL_DIRECT
+        L.check(self._lib.nfgs_advance_ex(self._plan, L.C.byref(self._bnd), L.C.byref(self._ws),
+                                          L.ptr(phi), L.ptr(az),
+                                          L.ptr(self._Bwork) if want_b else None,
+                                          int(zero_source), mode, self._stream), "iteration")
 
     def _read_metrics(self):
         self._metrics_host.copy_(self._metrics, non_blocking=True)

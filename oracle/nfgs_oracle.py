@@ -415,7 +415,10 @@ class OracleNFGS:
                  calculation_domain_mode="mudpack", weak_field_floor_b2=1e-20,
                  alpha_limit_factor=1.0, source_zero_mode_policy="error",
                  source_zero_mode_tolerance=1e-2, d0_zero_mode_policy="error",
-                 mean_mode_tolerance=1e-6):
+                 mean_mode_tolerance=1e-6, poloidal_update_mode="direct"):
+        if poloidal_update_mode not in ("direct", "two_stage"):
+            raise ValueError("poloidal_update_mode must be 'direct' or 'two_stage'")
+        self.poloidal_update_mode = poloidal_update_mode
         self.input_shape_zyx = (int(Nz), int(Ny), int(Nx))
         cx, cy, cz, pad_xy = calculation_grid(int(Nx), int(Ny), int(Nz), pad_calculation_domain,
                                               calculation_domain_mode)
@@ -697,6 +700,36 @@ class OracleNFGS:
         return ratio
 
     def solve_phi_c(self, src):
+        """The poloidal update dispatch (nfgs_solver.py:5727-5733)."""
+        if self.poloidal_update_mode == "two_stage":
+            return self.solve_phi_c_two_stage(src)
+        return self.solve_phi_c_direct(src)
+
+    def solve_phi_c_two_stage(self, src):
+        """B_c,z from the modal Robin solve with the source undivided and the
+        bottom data scaled by lambda, then Phi_c_hat = DCT2(B_c,z) / lambda
+        plane by plane (nfgs_solver.py:5674-5725)."""
+        lam = self.lam_neumann()
+        ls = lam.copy()
+        ls[0, 0] = 1.0
+        self.diagnose_zero_mode(src)
+        sh = dct2(src)
+        sh[:, 0, 0] = 0.0
+        bh = dct2(self.Phi_above_bottom)
+        bh[0, 0] = 0.0
+        qh = None
+        if self.rho > 0.0:
+            qh = dct2(self.dphi_target)
+            qh[0, 0] = 0.0
+            qh = lam * qh
+        bcz = idct2(modal_robin_from_hat(sh, lam * bh, lam, self.k, self.dz, self.rho, qh))
+        ph = dct2(bcz) / ls[None]
+        ph[:, 0, 0] = 0.0
+        phi = idct2(ph)
+        phi[0] = self.Phi_above_bottom
+        return phi
+
+    def solve_phi_c_direct(self, src):
         lam = self.lam_neumann()
         ls = lam.copy()
         ls[0, 0] = 1.0

@@ -183,7 +183,7 @@ def test_iterate_matches_reference_trajectory(path):
 def test_unsupported_options_raise():
     from nfgs_amd import NFGSSolverGPU
     with pytest.raises(NotImplementedError):
-        NFGSSolverGPU(12, 10, 9, 1, 1, 1, poloidal_update_mode="two_stage")
+        NFGSSolverGPU(12, 10, 9, 1, 1, 1, bottom_divergence_projection_weight=0.5)
     with pytest.raises(ValueError):
         NFGSSolverGPU(12, 10, 9, 1, 1, 1, bottom_divergence_robin_weight=1.5)
     with pytest.raises(ValueError):
