@@ -244,6 +244,32 @@ __device__ __forceinline__ T lapr(const T* p, int y, int x, int ny, int nx, long
   return ax + ay;
 }
 
+// lapr with the centred form chosen per axis: IX (IY) when no point the pass
+// touches lies on an x (y) end of the grid.
+template <bool IX, bool IY, typename T>
+__device__ __forceinline__ T lapr2(const T* p, int y, int x, int ny, int nx, long ld, T idx2,
+                                   T idy2) {
+  if (IX && IY) return lapr<true>(p, y, x, ny, nx, ld, idx2, idy2);
+  if (!IX && !IY) return lapr<false>(p, y, x, ny, nx, ld, idx2, idy2);
+  const T* c = p + (long)y * ld + x;
+  T ax, ay;
+  if (IX) {
+    ax = (c[1] - T(2) * c[0] + c[-1]) * idx2;
+  } else {
+    const T wxp = x == 0 ? T(2) : (x == nx - 1 ? T(0) : T(1));
+    const T wxm = x == nx - 1 ? T(2) : (x == 0 ? T(0) : T(1));
+    ax = (wxp * c[1] - T(2) * c[0] + wxm * c[-1]) * idx2;
+  }
+  if (IY) {
+    ay = (c[ld] - T(2) * c[0] + c[-ld]) * idy2;
+  } else {
+    const T wyp = y == 0 ? T(2) : (y == ny - 1 ? T(0) : T(1));
+    const T wym = y == ny - 1 ? T(2) : (y == 0 ? T(0) : T(1));
+    ay = (wyp * c[ld] - T(2) * c[0] + wym * c[-ld]) * idy2;
+  }
+  return ax + ay;
+}
+
 template <typename T>
 __device__ __forceinline__ T lds_dx(const T* s, const Tile& t, const Grid<T>& g) {
   return fdr<false>(s + t.c, 1, t.x, g.nx, g.i2dx);
@@ -602,7 +628,7 @@ struct StateLds {
   double red[NFGS_NQ * 16];
 };
 
-template <typename T, bool INNER, bool ITER>
+template <typename T, bool IX, bool IY, bool ITER>
 __device__ __forceinline__ void state_pass_body(
     StateLds<T>& L, const Tile& t, const Grid<T>& g, const T* __restrict__ phi,
     const T* __restrict__ az, const T* __restrict__ bg, const T* __restrict__ bz_obs,
@@ -618,7 +644,7 @@ __device__ __forceinline__ void state_pass_body(
   const bool has_pa = ITER || partials != nullptr;
   auto pl = [&](auto* base, int k) { return base + (long)k * g.plane; };
   auto lo_of = [&](int k) { return max(0, min(k - 1, g.nz - 3)); };
-  const bool edge = !INNER && (t.y == 0 || t.y == g.ny - 1 || t.x == 0 || t.x == g.nx - 1);
+  const bool edge = (!IY && (t.y == 0 || t.y == g.ny - 1)) || (!IX && (t.x == 0 || t.x == g.nx - 1));
   const bool in_win = t.y >= w.y0 && t.y < w.y1 && t.x >= w.x0 && t.x < w.x1;
   const T jz0 = ld(jzt, h.own);
 
@@ -687,16 +713,16 @@ __device__ __forceinline__ void state_pass_body(
         const int r = threadIdx.x + 256 * i;
         if (r < R1N) {
           // narrow tiles: column-major, so the in-grid ring points share a wave
-          const bool cm = !INNER && t.narrow;
+          const bool cm = !IX && t.narrow;
           const int ly = 1 + (cm ? r % R1H : r / R1W), lx = 1 + (cm ? r / R1H : r % R1W);
           const int e = ly * PW + lx;
           const int y = t.y0 + ly - HH, x = t.x0 + lx - HH;
           T vbx = T(0), vby = T(0), vcz = T(0);
-          if (INNER || (x >= 0 && x < g.nx && y >= 0 && y < g.ny)) {
-            const T dxd = fdr<INNER>(L.sdz + e, 1, x, g.nx, g.i2dx);
-            const T dyd = fdr<INNER>(L.sdz + e, (long)PW, y, g.ny, g.i2dy);
-            const T dxa = fdr<INNER>(L.saz[sl] + e, 1, x, g.nx, g.i2dx);
-            const T dya = fdr<INNER>(L.saz[sl] + e, (long)PW, y, g.ny, g.i2dy);
+          if ((IX || (x >= 0 && x < g.nx)) && (IY || (y >= 0 && y < g.ny))) {
+            const T dxd = fdr<IX>(L.sdz + e, 1, x, g.nx, g.i2dx);
+            const T dyd = fdr<IY>(L.sdz + e, (long)PW, y, g.ny, g.i2dy);
+            const T dxa = fdr<IX>(L.saz[sl] + e, 1, x, g.nx, g.i2dx);
+            const T dya = fdr<IY>(L.saz[sl] + e, (long)PW, y, g.ny, g.i2dy);
             const T bcx = -dxd + dya;
             const T bcy = -dyd - dxa;
             vbx = (L.sgx[sl][e] + bcx) + hx;
@@ -705,7 +731,7 @@ __device__ __forceinline__ void state_pass_body(
               const unsigned o = (unsigned)((y * g.nx + x) * (int)sizeof(T));
               vcz = ld(bz_obs, o) - ld(bg + 2 * g.N, o);
             } else {
-              vcz = lapr<INNER>(L.sphi[r0] + e - (y * PW + x), y, x, g.ny, g.nx, (long)PW, g.idx2,
+              vcz = lapr2<IX, IY>(L.sphi[r0] + e - (y * PW + x), y, x, g.ny, g.nx, (long)PW, g.idx2,
                               g.idy2);
             }
           }
@@ -719,7 +745,7 @@ __device__ __forceinline__ void state_pass_body(
     if (!act) return;
     // (C) face pass at the own point
     T cb[3] = {0, 0, 0}, cg[2] = {0, 0}, cjz = 0;
-    if ((INNER || t.valid)) {
+    if (((IX && IY) || t.valid)) {
       cb[0] = L.sbx[sl][t.c];
       cb[1] = L.sby[sl][t.c];
       cb[2] = p == 0 ? ld(bz_obs, h.own) : L.sgz[sl][threadIdx.x] + L.scz[sl][t.c];
@@ -730,7 +756,7 @@ __device__ __forceinline__ void state_pass_body(
       else if (p == g.nz - 1 || edge)
         cjz = T(0);
       else
-        cjz = -lapr<INNER>(L.saz[sl] + t.c - (t.y * PW + t.x), t.y, t.x, g.ny, g.nx, (long)PW,
+        cjz = -lapr2<IX, IY>(L.saz[sl] + t.c - (t.y * PW + t.x), t.y, t.x, g.ny, g.nx, (long)PW,
                          g.idx2, g.idy2);
       if (bout && p >= k0 && p < k1) {
         st(pl(bout, p), h.own, cb[0]);
@@ -738,7 +764,7 @@ __device__ __forceinline__ void state_pass_body(
         st(pl(bout + 2 * g.N, p), h.own, cb[2]);
       }
     }
-    if (p > ps && (INNER || t.valid)) {
+    if (p > ps && ((IX && IY) || t.valid)) {
       const int f = p - 1;  // face between planes f and p
       FaceVal<T> F;
       F.b[0] = T(0.5) * (cb[0] + pb[0]);
@@ -749,8 +775,8 @@ __device__ __forceinline__ void state_pass_body(
       const T* s1 = L.scz[sl];
       const T* s0 = L.scz[f & 1];
       auto cfz = [&](int o) { return T(0.5) * (s1[t.c + o] + s0[t.c + o]); };
-      const T dyc = fdgr<T, INNER>([&](int o) { return cfz(o * PW); }, t.y, g.ny, g.i2dy);
-      const T dxc = fdgr<T, INNER>([&](int o) { return cfz(o); }, t.x, g.nx, g.i2dx);
+      const T dyc = fdgr<T, IY>([&](int o) { return cfz(o * PW); }, t.y, g.ny, g.i2dy);
+      const T dxc = fdgr<T, IX>([&](int o) { return cfz(o); }, t.x, g.nx, g.i2dx);
       F.j[0] = dyc - dby;
       F.j[1] = dbx - dxc;
       F.j[2] = T(0.5) * (cjz + pjz);
@@ -803,13 +829,13 @@ __device__ __forceinline__ void state_pass_body(
             acc[0] += mdiv(msqrt(c0 * c0 + c1 * c1 + c2 * c2), bn);
             acc[1] += jn;
           }
-          const T dvx = fdr<INNER>(L.sbx[k & 1] + t.c, 1, t.x, g.nx, g.i2dx);
-          const T dvy = fdr<INNER>(L.sby[k & 1] + t.c, (long)PW, t.y, g.ny, g.i2dy);
+          const T dvx = fdr<IX>(L.sbx[k & 1] + t.c, 1, t.x, g.nx, g.i2dx);
+          const T dvy = fdr<IY>(L.sby[k & 1] + t.c, (long)PW, t.y, g.ny, g.i2dy);
           T dvz;
           if (k == 0) {
             // Bz of plane 2 = B_bg,z + lap_xy(Phi_c) there (Phi ring holds 0..2)
             const T bz2 = ld(pl(bg + 2 * g.N, 2), h.own) +
-                          lapr<INNER>(L.sphi[2] + t.c - (t.y * PW + t.x), t.y, t.x, g.ny, g.nx,
+                          lapr2<IX, IY>(L.sphi[2] + t.c - (t.y * PW + t.x), t.y, t.x, g.ny, g.nx,
                                     (long)PW, g.idx2, g.idy2);
             dvz = (T(-3) * pb[2] + T(4) * cb[2] - bz2) * g.i2dz;
           } else {
@@ -835,7 +861,7 @@ __device__ __forceinline__ void state_pass_body(
   for (int p = ps; p <= pe; ++p) step(sa, p);
   // top node: no face above it (planes kt, kt-1, kt-2 of Bz are pb, pz2, pz3)
   const int kt = g.nz - 1;
-  if (k1 == g.nz && (INNER || t.valid)) {
+  if (k1 == g.nz && ((IX && IY) || t.valid)) {
     if (has_jp) {
       st(pl(jp, kt), h.own, fprev.p[0]);
       st(pl(jp + g.N, kt), h.own, fprev.p[1]);
@@ -848,8 +874,8 @@ __device__ __forceinline__ void state_pass_body(
       st(pl(jout + 2 * g.N, kt), h.own, T(0));
     }
     if (has_pa) {
-      const T dvx = fdr<INNER>(L.sbx[kt & 1] + t.c, 1, t.x, g.nx, g.i2dx);
-      const T dvy = fdr<INNER>(L.sby[kt & 1] + t.c, (long)PW, t.y, g.ny, g.i2dy);
+      const T dvx = fdr<IX>(L.sbx[kt & 1] + t.c, 1, t.x, g.nx, g.i2dx);
+      const T dvy = fdr<IY>(L.sby[kt & 1] + t.c, (long)PW, t.y, g.ny, g.i2dy);
       const T dvz = (T(3) * pb[2] - T(4) * pz2 + pz3) * g.i2dz;
       const T div = dvx + dvy + dvz;
       const T b2 = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
@@ -883,13 +909,22 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
     long pstride, T floor_b2, T amax, Window w) {
   __shared__ StateLds<T> L;
   const Tile t = make_tile(g);
-  const bool inner = t.x0 >= 2 && t.x0 + TX <= g.nx - 2 && t.y0 >= 2 && t.y0 + TY <= g.ny - 2;
-  if (inner)
-    state_pass_body<T, true, ITER>(L, t, g, phi, az, bg, bz_obs, hprof, jzt, bout, jp, jout,
-                                   partials, pstride, floor_b2, amax, w);
+  // centred stencils along an axis when no point the pass touches lies on
+  // that axis' ends; the one-sided selects only where a tile meets an end
+  const bool ix = t.x0 >= 2 && t.x0 + TX <= g.nx - 2;
+  const bool iy = t.y0 >= 2 && t.y0 + TY <= g.ny - 2;
+#define NFGS_STATE_BODY(IX_, IY_)                                                            \
+  state_pass_body<T, IX_, IY_, ITER>(L, t, g, phi, az, bg, bz_obs, hprof, jzt, bout, jp, jout, \
+                                     partials, pstride, floor_b2, amax, w)
+  if (ix && iy)
+    NFGS_STATE_BODY(true, true);
+  else if (ix)
+    NFGS_STATE_BODY(true, false);
+  else if (iy)
+    NFGS_STATE_BODY(false, true);
   else
-    state_pass_body<T, false, ITER>(L, t, g, phi, az, bg, bz_obs, hprof, jzt, bout, jp, jout,
-                                    partials, pstride, floor_b2, amax, w);
+    NFGS_STATE_BODY(false, false);
+#undef NFGS_STATE_BODY
 }
 
 // Register set of the loads one node of the sources needs.
