@@ -219,7 +219,7 @@ def cpu_baseline(n, rho, planes_np, iters, gpu_iters):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus "
+        print(f"bench.py measures one GPU only (the library has no multi-GPU path); got --gpus "
               f"{a.gpus}", file=sys.stderr)
         sys.exit(2)
     import numpy as np

@@ -73,7 +73,7 @@ def main(argv=None) -> int:
         p.error("--rho must be in [0,1]")
     device = "cuda" if a.device == "auto" else a.device
     if not str(device).startswith("cuda"):
-        p.error("this build solves on a ROCm GPU only (--device cuda or auto)")
+        p.error("nfgs_amd solves on a ROCm GPU only (--device cuda or auto)")
     bx, by, bz, nz, spacing = read_boundary(a.boundary)
     solver, history = reconstruct(bx, by, bz, nz, spacing, rho=a.rho, max_iter=a.max_iter,
                                   device=device)
