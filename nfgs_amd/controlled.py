@@ -118,8 +118,7 @@ def evaluate(reference_path, field_path, boundary_path, config_path) -> dict:
     spec, box = _scoring_box(config, ref["Bx"].shape)
     spacing = tuple(float(np.diff(ref[k])[0]) for k in ("x", "y", "z"))
 
-    j = scores.curl(field, spacing)
-    div = scores.divergence(field, spacing)
+    j, div = scores.curl_and_divergence(field, spacing)
     # 6 / h_eff = 2 (1/dx + 1/dy + 1/dz), the isotropic convention
     h_eff = 3.0 / sum(1.0 / h for h in spacing)
     metrics = scores.compute_metrics(ref, field, j, div, box, h_eff)

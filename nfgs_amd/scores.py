@@ -106,6 +106,13 @@ def divergence(fields, spacing):
     return d.cpu().numpy()
 
 
+def curl_and_divergence(fields, spacing):
+    """curl and divergence of one field in a single pass (NumPy out)."""
+    j, d = _curl_div(_vec([fields[k] for k in _B]), spacing, want_j=True, want_div=True)
+    jh = j.cpu().numpy()
+    return {k: jh[i] for i, k in enumerate(_J)}, d.cpu().numpy()
+
+
 def sigma_j(fields, spacing, slices) -> float:
     """sigma_J of the common curl on a box (common_fd2.py:64-83)."""
     B = _vec([fields[k] for k in _B])
