@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 error reference")
     p.add_argument("--no-warm", action="store_true", help="skip the warm-start solves")
+    p.add_argument("--no-two-stage", action="store_true",
+                   help="skip the two-stage poloidal update solve")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run a 256 MiB device copy for counter calibration")
     return p.parse_args()
@@ -69,11 +71,14 @@ ITER_KW = dict(init_mode="gs_cfcf", tol=1e-5, mu=1, patience=40, sigma_improveme
                stop_mode="sigma_best", verbose=False)
 
 
-def solve(torch, NFGSSolverGPU, planes, n, rho, dtype, max_iter, start=None):
+def solve(torch, NFGSSolverGPU, planes, n, rho, dtype, max_iter, start=None, extra=None):
     """One timed solve with the boundary already resident on the device.
-    start=(B, Az): init_mode="field" from that state instead of gs_cfcf."""
+    start=(B, Az): init_mode="field" from that state instead of gs_cfcf;
+    extra: more constructor keywords."""
     bx, by, bz = (p.to(dtype) for p in planes)
-    s = NFGSSolverGPU(n, n, n, 1.0, 1.0, 1.0, device="cuda", dtype=dtype, **solver_kwargs(rho))
+    kw = solver_kwargs(rho)
+    kw.update(extra or {})
+    s = NFGSSolverGPU(n, n, n, 1.0, 1.0, 1.0, device="cuda", dtype=dtype, **kw)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     s.set_bottom_vector_boundary(bx, by, bz, zero_net_current=False, smooth_passes=0, ijzbcor=3)
@@ -346,6 +351,25 @@ def main():
             f"B diff {bdiff:.3e}")
         del sp, sw, B0, Az0
 
+    # the two-stage poloidal update (the reference's conditioning ablation) on
+    # the same boundary: its time to converge and its distance from the direct
+    # solve's fields
+    two = None
+    if not a.no_two_stage:
+        log("two-stage poloidal update, fp32 rho=0.5")
+        s2, h2, _, t2 = solve(torch, NFGSSolverGPU, planes, n, 0.5, torch.float32, a.max_iter,
+                              extra=dict(poloidal_update_mode="two_stage"))
+        f2, fd = s2.fields_to_numpy(), s.fields_to_numpy()
+        bd = float(np.sqrt(sum(np.sum((f2[k].astype(np.float64) - fd[k]) ** 2)
+                               for k in ("Bx", "By", "Bz")) /
+                           sum(np.sum(fd[k].astype(np.float64) ** 2) for k in ("Bx", "By", "Bz"))))
+        two = dict(sec_to_converge=round(t2, 4), iterations=h2["final_iter"],
+                   best_iter=h2["best_iter"], best_sigma=h2["best_sigma"],
+                   stop_reason=h2["stop_reason"], B_rel_diff_vs_direct=bd)
+        log(f"  two-stage {t2:.3f} s ({h2['final_iter']} it, best {h2['best_sigma']:.5e}); "
+            f"B diff vs direct {bd:.3e}")
+        del s2
+
     # error of the FP32 solve against an FP64 solve of the same boundary
     err = None
     if not a.no_fp64:
@@ -423,6 +447,7 @@ def main():
         "stencil_roofline": stencil_roof,
         "kernels": kernels,
         "warm_start": warm,
+        "two_stage": two,
         "err_vs_fp64": err,
         "cpu_baseline": cpu,
     }
