@@ -348,6 +348,24 @@ int nfgs_field_scores(const double* b_ref, const double* b, const double* j, con
                       int nz, int ny, int nx, int z0, int z1, int y0, int y1, int x0, int x1,
                       double b2min, double* workspace, double* out, void* stream);
 
+/* Relative error of an outward bottom field against a target, for the CCI
+ * comparison.  b (target) and c (candidate) are three planes (x, y, z
+ * components) of ny*nx FP64.  Over the window [y0,y1) x [x0,x1):
+ *   out[0] = sum |c-b|^2, out[1] = sum |b|^2, out[2] = max |b|,
+ *   out[3] = number of pixels in the window;
+ * and for each of the ncut (<= 16) threshold fractions frac[k] (host array),
+ * with t_k = frac[k] * (max|b| + eps):
+ *   out[4+2k] = number of pixels with |b| > t_k,
+ *   out[5+2k] = sum over those pixels of |c-b| / |b|.
+ * out holds 4 + 2*ncut device doubles.  Replaces the host arithmetic of
+ * cci_errors (nfgs/controlled.py:55-68; thresholds frac*max|b|, eps = 0) and
+ * of bottom_bgt_relative_error
+ * (nfgs/_backend/scripts/plot_nfgs_td_robin_metrics_vs_er.py:299-326;
+ * floor_frac*(max|b| + 1e-30)). */
+int nfgs_outward_errors(const double* b, const double* c, int ny, int nx, int y0, int y1, int x0,
+                        int x1, const double* frac, int ncut, double eps, double* out,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -152,3 +152,115 @@ extern "C" int nfgs_field_scores(const double* b_ref, const double* b, const dou
   NFGS_LAUNCH_CHECK();
   return NFGS_OK;
 }
+
+// ---- relative error of an outward bottom field (the CCI comparison) -------
+//
+// One workgroup walks the window twice: the first pass sums |c-b|^2 and
+// |b|^2 and finds max|b|; the second counts, per threshold
+// t_k = frac_k * (max|b| + eps), the pixels with |b| > t_k and sums
+// |c-b| / |b| over them.  Fixed-order tree sums make the result independent
+// of scheduling.
+
+namespace nfgs {
+namespace {
+
+constexpr int kMaxCuts = 16;
+constexpr int kOutwardThreads = 1024;
+
+struct CutList {
+  double frac[kMaxCuts];
+  int n;
+};
+
+// sum (or max) of one value per thread over the workgroup, valid in every
+// thread afterwards; `lds` holds kOutwardThreads doubles
+__device__ double tree_reduce(double v, double* lds, bool take_max) {
+  const int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (int w = kOutwardThreads / 2; w > 0; w >>= 1) {
+    if (t < w) lds[t] = take_max ? fmax(lds[t], lds[t + w]) : lds[t] + lds[t + w];
+    __syncthreads();
+  }
+  const double r = lds[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kOutwardThreads) void outward_errors_kernel(
+    const double* __restrict__ b, const double* __restrict__ c, int ny, int nx, int y0, int x0,
+    int wy, int wx, CutList cuts, double eps, double* __restrict__ out) {
+  __shared__ double lds[kOutwardThreads];
+  const long pl = (long)ny * nx;
+  const int M = wy * wx;
+  double d2 = 0.0, b2 = 0.0, bmax = 0.0;
+  for (int k = threadIdx.x; k < M; k += kOutwardThreads) {
+    const int y = k / wx, x = k - y * wx;
+    const long i = (long)(y0 + y) * nx + (x0 + x);
+    const double t0 = b[i], t1 = b[pl + i], t2 = b[2 * pl + i];
+    const double e0 = c[i] - t0, e1 = c[pl + i] - t1, e2 = c[2 * pl + i] - t2;
+    const double m2 = t0 * t0 + t1 * t1 + t2 * t2;
+    d2 += e0 * e0 + e1 * e1 + e2 * e2;
+    b2 += m2;
+    bmax = fmax(bmax, sqrt(m2));
+  }
+  d2 = tree_reduce(d2, lds, false);
+  b2 = tree_reduce(b2, lds, false);
+  bmax = tree_reduce(bmax, lds, true);
+  double cnt[kMaxCuts], rel[kMaxCuts], thr[kMaxCuts];
+  for (int q = 0; q < kMaxCuts; ++q) {
+    cnt[q] = 0.0;
+    rel[q] = 0.0;
+    thr[q] = q < cuts.n ? cuts.frac[q] * (bmax + eps) : 0.0;
+  }
+  for (int k = threadIdx.x; k < M; k += kOutwardThreads) {
+    const int y = k / wx, x = k - y * wx;
+    const long i = (long)(y0 + y) * nx + (x0 + x);
+    const double t0 = b[i], t1 = b[pl + i], t2 = b[2 * pl + i];
+    const double e0 = c[i] - t0, e1 = c[pl + i] - t1, e2 = c[2 * pl + i] - t2;
+    const double mag = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+    const double q_rel = sqrt(e0 * e0 + e1 * e1 + e2 * e2) / mag;
+    for (int q = 0; q < cuts.n; ++q) {
+      const bool keep = mag > thr[q];
+      cnt[q] += keep ? 1.0 : 0.0;
+      rel[q] += keep ? q_rel : 0.0;
+    }
+  }
+  for (int q = 0; q < cuts.n; ++q) {
+    cnt[q] = tree_reduce(cnt[q], lds, false);
+    rel[q] = tree_reduce(rel[q], lds, false);
+  }
+  if (threadIdx.x == 0) {
+    out[0] = d2;
+    out[1] = b2;
+    out[2] = bmax;
+    out[3] = (double)M;
+    for (int q = 0; q < cuts.n; ++q) {
+      out[4 + 2 * q] = cnt[q];
+      out[5 + 2 * q] = rel[q];
+    }
+  }
+}
+
+}  // namespace
+}  // namespace nfgs
+
+extern "C" int nfgs_outward_errors(const double* b, const double* c, int ny, int nx, int y0,
+                                   int y1, int x0, int x1, const double* frac, int ncut,
+                                   double eps, double* out, void* stream) {
+  NFGS_REQUIRE(b && c && out && (ncut == 0 || frac), NFGS_ERR_ARG,
+               "outward_errors: bad argument");
+  NFGS_REQUIRE(ncut >= 0 && ncut <= kMaxCuts, NFGS_ERR_ARG,
+               "outward_errors: at most 16 thresholds");
+  NFGS_REQUIRE(0 <= y0 && y0 < y1 && y1 <= ny && 0 <= x0 && x0 < x1 && x1 <= nx, NFGS_ERR_ARG,
+               "outward_errors: the window is empty or outside the plane");
+  NFGS_REQUIRE((long)(y1 - y0) * (x1 - x0) < (1L << 30), NFGS_ERR_ARG,
+               "outward_errors: window too large");
+  CutList cuts{};
+  cuts.n = ncut;
+  for (int q = 0; q < ncut; ++q) cuts.frac[q] = frac[q];
+  hipLaunchKernelGGL(outward_errors_kernel, dim3(1), dim3(kOutwardThreads), 0, as_stream(stream),
+                     b, c, ny, nx, y0, x0, y1 - y0, x1 - x0, cuts, eps, out);
+  NFGS_LAUNCH_CHECK();
+  return NFGS_OK;
+}

@@ -1161,3 +1161,26 @@ def td_metrics(reference, candidate, current, divergence, slices, spacing):
     div_mean = float(np.mean(np.abs(np.asarray(divergence, dtype=np.float64)[sz, sy, sx])))
     return {k: m[k] for k in ("C_vec", "C_CS", "Eprime_n", "Eprime_m", "epsilon", "sigma_J")} | {
         "divB_normalized": div_mean / (6.0 * bn_mean / float(spacing) + 1e-12)}
+
+
+def outward_errors(target, candidate, slices, cutoffs):
+    """Errors of an outward bottom field against a target on the y/x window of
+    `slices` (nfgs/controlled.py:55-68): ||c - b|| / ||b|| over the window
+    (None for a zero target) and, per cutoff fraction, the mean of
+    |c - b| / |b| over the pixels with |b| > cutoff * max|b| (None when no
+    pixel is kept), with the kept count and fraction."""
+    _, sy, sx = slices
+    b = np.asarray(target, dtype=np.float64)[:, sy, sx]
+    c = np.asarray(candidate, dtype=np.float64)[:, sy, sx]
+    mag = np.sqrt(np.sum(b * b, axis=0))
+    err = np.sqrt(np.sum((c - b) ** 2, axis=0))
+    tot = float(np.sqrt(np.sum(b * b)))
+    res = {"E_rel_Bgt_L2": float(np.sqrt(np.sum((c - b) ** 2)) / tot) if tot > 0 else None,
+           "CCI_pixels_total": int(mag.size), "CCI_cutoff_results": {}}
+    for cut in cutoffs:
+        keep = mag > float(cut) * float(mag.max())
+        res["CCI_cutoff_results"][str(cut)] = {
+            "E_rel_Bgt": float(np.mean(err[keep] / mag[keep])) if keep.any() else None,
+            "retained_pixels": int(keep.sum()), "retained_fraction": float(keep.mean())}
+    res["E_rel_Bgt"] = res["CCI_cutoff_results"][str(cutoffs[0])]["E_rel_Bgt"]
+    return res

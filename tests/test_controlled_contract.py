@@ -1,9 +1,12 @@
 """Host side of the controlled evaluation (nfgs_amd.controlled) and the
 oracle's td_metrics.
 
-The NPZ contract checks and cci_errors follow nfgs/controlled.py:25-68 of the
-reference and its tests/test_controlled_evaluation.py:42-50 (coordinate and
-unit mismatches, a zero target scoring None).  td_metrics (the metrics of
+The NPZ contract checks follow nfgs/controlled.py:25-52 of the reference and
+its tests/test_controlled_evaluation.py:42-50 (coordinate and unit
+mismatches).  The oracle's outward_errors (the host arithmetic of the
+reference's cci_errors, controlled.py:55-68) is pinned by closed forms: a
+zero target scores None, a candidate equal to the target scores 0 and one of
+1.5 times the target scores 0.5 at every pixel and in L2.  td_metrics (the metrics of
 recompute_td_referee_diagnostics.py:344-400) is pinned by its divergence term
 on fields of known divergence, where the FD2 divergence is exact, and by the
 identity case.
@@ -11,7 +14,7 @@ identity case.
 import numpy as np
 import pytest
 
-from nfgs_amd.controlled import cci_errors, load, match
+from nfgs_amd.controlled import load, match
 from oracle import nfgs_oracle as O
 
 
@@ -76,7 +79,8 @@ def test_load_rejects_broken_files(tmp_path):
         load(tmp_path / "nan.npz")
 
 
-def test_cci_errors_zero_target_and_closed_forms():
+def test_outward_errors_zero_target_and_closed_forms():
+    cci_errors = O.outward_errors
     res = cci_errors(np.zeros((3, 7, 9)), np.ones((3, 7, 9)), (slice(None),) * 3, [1e-8, .01, .05])
     assert res["E_rel_Bgt_L2"] is None and res["E_rel_Bgt"] is None
     rng = np.random.default_rng(0)

@@ -4,7 +4,9 @@ The reference's own evaluation test (tests/test_controlled_evaluation.py:35-50:
 the anisotropic polynomial bundle with the example configuration, held here
 as tests/golden/evaluation_example.json) and every metric against the same
 evaluation assembled from the oracle (FD2 curl/divergence, td_metrics, the
-outward Gaussian field) and the host cci_errors.
+outward Gaussian field, outward_errors).  The device cci_errors
+(nfgs_outward_errors) is also checked against the oracle's outward_errors
+on random planes, windows and cutoffs, and on a zero target.
 """
 import sys
 from pathlib import Path
@@ -60,7 +62,7 @@ def test_identity_and_actual_observation_target(tmp_path):
 
 
 def _oracle_metrics(ref, field, obs, cfg):
-    from nfgs_amd.controlled import cci_errors
+    cci_errors = O.outward_errors
     box = tuple(slice(*s) for s in cfg["scoring_slices_zyx"])
     h = tuple(float(np.diff(ref[k])[0]) for k in ("x", "y", "z"))
     j = O.fd2_curl(field, h)
@@ -134,3 +136,21 @@ def test_bad_configs_are_rejected(tmp_path):
     (tmp_path / "v2.json").write_text(json.dumps(dict(cfg, schema_version=2)))
     with pytest.raises(ValueError, match="Unsupported evaluation config"):
         evaluate(f, f, tmp_path / "boundary.npz", tmp_path / "v2.json")
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_device_cci_errors_match_the_oracle(seed):
+    _announce(f"device cci_errors seed {seed}")
+    from nfgs_amd.controlled import cci_errors
+    rng = np.random.default_rng(seed)
+    ny, nx = 37 + 5 * seed, 53 - 3 * seed
+    b = rng.standard_normal((3, ny, nx)) * np.exp(rng.standard_normal((1, ny, nx)))
+    c = b + 0.1 * rng.standard_normal(b.shape)
+    cuts = [1e-8, 0.01, 0.05, 0.3, 0.9]
+    for box in ((slice(None),) * 3, (slice(None), slice(3, ny - 4), slice(1, nx - 7))):
+        got, want = cci_errors(b, c, box, cuts), O.outward_errors(b, c, box, cuts)
+        _close(got, want)
+    zero = cci_errors(np.zeros((3, 7, 9)), np.ones((3, 7, 9)), (slice(None),) * 3, [1e-8, .5])
+    assert zero == O.outward_errors(np.zeros((3, 7, 9)), np.ones((3, 7, 9)), (slice(None),) * 3,
+                                    [1e-8, .5])
+    assert zero["E_rel_Bgt_L2"] is None and zero["E_rel_Bgt"] is None

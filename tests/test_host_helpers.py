@@ -21,12 +21,13 @@ def test_scoring_box_and_spacing_checks():
 
 def test_evaluation_scoring_box():
     cfg = {"scoring_slices_zyx": [[0, 5, 1], [1, 6], [2, 8, None]]}
-    spec, box = controlled._scoring_box(cfg, (5, 7, 9))
-    assert box == (slice(0, 5, 1), slice(1, 6), slice(2, 8, None))
+    box = controlled.ScoringBox.parse(cfg["scoring_slices_zyx"], (5, 7, 9))
+    assert box.slices == (slice(0, 5, 1), slice(1, 6), slice(2, 8, None))
+    assert box.spec == cfg["scoring_slices_zyx"] and box.window == (slice(1, 6), slice(2, 8, None))
     for bad in ([[0, 6, 1], [0, 7, 1], [0, 9, 1]], [[0, 5, 2], [0, 7, 1], [0, 9, 1]],
                 [[0, 5, 1], [0, 7, 1]]):
         with pytest.raises(ValueError):
-            controlled._scoring_box({"scoring_slices_zyx": bad}, (5, 7, 9))
+            controlled.ScoringBox.parse(bad, (5, 7, 9))
 
 
 def test_npz_boundary_reader_and_arguments(tmp_path):
