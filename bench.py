@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 METRIC = "sec-to-converge at 256^3 (warm/batched vs plain), stencil HBM GB/s, err vs FP64"
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
+INT8_PEAK_TOPS = 5000.0  # dense int8 MFMA, MI355X
 
 
 def log(msg):
@@ -46,7 +47,11 @@ def parse():
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--max-iter", type=int, default=500)
     p.add_argument("--kernel-reps", type=int, default=20)
-    p.add_argument("--cpu-iters", type=int, default=2)
+    p.add_argument("--cpu-iters", type=int, default=3,
+                   help="timed CPU oracle iterations per thread count")
+    p.add_argument("--cpu-repeats", type=int, default=2,
+                   help="timed repeats of the CPU oracle's 64^3 solve (after one warm-up)")
+    p.add_argument("--repeats", type=int, default=5, help="timed GPU solves per rho")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 error reference")
     p.add_argument("--no-warm", action="store_true", help="skip the warm-start solves")
@@ -161,11 +166,25 @@ def kernel_model(s, esize):
         "reconstruct_b": dict(bytes=(1 + 1 + 3 + 3) * N * esize, words=8),
         "robin_sweep": dict(bytes=2 * N * esize, words=2),
         "dirichlet_sweep": dict(bytes=2 * NI * esize, words=2),
-        "dct_x": dict(bytes=2 * N * esize, flops=2.0 * Nx * N),
-        "dct_y": dict(bytes=2 * N * esize, flops=2.0 * Ny * N),
-        "dst_x": dict(bytes=2 * NI * esize, flops=2.0 * (Nx - 2) * NI),
-        "dst_y": dict(bytes=2 * NI * esize, flops=2.0 * (Ny - 2) * NI),
+        "dct_x": dict(bytes=2 * N * esize, flops=2.0 * Nx * N,
+                      int8_ops=int8_ops(Nx, Ny * Nz)),
+        "dct_y": dict(bytes=2 * N * esize, flops=2.0 * Ny * N,
+                      int8_ops=int8_ops(Ny, Nx * Nz)),
+        "dst_x": dict(bytes=2 * NI * esize, flops=2.0 * (Nx - 2) * NI,
+                      int8_ops=int8_ops(Nx - 2, (Ny - 2) * (Nz - 2))),
+        "dst_y": dict(bytes=2 * NI * esize, flops=2.0 * (Ny - 2) * NI,
+                      int8_ops=int8_ops(Ny - 2, (Nx - 2) * (Nz - 2))),
     }
+
+
+def int8_ops(n, nvec):
+    """int8 operations the FP32 transform kernel issues for nvec vectors of
+    length n: per 16-vector tile, 8 wavefronts x nch K-chunks x 2 halves x 10
+    digit pairs of v_mfma_i32_16x16x64_i8 (16*16*64 multiply-adds each), with
+    nch = max(1, (ceil(n/2) + 62) // 64) as the plan chooses it."""
+    nch = max(1, ((n + 1) // 2 + 62) // 64)
+    tiles = (nvec + 15) // 16
+    return tiles * 8 * nch * 2 * 10 * 16 * 16 * 64 * 2
 
 
 # launches of each kernel class in one iteration (forward and inverse passes
@@ -197,28 +216,71 @@ def measured_traffic(kernel, grid_zyx):
     return int(round(k["fetch_bytes"] + k["write_bytes"]))
 
 
-def cpu_baseline(n, rho, planes_np, iters, gpu_iters):
-    """CPU oracle (NumPy, one thread of the host) on a bounded sample: setup,
-    gs_cfcf start and `iters` iterations at the full grid, extrapolated to
-    the GPU solve's iteration count."""
-    import numpy as np
-    from oracle.nfgs_oracle import OracleNFGS
+def cpu_threads():
+    """Thread counts of the CPU baseline: the host's cores (at most 16, this
+    process's share of a one-GPU box) and 8, the reference runner's
+    (scripts/run_controlled_native.py:24)."""
+    host = min(os.cpu_count() or 1, 16)
+    return sorted({host, min(8, host)}, reverse=True)
 
+
+def cpu_baseline(n, rho, planes_np, iters, gpu_iters, threads):
+    """The CPU oracle (NumPy, its FFTs on `threads` threads through
+    scipy.fft workers; the elementwise work is NumPy's single thread) on a
+    bounded sample of the benchmark solve: setup and gs_cfcf start once, then
+    `iters` iterations timed one by one; extrapolated to the GPU solve's
+    iteration count with the median iteration time."""
+    import numpy as np
+    from oracle import nfgs_oracle as O
+
+    O.set_fft_workers(threads[0])
     bx, by, bz = planes_np
     t0 = time.perf_counter()
-    o = OracleNFGS(n, n, n, 1.0, 1.0, 1.0, dtype=np.float32, bottom_divergence_robin_weight=rho,
-                   source_zero_mode_policy="project", d0_zero_mode_policy="project")
+    o = O.OracleNFGS(n, n, n, 1.0, 1.0, 1.0, dtype=np.float32, bottom_divergence_robin_weight=rho,
+                     source_zero_mode_policy="project", d0_zero_mode_policy="project")
     o.set_bottom_vector_boundary(bx, by, bz)
     st = o.initial_state()
     o.sigma(st[0], st[1])
-    t1 = time.perf_counter()
-    for _ in range(iters):
-        st = o.step(*st)
-        o.sigma(st[0], st[1]), o.divB(st[0]), o.energy(st[0])
-    t2 = time.perf_counter()
-    per_it = (t2 - t1) / max(iters, 1)
-    return dict(setup_init_s=t1 - t0, s_per_iter=per_it,
-                extrapolated_s=(t1 - t0) + per_it * gpu_iters)
+    setup_init = time.perf_counter() - t0
+    per_thread = {}
+    for th in threads:
+        O.set_fft_workers(th)
+        times = []
+        for _ in range(iters):
+            t1 = time.perf_counter()
+            st = o.step(*st)
+            o.sigma(st[0], st[1]), o.divB(st[0]), o.energy(st[0])
+            times.append(time.perf_counter() - t1)
+        per_thread[th] = times
+    O.set_fft_workers(None)
+    med = float(np.median(per_thread[threads[0]]))
+    return dict(setup_init_s=setup_init, s_per_iter=med, per_thread=per_thread,
+                extrapolated_s=setup_init + med * gpu_iters)
+
+
+def cpu_cube64(threads, repeats):
+    """The CPU oracle's whole 64^3 configuration solve (two poles, FP32,
+    rho = 0.5, the sigma_best stop), 1 warm-up + `repeats` timed runs."""
+    import numpy as np
+    from oracle import nfgs_oracle as O
+    from nfgs_amd.boundary import dipole_boundary
+
+    planes = dipole_boundary(64, seed=0)
+    O.set_fft_workers(threads)
+    times, h = [], None
+    for r in range(repeats + 1):
+        t0 = time.perf_counter()
+        o = O.OracleNFGS(64, 64, 64, 1.0, 1.0, 1.0, dtype=np.float32,
+                         bottom_divergence_robin_weight=0.5, source_zero_mode_policy="project",
+                         d0_zero_mode_policy="project")
+        o.set_bottom_vector_boundary(*planes)
+        h = o.iterate(max_iter=500, tol=1e-5, patience=40, sigma_improvement_tol=1e-5)
+        if r:
+            times.append(time.perf_counter() - t0)
+    O.set_fft_workers(None)
+    return dict(median_s=float(np.median(times)), min_s=min(times), max_s=max(times),
+                repeats=repeats, iterations=h["final_iter"], best_iter=h["best_iter"],
+                stop_reason=h["stop_reason"])
 
 
 def main():
@@ -260,14 +322,22 @@ def main():
     per_rho = {}
     keep = None
     for rho in (0.5, 0.0, 1.0):
-        log(f"solve fp32 rho={rho}")
-        s, h, setup_s, total_s = solve(torch, NFGSSolverGPU, planes, n, rho, torch.float32,
-                                       a.max_iter)
-        per_rho[str(rho)] = dict(sec_to_converge=total_s, setup_s=setup_s,
+        times, setups = [], []
+        for r in range(max(a.repeats, 1)):
+            s, h, setup_s, total_s = solve(torch, NFGSSolverGPU, planes, n, rho, torch.float32,
+                                           a.max_iter)
+            times.append(total_s)
+            setups.append(setup_s)
+            if r + 1 < a.repeats:
+                del s
+        per_rho[str(rho)] = dict(sec_to_converge=float(np.median(times)),
+                                 sec_to_converge_min=min(times), sec_to_converge_max=max(times),
+                                 repeats=len(times), setup_s=float(np.median(setups)),
                                  iterations=h["final_iter"], best_iter=h["best_iter"],
                                  best_sigma=h["best_sigma"], stop_reason=h["stop_reason"])
-        log(f"  {total_s:.3f} s, {h['final_iter']} it, best {h['best_sigma']:.5e} "
-            f"({h['stop_reason']})")
+        log(f"solve fp32 rho={rho}: median {np.median(times):.4f} s over {len(times)} "
+            f"(min {min(times):.4f}, max {max(times):.4f}), {h['final_iter']} it, "
+            f"best {h['best_sigma']:.5e} ({h['stop_reason']})")
         if rho == 0.5:
             keep = (s, h)
         else:
@@ -299,7 +369,12 @@ def main():
         rec = dict(us=round(t * 1e6, 2), per_iter=PER_ITER[k],
                    hbm_gbs=round(m["bytes"] / t / 1e9, 1))
         if "flops" in m:
-            rec["tflops"] = round(m["flops"] / t / 1e12, 2)
+            # the FP32-equivalent rate of the exact-integer scheme (2n flops
+            # per output as a plain FP32 product would count them); not a
+            # utilisation of any unit
+            rec["fp32_equivalent_tflops"] = round(m["flops"] / t / 1e12, 2)
+            rec["int8_tops_issued"] = round(m["int8_ops"] / t / 1e12, 1)
+            rec["int8_mfma_frac"] = round(m["int8_ops"] / t / 1e12 / INT8_PEAK_TOPS, 4)
         kernels[k] = rec
     dom = max(share, key=share.get)
     stencils = [k for k in ktimes if "flops" not in model[k]]
@@ -307,10 +382,11 @@ def main():
     def roof(k):
         m, t = model[k], ktimes[k]
         if "flops" in m:
-            ach = m["flops"] / t / 1e12
-            return dict(kernel=k, bound="mfma", achieved=round(ach, 2),
-                        peak=MFMA_PEAK_TFLOPS["f32"], unit="TFLOP/s",
-                        frac=round(ach / MFMA_PEAK_TFLOPS["f32"], 4), traffic=None,
+            # issued int8 MFMA operations against the int8 peak
+            ach = m["int8_ops"] / t / 1e12
+            return dict(kernel=k, bound="mfma", achieved=round(ach, 1),
+                        peak=INT8_PEAK_TOPS, unit="TOP/s (int8)",
+                        frac=round(ach / INT8_PEAK_TOPS, 4), traffic=None,
                         avg_launch_us=round(t * 1e6, 2))
         ach = m["bytes"] / t / 1e9
         return dict(kernel=k, bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
@@ -320,6 +396,10 @@ def main():
     stencil_roof = roof(dom_sten)
     for r in (roofline, stencil_roof):
         r["traffic"] = measured_traffic(r["kernel"], [s.Nz, s.Ny, s.Nx])
+        if r["traffic"] is not None:
+            # not measured by this run: counters cannot be read beside timing
+            r["traffic_source"] = ("profiles/pmc_traffic_258x258x257_f32.json (rocprofv3 "
+                                   "FETCH_SIZE and WRITE_SIZE passes, calibrated)")
     log(f"dominant kernel {dom}: {roofline}")
 
     # warm start (init_mode="field"): the next frame of the same region (poles
@@ -414,15 +494,24 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        log(f"CPU oracle sample: setup + init + {a.cpu_iters} iterations")
-        cb = cpu_baseline(n, 0.5, (bx, by, bz), a.cpu_iters, head["iterations"])
-        cpu = dict(value=round(cb["extrapolated_s"], 2), unit="s", cores=1, kind="port",
-                   sample=f"NumPy oracle, FP32, 258x258x257, rho=0.5: setup+gs_cfcf init "
-                          f"{cb['setup_init_s']:.2f} s measured, {a.cpu_iters} iterations at "
-                          f"{cb['s_per_iter']:.2f} s each, extrapolated to the GPU solve's "
-                          f"{head['iterations']} iterations",
-                   s_per_iter=round(cb["s_per_iter"], 3))
-
+        threads = cpu_threads()
+        log(f"CPU oracle sample: setup + init + {a.cpu_iters} iterations per thread count "
+            f"{threads}; 64^3 solve x {a.cpu_repeats}")
+        cb = cpu_baseline(n, 0.5, (bx, by, bz), a.cpu_iters, head["iterations"], threads)
+        c64 = cpu_cube64(threads[0], a.cpu_repeats)
+        per = {str(th): dict(s_per_iter_median=round(float(np.median(v)), 3),
+                             s_per_iter_min=round(min(v), 3), s_per_iter_max=round(max(v), 3))
+               for th, v in cb["per_thread"].items()}
+        cpu = dict(value=round(cb["extrapolated_s"], 2), unit="s", cores=threads[0], kind="port",
+                   sample=f"NumPy oracle (FFTs on {threads[0]} threads), FP32, 258x258x257, "
+                          f"rho=0.5: setup+gs_cfcf init {cb['setup_init_s']:.2f} s measured once, "
+                          f"{a.cpu_iters} iterations timed at {threads} threads (median "
+                          f"{cb['s_per_iter']:.2f} s each at {threads[0]}), extrapolated to the "
+                          f"GPU solve's {head['iterations']} iterations",
+                   s_per_iter=round(cb["s_per_iter"], 3), per_threads=per,
+                   host_cpu_count=os.cpu_count(),
+                   cube64_full_solve=c64)
+        log(f"  CPU: {cpu}")
     line = {
         "metric": METRIC,
         "value": round(head["sec_to_converge"], 4),

@@ -102,6 +102,12 @@ long nfgs_partials_count(const nfgs_plan* plan);
  * lambda (ny*nx), 1 k = sqrt(-lambda) (ny*nx), 2 endpoint-Dirichlet lambda
  * ((ny-2)*(nx-2)), 3 alpha_x (nx), 4 alpha_y (ny). */
 const void* nfgs_plan_table(const nfgs_plan* plan, int which);
+/* Element count of plan table `which` (0 if none). */
+long nfgs_plan_table_size(const nfgs_plan* plan, int which);
+/* Copy plan table `which` (in the plan dtype) to the caller's device buffer
+ * dst.  Table 0 replaces _endpoint_neumann_lam_xy (nfgs_solver.py:1098-1109),
+ * table 2 _dirichlet_lam_xy_interior (:1083-1096). */
+int nfgs_copy_plan_table(const nfgs_plan* plan, int which, void* dst, void* stream);
 
 /* Unnormalised type-I transform (forward) or its inverse along one axis of a
  * batch of vectors: vector g = p*nvec + q starts at p*plane_stride +
@@ -122,6 +128,21 @@ int nfgs_transform(const nfgs_plan* plan, int kind, int direction, const void* i
  * (nfgs_solver.py:5627-5672, :5512-5625). */
 int nfgs_robin_solve(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* src_hat,
                      void* out_hat, void* scratch, double* partials, void* stream);
+
+/* The modal Robin solve with every input explicit: per endpoint-Neumann
+ * mode (d_zz + lambda) u = S (or S / lambda with divide_by_lambda), u(0) =
+ * bottom_hat, the first interior row blended with the bottom derivative
+ * condition by rho (dtarget_hat, required when rho > 0), and the top Robin
+ * row (d_z + k) u = g with g = top_hat (NULL: 0) after eliminating u_{N-2}.
+ * src_hat, out_hat and scratch are nz planes of ny*nx; the (0,0) mode of
+ * out_hat is 0.  zero_mode_sum (one device double, may be NULL) receives the
+ * sum over heights of (src_hat[z,0,0] / (4 (nx-1)(ny-1)))^2.  Replaces
+ * _solve_modal_robin_from_hat (nfgs_solver.py:5512-5625) up to its final
+ * inverse DCT2. */
+int nfgs_modal_robin_solve(const nfgs_plan* plan, const void* src_hat, const void* bottom_hat,
+                           const void* dtarget_hat, const void* top_hat, double rho,
+                           int divide_by_lambda, void* out_hat, void* scratch,
+                           double* zero_mode_sum, void* stream);
 
 /* Dirichlet z-solve (d_zz + lambda_D) Az = -S/lambda_D per interior DST-I
  * mode, Az(0) = az_bottom_hat, Az(top) = 0, on compact (nz,ny-2,nx-2)
@@ -146,6 +167,37 @@ int nfgs_reconstruct_b(const nfgs_plan* plan, const nfgs_boundary* bnd, const vo
  * and compute_energy (:2689-2697). */
 int nfgs_face_pass(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* b,
                    const void* az, void* jp_out, void* j_out, double* partials, void* stream);
+
+/* nfgs_face_pass with the current taken from a caller's J (3*N) instead of
+ * Az: the centre Jz of the face state is J[2] and the bottom row of J_par is
+ * J[:, 0], as the reference's iteration_step takes them from its J argument
+ * (compute_parallel_currents(J, B) and J_para[:, 0] = J[:, 0],
+ * nfgs_solver.py:6690-6711, :5811-5839).  For the state's own current the
+ * result equals nfgs_face_pass's.  jp_out or partials may be NULL (not both);
+ * the partials give compute_sigma, compute_divB and compute_energy
+ * (nfgs_solver.py:5882-5895, :2689-2697) through nfgs_reduce. */
+int nfgs_face_pass_j(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* b,
+                     const void* j, void* jp_out, double* partials, void* stream);
+
+/* Collocated reconstruction of the endpoint-Neumann PT subspace from the
+ * scalars (Phi_c, Az, Bc_z; N each): Bc = (-Dx Dz Phi + Dy Az,
+ * -Dy Dz Phi - Dx Az, Bc_z), J = (Dx Dz Az + Dy L, Dy Dz Az - Dx L,
+ * -lap_xy Az) with L = Bc_z + D2z Phi, B = B_bg + Bc, and the boundary
+ * values (J = 0 on the side walls and the top, B_z(0) = Bz_obs,
+ * J_z(0) = Jz_T).  Any of b_out, j_out, bc_out (3*N each) may be NULL.
+ * Replaces _reconstruct_pt_state_from_scalars (nfgs_solver.py:5897-5899,
+ * :3311-3341). */
+int nfgs_reconstruct_pt(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* phi,
+                        const void* az, const void* bcz, void* b_out, void* j_out, void* bc_out,
+                        void* stream);
+
+/* The explicit k_h = 0 field and current of a profile (2*nz: H_x then H_y):
+ * b = (H_x, H_y, 0), j = (-dH_y/dz, dH_x/dz, 0) with the staggered-face
+ * centre derivative.  b_out or j_out (3*N) may be NULL.  Replaces
+ * _k0_field_and_current and _k0_center_derivative
+ * (nfgs_solver.py:5400-5455). */
+int nfgs_k0_field(const nfgs_plan* plan, const void* profile, void* b_out, void* j_out,
+                  void* stream);
 
 /* Sources S_Bcz = Dy Jpx - Dx Jpy (full grid) and S_Jz = lap_xy Jpz -
  * D2xz Jpx - D2yz Jpy (interior, compact), plus the S_Bcz^2 partial.
@@ -198,6 +250,20 @@ int nfgs_advance_ex(const nfgs_plan* plan, const nfgs_boundary* bnd, const nfgs_
  * edge zeroing.  ijzbcor in {0..4}. */
 int nfgs_setup_jz(const nfgs_plan* plan, const void* bx, const void* by, void* jz, int hny,
                   int hnx, int bfy, int bfx, int ijzbcor, int legacy, void* stream);
+
+/* nfgs_setup_jz without a plan, for callers that hold only the grid
+ * (NFGSSolverGPU._derive_bottom_jz_fortran called on a plain namespace, as
+ * the reference's tests do: tests/test_bottom_current_padding.py:11-41):
+ * dtype NFGS_F32 or NFGS_F64, spacings dx, dy, the rest as there
+ * (nfgs_solver.py:4489-4546, :1498-1567). */
+int nfgs_bottom_jz(const void* bx, const void* by, void* jz, int ny, int nx, double dx, double dy,
+                   int dtype, int hny, int hnx, int bfy, int bfx, int ijzbcor, int legacy,
+                   void* stream);
+
+/* Mirror-Neumann 5-point lateral Laplacian of nplanes stacked (ny, nx)
+ * planes (every node; ends take the mirrored neighbour).  Replaces _lap_xy
+ * (nfgs_solver.py:1741-1749). */
+int nfgs_lap_xy(const nfgs_plan* plan, const void* in, void* out, int nplanes, void* stream);
 
 /* Endpoint-Dirichlet 2-D Poisson of scale*src on one plane: interior solve,
  * zero edges (nfgs_solver.py:1132-1144).  work: 2*(ny-2)*(nx-2) elements. */

@@ -65,12 +65,23 @@ def _declare(lib):
         "nfgs_plan_destroy": (C.c_int, [vp]),
         "nfgs_partials_count": (C.c_long, [vp]),
         "nfgs_plan_table": (vp, [vp, C.c_int]),
+        "nfgs_plan_table_size": (C.c_long, [vp, C.c_int]),
+        "nfgs_copy_plan_table": (C.c_int, [vp, C.c_int, vp, vp]),
+        "nfgs_modal_robin_solve": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp,
+                                             vp]),
+        "nfgs_bottom_jz": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_double, C.c_double,
+                                     C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, vp]),
+        "nfgs_lap_xy": (C.c_int, [vp, vp, vp, C.c_int, vp]),
+        "nfgs_reconstruct_pt": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp]),
+        "nfgs_k0_field": (C.c_int, [vp, vp, vp, vp, vp]),
         "nfgs_transform": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_long, C.c_long, C.c_long, vp,
                                      C.c_long, C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, vp]),
         "nfgs_robin_solve": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp]),
         "nfgs_dirichlet_solve": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp]),
         "nfgs_reconstruct_b": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp]),
         "nfgs_face_pass": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp]),
+        "nfgs_face_pass_j": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp]),
         "nfgs_state_pass": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp]),
         "nfgs_sources": (C.c_int, [vp, vp, vp, vp, vp, vp]),
         "nfgs_reduce": (C.c_int, [vp, vp, vp, vp]),

@@ -305,3 +305,24 @@ extern "C" const void* nfgs_plan_table(const nfgs_plan* p, int which) {
     default: return nullptr;
   }
 }
+
+extern "C" long nfgs_plan_table_size(const nfgs_plan* p, int which) {
+  if (!p) return 0;
+  switch (which) {
+    case 0:
+    case 1: return p->plane;
+    case 2: return p->iplane;
+    case 3: return p->d.nx;
+    case 4: return p->d.ny;
+    default: return 0;
+  }
+}
+
+extern "C" int nfgs_copy_plan_table(const nfgs_plan* p, int which, void* dst, void* stream) {
+  NFGS_REQUIRE(p && dst, NFGS_ERR_ARG, "copy_plan_table: null argument");
+  const void* src = nfgs_plan_table(p, which);
+  NFGS_REQUIRE(src, NFGS_ERR_ARG, "copy_plan_table: no such table");
+  NFGS_HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)nfgs_plan_table_size(p, which) * p->esize,
+                                hipMemcpyDeviceToDevice, nfgs::as_stream(stream)));
+  return NFGS_OK;
+}

@@ -525,3 +525,48 @@ extern "C" int nfgs_setup_gaussian(const nfgs_plan* p, const void* bx, const voi
       static_cast<double*>(dphi_target), static_cast<double*>(d0),
       static_cast<double*>(phi_above_hat), static_cast<double*>(h_target), diag, workspace, s);
 }
+
+extern "C" int nfgs_bottom_jz(const void* bx, const void* by, void* jz, int ny, int nx, double dx,
+                              double dy, int dtype, int hny, int hnx, int bfy, int bfx,
+                              int ijzbcor, int legacy, void* stream) {
+  NFGS_REQUIRE(bx && by && jz && ny >= 1 && nx >= 1, NFGS_ERR_ARG, "bottom_jz: bad argument");
+  NFGS_REQUIRE(dx > 0.0 && dy > 0.0, NFGS_ERR_ARG, "bottom_jz: spacing must be positive");
+  NFGS_REQUIRE(dtype == NFGS_F32 || dtype == NFGS_F64, NFGS_ERR_ARG, "bottom_jz: bad dtype");
+  NFGS_REQUIRE(hny >= ny + bfy && hnx >= nx + bfx && bfy >= 0 && bfx >= 0, NFGS_ERR_ARG,
+               "bottom_jz: helper grid does not contain the calculation grid");
+  hipStream_t s = as_stream(stream);
+  if (dtype == NFGS_F32)
+    hipLaunchKernelGGL((jz_kernel<float>), dim3(1), dim3(1024), 0, s,
+                       static_cast<const float*>(bx), static_cast<const float*>(by),
+                       static_cast<float*>(jz), ny, nx, hny, hnx, bfy, bfx, float(2.0 * dx),
+                       float(2.0 * dy), ijzbcor, legacy);
+  else
+    hipLaunchKernelGGL((jz_kernel<double>), dim3(1), dim3(1024), 0, s,
+                       static_cast<const double*>(bx), static_cast<const double*>(by),
+                       static_cast<double*>(jz), ny, nx, hny, hnx, bfy, bfx, 2.0 * dx, 2.0 * dy,
+                       ijzbcor, legacy);
+  NFGS_LAUNCH_CHECK();
+  return NFGS_OK;
+}
+
+extern "C" int nfgs_lap_xy(const nfgs_plan* p, const void* in, void* out, int nplanes,
+                           void* stream) {
+  NFGS_REQUIRE(p && in && out && in != out && nplanes >= 1, NFGS_ERR_ARG,
+               "lap_xy: bad argument");
+  const int nx = p->d.nx, ny = p->d.ny;
+  const long pl = p->plane;
+  hipStream_t s = as_stream(stream);
+  auto run = [&](auto tag) {
+    using T = decltype(tag);
+    const T* a = static_cast<const T*>(in);
+    T* o = static_cast<T*>(out);
+    const T idx2 = T(1.0 / (p->d.dx * p->d.dx)), idy2 = T(1.0 / (p->d.dy * p->d.dy));
+    return launch_map((long)nplanes * pl, [=] __device__(long i) {
+      const long q = i / pl, r = i - q * pl;
+      const int y = (int)(r / nx), x = (int)(r - (long)y * nx);
+      o[i] = lap_xy_at(a + q * pl, y, x, ny, nx, (long)nx, idx2, idy2);
+    }, s);
+  };
+  if (p->d.dtype == NFGS_F32) return run(0.0f);
+  return run(0.0);
+}

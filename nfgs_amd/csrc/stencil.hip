@@ -394,11 +394,15 @@ struct Window {
   int z0, z1, y0, y1, x0, x1;
 };
 
+// With jc (a caller's current J, 3 volumes) the centre Jz is J[2] and the
+// bottom row of J_par is J[:, 0], as the reference's iteration_step takes
+// them from its J argument (:6690-6711); az is then not read.  Without jc,
+// Jz is -lap_xy(Az) with its boundary values (the state's own current).
 template <typename T>
 __global__ __launch_bounds__(256) void face_pass_kernel(
     Grid<T> g, const T* __restrict__ B, const T* __restrict__ bg, const T* __restrict__ az,
-    const T* __restrict__ jzt, T* __restrict__ jp, T* __restrict__ jout, double* partials,
-    long pstride, T floor_b2, T amax, Window w) {
+    const T* __restrict__ jc, const T* __restrict__ jzt, T* __restrict__ jp,
+    T* __restrict__ jout, double* partials, long pstride, T floor_b2, T amax, Window w) {
   __shared__ T scz[2][PSA];
   __shared__ T sbx[2][PSA];
   __shared__ T sby[2][PSA];
@@ -415,7 +419,8 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
   const T* Bx = B;
   const T* By = B + g.N;
   const T* Bz = B + 2 * g.N;
-  const T jz0 = ld(jzt, h.own);  // toroidal bottom current at the own point
+  // bottom current at the own point: the toroidal one, or the caller's J[2, 0]
+  const T jz0 = jc ? ld(jc + 2 * g.N, h.own) : ld(jzt, h.own);
   // per-thread sums over the z-chunk in the solver dtype; blocks combine
   // them in FP64
   T acc[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -423,7 +428,7 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
   T pb[3] = {0, 0, 0}, pg[2] = {0, 0}, pjz = 0, pz2 = 0;
   FaceVal<T> fprev;
   Fetch<T> fbz, fgz, fbx, fby, faz;
-  T obz, ogx, ogy;  // own Bz, bg_x, bg_y of the fetched plane
+  T obz, ogx, ogy, ojz = 0;  // own Bz, bg_x, bg_y (and caller Jz) of the fetched plane
   // Every iteration loads a plane (clamped to pe) so the waits before each
   // commit are exact.
   auto fetch_plane = [&](int q) {
@@ -432,7 +437,10 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
     fetch(fgz, pl(bg + 2 * g.N, q), h);
     fetch(fbx, pl(Bx, q), h);
     fetch(fby, pl(By, q), h);
-    fetch(faz, pl(az, q), h);
+    if (jc)
+      ojz = ld(pl(jc + 2 * g.N, q), h.own);
+    else
+      fetch(faz, pl(az, q), h);
     obz = ld(pl(Bz, q), h.own);
     ogx = ld(pl(bg, q), h.own);
     ogy = ld(pl(bg + g.N, q), h.own);
@@ -442,10 +450,11 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
     commit_diff(scz[p & 1], fbz, fgz);
     commit(sbx[p & 1], fbx);
     commit(sby[p & 1], fby);
-    commit(saz, faz);
+    if (!jc) commit(saz, faz);
     sown[0][threadIdx.x] = obz;
     sown[1][threadIdx.x] = ogx;
     sown[2][threadIdx.x] = ogy;
+    const T jz_given = ojz;
     lds_barrier();
     fetch_plane(p + 1);
     const T cbz = sown[0][threadIdx.x], cgx = sown[1][threadIdx.x], cgy = sown[2][threadIdx.x];
@@ -456,7 +465,9 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
       cb[2] = cbz;
       cg[0] = cgx;
       cg[1] = cgy;
-      if (p == 0)
+      if (jc)
+        cjz = jz_given;
+      else if (p == 0)
         cjz = jz0;
       else if (p == g.nz - 1 || edge)
         cjz = T(0);
@@ -492,7 +503,11 @@ __global__ __launch_bounds__(256) void face_pass_kernel(
       const int k = f;  // node finalised now
       if (k >= k0 && k < k1) {
         if (jp) {
-          if (k == 0) {
+          if (k == 0 && jc) {
+            st(pl(jp, k), h.own, ld(jc, h.own));
+            st(pl(jp + g.N, k), h.own, ld(jc + g.N, h.own));
+            st(pl(jp + 2 * g.N, k), h.own, jz0);
+          } else if (k == 0) {
             st(pl(jp, k), h.own, edge ? T(0) : F.j[0]);
             st(pl(jp + g.N, k), h.own, edge ? T(0) : F.j[1]);
             st(pl(jp + 2 * g.N, k), h.own, jz0);
@@ -1077,13 +1092,14 @@ int reconstruct_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi,
 }
 
 template <typename T>
-int face_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* b, const void* az, void* jp,
-           void* jout, double* partials, hipStream_t s) {
+int face_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* b, const void* az,
+           const void* jc, void* jp, void* jout, double* partials, hipStream_t s) {
   const Grid<T> g = make_grid<T>(p);
   const Window w{p->d.win_z0, p->d.win_z1, p->d.win_y0, p->d.win_y1, p->d.win_x0, p->d.win_x1};
   hipLaunchKernelGGL((face_pass_kernel<T>), tiled_grid(p), dim3(TX * TY), 0, s, g,
                      static_cast<const T*>(b), static_cast<const T*>(bnd->b_bg),
-                     static_cast<const T*>(az), static_cast<const T*>(bnd->jz_t),
+                     static_cast<const T*>(az), static_cast<const T*>(jc),
+                     static_cast<const T*>(bnd->jz_t),
                      static_cast<T*>(jp), static_cast<T*>(jout), partials, (long)p->pblocks,
                      T(p->d.weak_field_floor_b2), T(p->d.alpha_max), w);
   NFGS_LAUNCH_CHECK();
@@ -1167,8 +1183,18 @@ extern "C" int nfgs_face_pass(const nfgs_plan* p, const nfgs_boundary* bnd, cons
   NFGS_REQUIRE(p && bnd && b && az, NFGS_ERR_ARG, "face_pass: null argument");
   NFGS_REQUIRE(bnd->b_bg && bnd->jz_t, NFGS_ERR_STATE, "face_pass: boundary is not set");
   if (p->d.dtype == NFGS_F32)
-    return face_t<float>(p, bnd, b, az, jp_out, j_out, partials, as_stream(stream));
-  return face_t<double>(p, bnd, b, az, jp_out, j_out, partials, as_stream(stream));
+    return face_t<float>(p, bnd, b, az, nullptr, jp_out, j_out, partials, as_stream(stream));
+  return face_t<double>(p, bnd, b, az, nullptr, jp_out, j_out, partials, as_stream(stream));
+}
+
+extern "C" int nfgs_face_pass_j(const nfgs_plan* p, const nfgs_boundary* bnd, const void* b,
+                                const void* j, void* jp_out, double* partials, void* stream) {
+  NFGS_REQUIRE(p && bnd && b && j && (jp_out || partials), NFGS_ERR_ARG,
+               "face_pass_j: null argument");
+  NFGS_REQUIRE(bnd->b_bg && bnd->jz_t, NFGS_ERR_STATE, "face_pass_j: boundary is not set");
+  if (p->d.dtype == NFGS_F32)
+    return face_t<float>(p, bnd, b, nullptr, j, jp_out, nullptr, partials, as_stream(stream));
+  return face_t<double>(p, bnd, b, nullptr, j, jp_out, nullptr, partials, as_stream(stream));
 }
 
 extern "C" int nfgs_state_pass(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi,

@@ -85,9 +85,10 @@ __device__ __forceinline__ void back_substitute(T* out, const T* cs, long plane,
 template <typename T>
 __global__ __launch_bounds__(256) void robin_sweep_kernel(
     const T* __restrict__ src, T* __restrict__ out, T* __restrict__ cs, const T* __restrict__ lam,
-    const T* __restrict__ kk, const T* __restrict__ bh, const T* __restrict__ qh, int nz,
-    long plane, T inv_dz2, T two_inv_dz2, T dz, T dz2, T two_dz, T rho, T one_m_rho, T kb,
-    T fu_rho, T rho_over_dz, double* zm_partial, double norm, bool div_src) {
+    const T* __restrict__ kk, const T* __restrict__ bh, const T* __restrict__ qh,
+    const T* __restrict__ gh, int nz, long plane, T inv_dz2, T two_inv_dz2, T dz, T dz2,
+    T two_dz, T rho, T one_m_rho, T kb, T fu_rho, T rho_over_dz, double* zm_partial, double norm,
+    bool div_src) {
   const long m = (long)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (m >= plane) return;
   if (m == 0) {
@@ -141,7 +142,8 @@ __global__ __launch_bounds__(256) void robin_sweep_kernel(
     const T s = src[(long)(nz - 2) * plane + m] / ls;
     const T rl = (-l) * dz2 - T(2);
     const T rd = T(2) + two_dz * kk[m];
-    const T rr = T(0) - s * dz2;
+    // top Robin value g of (d_z + k) u (0 on the NFGS path)
+    const T rr = (gh ? two_dz * gh[m] : T(0)) - s * dz2;
     const T den = rd - rl * cp;
     dp = (rr - rl * dp) / den;
   }
@@ -370,9 +372,10 @@ __device__ __forceinline__ void seg_solve(SegLds<T>& L, T (&d)[SEG], const SegRo
 template <typename T, int SEG>
 __global__ __launch_bounds__(64 * kSegMax) void robin_seg_kernel(
     const T* __restrict__ src, T* __restrict__ out, const T* __restrict__ lam,
-    const T* __restrict__ kk, const T* __restrict__ bh, const T* __restrict__ qh, int nz,
-    long plane, T inv_dz2, T two_inv_dz2, T dz, T dz2, T two_dz, T rho, T one_m_rho, T kb,
-    T fu_rho, T rho_over_dz, double* zm_partial, double norm, bool div_src) {
+    const T* __restrict__ kk, const T* __restrict__ bh, const T* __restrict__ qh,
+    const T* __restrict__ gh, int nz, long plane, T inv_dz2, T two_inv_dz2, T dz, T dz2,
+    T two_dz, T rho, T one_m_rho, T kb, T fu_rho, T rho_over_dz, double* zm_partial, double norm,
+    bool div_src) {
   __shared__ SegLds<T> L;
   // the segment index is uniform per wavefront: keep it (and the row
   // addresses) scalar
@@ -402,8 +405,10 @@ __global__ __launch_bounds__(64 * kSegMax) void robin_seg_kernel(
       rp += pb;
     });
   }
-  const T stop = (last >= 0 && last < SEG) ? ld(reinterpret_cast<const T*>(row_ptr(src, nz - 2)), mo)
-                                           : T(0);
+  const bool has_top = last >= 0 && last < SEG;
+  const T stop = has_top ? ld(reinterpret_cast<const T*>(row_ptr(src, nz - 2)), mo) : T(0);
+  // top Robin value g of (d_z + k) u (0 on the NFGS path)
+  const T gtop = (has_top && gh) ? two_dz * ld(gh, mo) : T(0);
   if (m == 0) {
     // (0,0) mode: no correction; sum of squares of the normalised source
     double acc = 0.0;
@@ -437,7 +442,7 @@ __global__ __launch_bounds__(64 * kSegMax) void robin_seg_kernel(
     T v = d[I] * il;
     if (I == 0 && k == 0)
       v = rho > T(0) ? (one_m_rho * v - rho_over_dz * q0) - kb * b0 : v - inv_dz2 * b0;
-    if (I == last) v = T(0) - (stop * il) * dz2;
+    if (I == last) v = gtop - (stop * il) * dz2;
     if (I > last) v = T(0);
     d[I] = v;
   });
@@ -514,19 +519,19 @@ __global__ __launch_bounds__(64 * kSegMax) void dirichlet_seg_kernel(
 }
 
 template <typename T>
-int robin_t(const nfgs_plan* p, const void* bh, const void* qh, const void* src, void* out,
-            void* scr, double* partials, bool div_src, hipStream_t s) {
+int robin_t(const nfgs_plan* p, const void* bh, const void* qh, const void* gh, const void* src,
+            void* out, void* scr, double* zm, double rho, bool div_src, hipStream_t s) {
   const nfgs_grid_desc& d = p->d;
-  const double dz = d.dz, dz2 = dz * dz, rho = d.rho;
+  const double dz = d.dz, dz2 = dz * dz;
   const double norm = 4.0 * (d.nx - 1) * (d.ny - 1);
-  double* zm = partials ? partials + (long)NFGS_Q_ZERO_MODE * p->pblocks : nullptr;
   constexpr int SEG = sizeof(T) == 4 ? 64 : 32;
   const int ns = (d.nz - 1 + SEG - 1) / SEG;
   if (ns <= kSegMax) {
     hipLaunchKernelGGL((robin_seg_kernel<T, SEG>), dim3((unsigned)((p->plane + 63) / 64)),
                        dim3(64 * ns), 0, s, static_cast<const T*>(src), static_cast<T*>(out),
                        static_cast<const T*>(p->lam), static_cast<const T*>(p->kk),
-                       static_cast<const T*>(bh), static_cast<const T*>(qh), d.nz, p->plane,
+                       static_cast<const T*>(bh), static_cast<const T*>(qh),
+                       static_cast<const T*>(gh), d.nz, p->plane,
                        T(1.0 / dz2),
                        T(2.0 / dz2), T(dz), T(dz2), T(2.0 * dz), T(rho), T(1.0 - rho),
                        T((1.0 + 0.5 * rho) / dz2), T((1.0 - 0.5 * rho) / dz2), T(rho / dz), zm,
@@ -538,7 +543,8 @@ int robin_t(const nfgs_plan* p, const void* bh, const void* qh, const void* src,
   hipLaunchKernelGGL((robin_sweep_kernel<T>), dim3(nb), dim3(256), 0, s,
                      static_cast<const T*>(src), static_cast<T*>(out), static_cast<T*>(scr),
                      static_cast<const T*>(p->lam), static_cast<const T*>(p->kk),
-                     static_cast<const T*>(bh), static_cast<const T*>(qh), d.nz, p->plane,
+                     static_cast<const T*>(bh), static_cast<const T*>(qh),
+                       static_cast<const T*>(gh), d.nz, p->plane,
                      T(1.0 / dz2), T(2.0 / dz2), T(dz), T(dz2), T(2.0 * dz), T(rho), T(1.0 - rho),
                      T((1.0 + 0.5 * rho) / dz2), T((1.0 - 0.5 * rho) / dz2), T(rho / dz), zm,
                      norm, div_src);
@@ -573,8 +579,10 @@ int dirichlet_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* src, v
 
 int robin_modal(const nfgs_plan* p, const void* bh, const void* qh, const void* src, void* out,
                 void* scr, double* partials, bool div_src, hipStream_t s) {
-  if (p->d.dtype == NFGS_F32) return robin_t<float>(p, bh, qh, src, out, scr, partials, div_src, s);
-  return robin_t<double>(p, bh, qh, src, out, scr, partials, div_src, s);
+  double* zm = partials ? partials + (long)NFGS_Q_ZERO_MODE * p->pblocks : nullptr;
+  if (p->d.dtype == NFGS_F32)
+    return robin_t<float>(p, bh, qh, nullptr, src, out, scr, zm, p->d.rho, div_src, s);
+  return robin_t<double>(p, bh, qh, nullptr, src, out, scr, zm, p->d.rho, div_src, s);
 }
 
 }  // namespace nfgs
@@ -602,4 +610,26 @@ extern "C" int nfgs_dirichlet_solve(const nfgs_plan* p, const nfgs_boundary* bnd
   if (p->d.dtype == NFGS_F32)
     return nfgs::dirichlet_t<float>(p, bnd, src_hat, out_hat, scratch, nfgs::as_stream(stream));
   return nfgs::dirichlet_t<double>(p, bnd, src_hat, out_hat, scratch, nfgs::as_stream(stream));
+}
+
+extern "C" int nfgs_modal_robin_solve(const nfgs_plan* p, const void* src_hat,
+                                      const void* bottom_hat, const void* dtarget_hat,
+                                      const void* top_hat, double rho, int divide_by_lambda,
+                                      void* out_hat, void* scratch, double* zero_mode_sum,
+                                      void* stream) {
+  NFGS_REQUIRE(p && src_hat && bottom_hat && out_hat && scratch, NFGS_ERR_ARG,
+               "modal_robin_solve: null argument");
+  NFGS_REQUIRE(rho >= 0.0 && rho <= 1.0, NFGS_ERR_ARG,
+               "bottom_robin_weight must be between 0 and 1");
+  NFGS_REQUIRE(rho == 0.0 || dtarget_hat, NFGS_ERR_ARG,
+               "bottom_derivative_hat is required for mixed bottom Robin");
+  NFGS_REQUIRE(src_hat != out_hat && out_hat != scratch, NFGS_ERR_ARG,
+               "modal_robin_solve: buffers must be distinct");
+  NFGS_REQUIRE(p->d.nz >= 3, NFGS_ERR_ARG, "modal_robin_solve: nz must be >= 3");
+  hipStream_t s = nfgs::as_stream(stream);
+  if (p->d.dtype == NFGS_F32)
+    return nfgs::robin_t<float>(p, bottom_hat, dtarget_hat, top_hat, src_hat, out_hat, scratch,
+                                zero_mode_sum, rho, divide_by_lambda != 0, s);
+  return nfgs::robin_t<double>(p, bottom_hat, dtarget_hat, top_hat, src_hat, out_hat, scratch,
+                               zero_mode_sum, rho, divide_by_lambda != 0, s);
 }

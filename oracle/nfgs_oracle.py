@@ -96,14 +96,74 @@ def _move(x, axis):
     return np.moveaxis(x, axis, -1)
 
 
+# Threads of the real FFTs behind the type-I transforms: None keeps numpy's
+# single-threaded rfft (the setting every parity test uses); a count n > 1
+# switches to scipy.fft.rfft(..., workers=n), the same pocketfft algorithm on
+# n threads, for the timed CPU baseline.
+_FFT_WORKERS = None
+
+
+def set_fft_workers(n):
+    """Use n threads for the transforms' FFTs (None or 1: numpy, one thread)."""
+    global _FFT_WORKERS
+    _FFT_WORKERS = None if n is None or int(n) <= 1 else int(n)
+
+
+def _rfft(y):
+    if _FFT_WORKERS is None:
+        return np.fft.rfft(y, axis=-1)
+    import scipy.fft
+    return scipy.fft.rfft(y, axis=-1, workers=_FFT_WORKERS)
+
+
+# Arithmetic of the type-I transforms: "fft" (mirrored real FFTs, the
+# reference's form and the default) or "dense" (products with the closed-form
+# matrices through BLAS).  The two are the same transforms in independent
+# FP64 roundings; the spread between whole solves run each way is the FP64
+# floor that conditioning alone sets at a grid size.
+_TRANSFORM = "fft"
+_DENSE = {}
+
+
+def set_transform(kind):
+    """Select "fft" (default) or "dense" transform arithmetic."""
+    global _TRANSFORM
+    if kind not in ("fft", "dense"):
+        raise ValueError("transform arithmetic must be 'fft' or 'dense'")
+    _TRANSFORM = kind
+
+
+def _dense_matrix(kind, n):
+    key = (kind, n)
+    if key not in _DENSE:
+        j = np.arange(n)[None, :]
+        k = np.arange(n)[:, None]
+        if kind == "dct":
+            m = 2.0 * np.cos(np.pi * j * k / (n - 1))
+            m[:, 0] = 1.0
+            m[:, -1] = (-1.0) ** np.arange(n)
+        else:
+            m = 2.0 * np.sin(np.pi * (j + 1) * (k + 1) / (n + 1))
+        _DENSE[key] = m.T.copy()
+    return _DENSE[key]
+
+
+def _dense(kind, x, axis):
+    x = np.asarray(x)
+    xm = np.moveaxis(x, axis, -1)
+    return np.moveaxis(xm @ _dense_matrix(kind, xm.shape[-1]), -1, axis).astype(x.dtype)
+
+
 def dst1(x, axis):
     """Unnormalised DST-I, nfgs_solver.py:1019-1045."""
+    if _TRANSFORM == "dense":
+        return _dense("dst", x, axis)
     x = _move(np.asarray(x), axis)
     n = x.shape[-1]
     y = np.zeros(x.shape[:-1] + (2 * (n + 1),), dtype=x.dtype)
     y[..., 1:n + 1] = x
     y[..., n + 2:] = -x[..., ::-1]
-    Y = np.fft.rfft(y, axis=-1)
+    Y = _rfft(y)
     return np.moveaxis((-Y.imag[..., 1:n + 1]).astype(x.dtype), -1, axis)
 
 
@@ -115,6 +175,8 @@ def idst1(X, axis):
 
 def dct1(x, axis):
     """Unnormalised DCT-I, nfgs_solver.py:1052-1074."""
+    if _TRANSFORM == "dense" and np.asarray(x).shape[axis] > 1:
+        return _dense("dct", x, axis)
     x = _move(np.asarray(x), axis)
     n = x.shape[-1]
     if n == 1:
@@ -122,7 +184,7 @@ def dct1(x, axis):
     y = np.empty(x.shape[:-1] + (2 * (n - 1),), dtype=x.dtype)
     y[..., :n] = x
     y[..., n:] = x[..., n - 2:0:-1]
-    Y = np.fft.rfft(y, axis=-1)
+    Y = _rfft(y)
     return np.moveaxis(Y.real[..., :n].astype(x.dtype), -1, axis)
 
 
@@ -217,6 +279,48 @@ def d2_dydz(f, dy, dz):
         return d2
     d2[1:-1, 1:-1, :] = (f[2:, 2:, :] - f[:-2, 2:, :] - f[2:, :-2, :] + f[:-2, :-2, :]) / (4.0 * dy * dz)
     return d2
+
+
+def d2_dz2(f, dz):
+    """Second z derivative, nfgs_solver.py:1717-1727: centred inside; at the
+    bottom the four-point one-sided form (two-point below 4 planes); at the
+    top the mirrored form."""
+    d2 = np.zeros_like(f)
+    dz2 = dz * dz
+    if f.shape[0] >= 3:
+        d2[1:-1] = (f[2:] - 2 * f[1:-1] + f[:-2]) / dz2
+    if f.shape[0] >= 2:
+        if f.shape[0] >= 4:
+            d2[0] = (2.0 * f[0] - 5.0 * f[1] + 4.0 * f[2] - f[3]) / dz2
+        else:
+            d2[0] = (f[1] - f[0]) / dz2
+        d2[-1] = (-2.0 * f[-1] + 2.0 * f[-2]) / dz2
+    return d2
+
+
+def reconstruct_pt(phi, az, bcz, b_pot, bz_obs, jz_bottom, dx, dy, dz):
+    """Collocated PT state from the scalars with ccu_pure_current
+    (nfgs_solver.py:3311-3341, :1711-1715, :1760-1768; boundary values
+    :6069-6078, :3255-3257).  Returns (B, J, B_c)."""
+    dphi = d_dz(phi, dz)
+    bcx = -d_dx(dphi, dx) + d_dy(az, dy)
+    bcy = -d_dy(dphi, dy) - d_dx(az, dx)
+    daz = d_dz(az, dz)
+    lphi = bcz + d2_dz2(phi, dz)
+    jx = d_dx(daz, dx) + d_dy(lphi, dy)
+    jy = d_dy(daz, dy) - d_dx(lphi, dx)
+    jz = -lap_xy(az, dx, dy)
+    Bc = np.stack([bcx, bcy, bcz])
+    J = np.stack([jx, jy, jz])
+    B = b_pot + Bc
+    J[:, :, :, 0] = 0.0
+    J[:, :, :, -1] = 0.0
+    J[:, :, 0, :] = 0.0
+    J[:, :, -1, :] = 0.0
+    J[:, -1] = 0.0
+    B[2, 0] = bz_obs
+    J[2, 0] = jz_bottom
+    return B, J, Bc
 
 
 def pad_or_crop_center_2d(a2, ty, tx):

@@ -174,10 +174,14 @@ def test_iterate_matches_reference_trajectory(path):
         for k, v in meta["source_diagnostics"].items():
             assert abs(sd[k] - v) <= 1e-7 * max(1.0, abs(v)), k
     else:
+        # The FP32 golden runs are fixed-length: 30 steps with sigma still
+        # falling at every one (stop max_iter, best at 30), so the stop
+        # decisions and the selected fields are compared unconditionally.
+        assert (h["stop_reason"], h["best_iter"], h["final_iter"]) == \
+            (meta["stop_reason"], meta["best_iter"], it["max_iter"])
         fields = s.fields_to_numpy()
-        if h["best_iter"] == meta["best_iter"]:
-            for k in ("Bx", "By", "Bz"):
-                assert rel_l2(fields[k], g["out_" + k]) < tol["B"] * 5, k
+        for k in ("Bx", "By", "Bz"):
+            assert rel_l2(fields[k], g["out_" + k]) < tol["B"] * 5, k
 
 
 def test_unsupported_options_raise():

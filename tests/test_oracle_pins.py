@@ -202,3 +202,32 @@ def test_gaussian_split_invariants():
     h1 = O.dct2(s.B_background[2, 2] - s.Bz_open_mean)
     mask = np.abs(h0) > 1e-8
     np.testing.assert_allclose((h1[mask] / h0[mask]), np.exp(-s.k[mask] * s.dz), rtol=1e-8)
+
+
+@pytest.mark.parametrize("n", [3, 9, 66, 130])
+def test_dense_and_threaded_transform_arithmetic_match_the_fft_form(n):
+    """The oracle's alternative arithmetic for the type-I transforms (dense
+    closed-form matrices; threaded FFTs) computes the same transforms: equal
+    to the mirrored-FFT form to FP64 roundoff and to scipy's type-1 routines."""
+    import scipy.fft as sf
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((4, n, n + 1))
+    ref = {(k, a): f(x, a) for k, f in (("c", O.dct1), ("s", O.dst1)) for a in (-1, -2)}
+    try:
+        for mode in ("dense", "threads"):
+            if mode == "dense":
+                O.set_transform("dense")
+            else:
+                O.set_transform("fft")
+                O.set_fft_workers(4)
+            for (k, a), want in ref.items():
+                got = (O.dct1 if k == "c" else O.dst1)(x, a)
+                scale = np.abs(want).max()
+                assert np.abs(got - want).max() <= 1e-13 * scale, (mode, k, a)
+                lib = (sf.dct if k == "c" else sf.dst)(x, type=1, axis=a)
+                assert np.abs(got - lib).max() <= 1e-13 * scale, (mode, k, a)
+    finally:
+        O.set_transform("fft")
+        O.set_fft_workers(None)
+    with pytest.raises(ValueError):
+        O.set_transform("fp32")
