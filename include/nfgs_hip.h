@@ -414,6 +414,18 @@ int nfgs_field_scores(const double* b_ref, const double* b, const double* j, con
                       int nz, int ny, int nx, int z0, int z1, int y0, int y1, int x0, int x1,
                       double b2min, double* workspace, double* out, void* stream);
 
+/* nfgs_field_scores without a workspace argument: the per-block partials
+ * live in a device array of the library (nfgs_field_scores_static_capacity()
+ * doubles, 88 KiB, held on each GPU it runs on until the process ends), so
+ * the call allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  Results equal nfgs_field_scores'.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU race. */
+long nfgs_field_scores_static_capacity(void);
+int nfgs_field_scores_static(const double* b_ref, const double* b, const double* j,
+                             const double* div, int nz, int ny, int nx, int z0, int z1, int y0,
+                             int y1, int x0, int x1, double b2min, double* out, void* stream);
+
 /* Relative error of an outward bottom field against a target, for the CCI
  * comparison.  b (target) and c (candidate) are three planes (x, y, z
  * components) of ny*nx FP64.  Over the window [y0,y1) x [x0,x1):

@@ -106,6 +106,10 @@ def _declare(lib):
         "nfgs_field_scores": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, vp, vp,
                                         vp]),
+        "nfgs_field_scores_static_capacity": (C.c_long, []),
+        "nfgs_field_scores_static": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                               C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                               C.c_double, vp, vp]),
         "nfgs_outward_errors": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                           C.c_int, P(C.c_double), C.c_int, C.c_double, vp, vp]),
         "nfgs_source_diagnostics_workspace_bytes": (C.c_long, [vp]),

@@ -7,6 +7,8 @@
 // reference evaluates them ("common_fd2_float64_full_volume_then_crop").
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "nfgs_common.h"
 
@@ -261,6 +263,75 @@ extern "C" int nfgs_outward_errors(const double* b, const double* c, int ny, int
   for (int q = 0; q < ncut; ++q) cuts.frac[q] = frac[q];
   hipLaunchKernelGGL(outward_errors_kernel, dim3(1), dim3(kOutwardThreads), 0, as_stream(stream),
                      b, c, ny, nx, y0, x0, y1 - y0, x1 - x0, cuts, eps, out);
+  NFGS_LAUNCH_CHECK();
+  return NFGS_OK;
+}
+
+// ---- field scores without a workspace argument ----------------------------
+//
+// nfgs_field_scores_static runs the kernels of nfgs_field_scores with their
+// per-block partials in `scratch`, a device array of this library: it
+// allocates nothing, for callers that cannot allocate (code under graph
+// capture, tight serving loops).  The partials of a launch are written for
+// blocks [0, nb) before the final kernel reads exactly those, so nothing needs
+// clearing between calls; calls that overlap on one GPU share the array and
+// race.
+
+static __device__ __align__(256) double scratch[(long)nfgs::kScores * nfgs::kScoreBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device.  Host code uses this address only; the array's own name is a host
+// stand-in and must never be passed to a kernel or a copy.
+int scratch_address(double** out) {
+  int dev = -1;
+  NFGS_HIP_CHECK(hipGetDevice(&dev));
+  NFGS_REQUIRE(dev >= 0 && dev < kMaxDevices, NFGS_ERR_ARG,
+               "field_scores_static: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      nfgs::set_error(std::string("field_scores_static: hipGetSymbolAddress: ") +
+                      hipGetErrorString(e));
+      return NFGS_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return NFGS_OK;
+}
+}  // namespace
+
+extern "C" long nfgs_field_scores_static_capacity(void) {
+  return (long)kScores * kScoreBlocks;
+}
+
+extern "C" int nfgs_field_scores_static(const double* b_ref, const double* b, const double* j,
+                                        const double* div, int nz, int ny, int nx, int z0, int z1,
+                                        int y0, int y1, int x0, int x1, double b2min, double* out,
+                                        void* stream) {
+  NFGS_REQUIRE(b && out, NFGS_ERR_ARG, "field_scores_static: bad argument");
+  NFGS_REQUIRE(0 <= z0 && z0 < z1 && z1 <= nz && 0 <= y0 && y0 < y1 && y1 <= ny && 0 <= x0 &&
+                   x0 < x1 && x1 <= nx,
+               NFGS_ERR_ARG, "field_scores_static: the box is empty or outside the field");
+  const long M = (long)(z1 - z0) * (y1 - y0) * (x1 - x0);
+  const long nb = std::min<long>(kScoreBlocks, (M + 255) / 256);
+  NFGS_REQUIRE(nb * kScores <= nfgs_field_scores_static_capacity(), NFGS_ERR_ARG,
+               "field_scores_static: the partials exceed the static scratch");
+  double* part = nullptr;
+  const int rc = scratch_address(&part);
+  if (rc != NFGS_OK) return rc;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(scores_kernel, dim3((unsigned)nb), dim3(256), 0, s, b_ref, b, j, div, nz, ny,
+                     nx, z0, y0, x0, z1 - z0, y1 - y0, x1 - x0, b2min, part);
+  NFGS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(scores_final_kernel, dim3(1), dim3(256), 0, s, part, (int)nb, out);
   NFGS_LAUNCH_CHECK();
   return NFGS_OK;
 }
