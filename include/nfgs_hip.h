@@ -90,6 +90,10 @@ typedef struct {
 } nfgs_workspace;
 
 const char* nfgs_last_error(void);
+/* SHA-256 (hex) of the sources this library was compiled from; the Python
+ * loader refuses a library whose digest differs from the sources beside it.
+ * No counterpart in the reference (a build-consistency check). */
+const char* nfgs_source_digest(void);
 
 /* Plan: transform matrices (paired DCT-I/DST-I halves), lambda/k tables.
  * Replaces the constructor's table caches (nfgs_solver.py:1083-1118,
@@ -108,6 +112,18 @@ long nfgs_plan_table_size(const nfgs_plan* plan, int which);
  * dst.  Table 0 replaces _endpoint_neumann_lam_xy (nfgs_solver.py:1098-1109),
  * table 2 _dirichlet_lam_xy_interior (:1083-1096). */
 int nfgs_copy_plan_table(const nfgs_plan* plan, int which, void* dst, void* stream);
+
+/* Launch tuning of a plan (a project autotuner picks the values per grid;
+ * the reference has no counterpart).  NFGS_TUNE_ZCHUNKS: number of z-chunks
+ * the tiled stencils split the grid into (0: the built-in cost model; a
+ * count whose last chunk would hold fewer than two planes is rejected).
+ * Chunking changes only the order in which the monitor partials are summed.
+ * NFGS_TUNE_TRANSFORM_BLOCKS_PER_CU: workgroups per CU of the FP32 transform
+ * (0: as many as fit; results do not depend on it). */
+enum { NFGS_TUNE_ZCHUNKS = 0, NFGS_TUNE_TRANSFORM_BLOCKS_PER_CU = 1 };
+int nfgs_plan_set_tuning(nfgs_plan* plan, int knob, int value);
+/* Current value of a knob (-1 for an unknown knob). */
+int nfgs_plan_get_tuning(const nfgs_plan* plan, int knob);
 
 /* Unnormalised type-I transform (forward) or its inverse along one axis of a
  * batch of vectors: vector g = p*nvec + q starts at p*plane_stride +

@@ -30,6 +30,8 @@ SD_COUNT = 10
 SC_COUNT = 11
 # nfgs_advance_ex poloidal update modes (NFGS_POLOIDAL_*)
 POLOIDAL_DIRECT, POLOIDAL_TWO_STAGE = 0, 1
+# nfgs_plan_set_tuning knobs (NFGS_TUNE_*)
+TUNE_ZCHUNKS, TUNE_TRANSFORM_BLOCKS_PER_CU = 0, 1
 
 vp = C.c_void_p
 
@@ -66,6 +68,9 @@ def _declare(lib):
         "nfgs_partials_count": (C.c_long, [vp]),
         "nfgs_plan_table": (vp, [vp, C.c_int]),
         "nfgs_plan_table_size": (C.c_long, [vp, C.c_int]),
+        "nfgs_source_digest": (C.c_char_p, []),
+        "nfgs_plan_set_tuning": (C.c_int, [vp, C.c_int, C.c_int]),
+        "nfgs_plan_get_tuning": (C.c_int, [vp, C.c_int]),
         "nfgs_copy_plan_table": (C.c_int, [vp, C.c_int, vp, vp]),
         "nfgs_modal_robin_solve": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp,
                                              vp]),
@@ -130,10 +135,19 @@ def load():
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing; build it with `python -m nfgs_amd.build`")
     try:
-        _lib = _declare(C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | getattr(os, "RTLD_GLOBAL", 0)))
-    except OSError as exc:  # pragma: no cover - depends on the machine
+        lib = _declare(C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | getattr(os, "RTLD_GLOBAL", 0)))
+    except (OSError, AttributeError) as exc:  # pragma: no cover - depends on the machine
         _load_error = exc
-        raise RuntimeError(f"cannot load {LIB_PATH}: {exc}") from exc
+        raise RuntimeError(f"cannot load {LIB_PATH} (rebuild it with `python -m nfgs_amd.build`): "
+                           f"{exc}") from exc
+    # the library must be built from the sources beside it
+    from .build import source_digest
+    built = lib.nfgs_source_digest().decode()
+    if built != source_digest():
+        raise RuntimeError(f"{LIB_PATH} was built from other sources than those in "
+                           f"{LIB_PATH.parent / 'csrc'} (library {built[:12]}, sources "
+                           f"{source_digest()[:12]}); rebuild it with `python -m nfgs_amd.build`")
+    _lib = lib
     return _lib
 
 

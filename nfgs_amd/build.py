@@ -7,6 +7,7 @@ The library lands next to this file; the Python layer loads it from there.
 from __future__ import annotations
 
 import concurrent.futures as cf
+import hashlib
 import os
 import shutil
 import subprocess
@@ -17,6 +18,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "libnfgs_hip.so"
+PUBLIC_HEADER = HERE.parent / "include" / "nfgs_hip.h"
 ARCH = os.environ.get("NFGS_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -38,9 +40,24 @@ def sources():
     return sorted(CSRC.glob("*.hip"))
 
 
-def _compile(src: Path) -> Path:
+def source_digest() -> str:
+    """SHA-256 over every file the library is compiled from (the kernels,
+    their headers and the public header), by name and content.  The build
+    bakes it into the library (nfgs_source_digest) and the loader compares
+    the two, so a library older than its sources is never loaded."""
+    h = hashlib.sha256()
+    files = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.h")) + [PUBLIC_HEADER]
+    for f in files:
+        h.update(f.name.encode() + b"\0")
+        h.update(f.read_bytes())
+        h.update(b"\0")
+    return h.hexdigest()
+
+
+def _compile(src: Path, digest: str) -> Path:
     obj = OBJ / (src.stem + ".o")
-    cmd = [_hipcc(), *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)]
+    cmd = [_hipcc(), *HIPCC_FLAGS, f'-DNFGS_SOURCE_DIGEST="{digest}"', "-c", str(src), "-o",
+           str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src.name}:\n{res.stderr[-4000:]}")
@@ -50,8 +67,9 @@ def _compile(src: Path) -> Path:
 def build(verbose: bool = True) -> Path:
     OBJ.mkdir(exist_ok=True)
     srcs = sources()
+    digest = source_digest()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda src: _compile(src, digest), srcs))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)

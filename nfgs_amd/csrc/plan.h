@@ -1,6 +1,7 @@
 // Internal plan layout shared by the NFGS kernels' launchers.
 #pragma once
 
+#include "launch_shape.h"
 #include "nfgs_common.h"
 
 constexpr int kRedScratch = 4096;
@@ -21,6 +22,7 @@ struct PairedMatrix {
   // laid out as the MFMA operand registers of each block; entries are
   // round(T * 2^shift).
   int nch = 0, tail = 0, hb = 0, rem = 0;
+  int blocks_per_cu = 0;  // launch override of the FP32 kernel (0: as many as fit)
   int shift[2] = {0, 0};
   void* dig[2] = {nullptr, nullptr};    // [direction] int8 [half][digit][chunk][block][lane][16]
   void* mtail[2] = {nullptr, nullptr};  // [direction] double [half][16*hb] tail row
@@ -39,6 +41,7 @@ struct nfgs_plan {
   void* lamd = nullptr;  // (ny-2)*(nx-2) endpoint-Dirichlet eigenvalues
   void* ax = nullptr;    // nx   2 sin(pi k / (2(nx-1))) / dx
   void* ay = nullptr;    // ny   2 sin(pi k / (2(ny-1))) / dy
+  int zchunks = 0;       // z-chunk count of the tiled stencils (0: the cost model)
   int pblocks = 0;       // blocks of the reducing kernels (partials per slot)
   double* red = nullptr; // kRedScratch doubles of reduction scratch (not re-entrant)
 };
@@ -46,6 +49,9 @@ struct nfgs_plan {
 namespace nfgs {
 const PairedMatrix* find_matrix(const nfgs_plan* p, int kind, int n);
 int fill_zero(const nfgs_plan* p, void* ptr, long n, hipStream_t s);
+// CUs of the current device (cached on first use)
+int cu_count_stencil();
+// Workgroups of a tiled stencil launch with the plan's current z-chunking.
 long tiled_blocks(const nfgs_plan* p);
 int fill_zero_f64(double* ptr, long n, hipStream_t s);
 int launch_transform(const nfgs_plan* p, const PairedMatrix* m, int direction, const void* in,

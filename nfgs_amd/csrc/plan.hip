@@ -188,6 +188,12 @@ const PairedMatrix* find_matrix(const nfgs_plan* p, int kind, int n) {
 
 extern "C" const char* nfgs_last_error(void) { return nfgs::g_last_error.c_str(); }
 
+// SHA-256 of the sources the library was compiled from (set by the build).
+#ifndef NFGS_SOURCE_DIGEST
+#define NFGS_SOURCE_DIGEST "unknown"
+#endif
+extern "C" const char* nfgs_source_digest(void) { return NFGS_SOURCE_DIGEST; }
+
 extern "C" int nfgs_plan_destroy(nfgs_plan* p) {
   if (!p) return NFGS_OK;
   nfgs::free_paired(&p->dct_x);
@@ -227,7 +233,9 @@ extern "C" int nfgs_plan_create(const nfgs_grid_desc* d, nfgs_plan** out) {
   p->iplane = (long)(d->ny - 2) * (d->nx - 2);
   p->N = p->plane * d->nz;
   p->esize = d->dtype == NFGS_F32 ? 4 : 8;
-  p->pblocks = (int)tiled_blocks(p);
+  // Partials are laid out for the largest block count any admissible
+  // z-chunking launches (the cost model's or a tuned count's).
+  p->pblocks = (int)shape::partial_capacity(d->nx, d->ny, d->nz, cu_count_stencil());
 
   int rc = build_paired(d->dtype, NFGS_DCT1, d->nx, &p->dct_x);
   if (!rc) rc = build_paired(d->dtype, NFGS_DCT1, d->ny, &p->dct_y);
@@ -325,4 +333,34 @@ extern "C" int nfgs_copy_plan_table(const nfgs_plan* p, int which, void* dst, vo
   NFGS_HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)nfgs_plan_table_size(p, which) * p->esize,
                                 hipMemcpyDeviceToDevice, nfgs::as_stream(stream)));
   return NFGS_OK;
+}
+
+extern "C" int nfgs_plan_set_tuning(nfgs_plan* p, int knob, int value) {
+  NFGS_REQUIRE(p, NFGS_ERR_ARG, "plan_set_tuning: null plan");
+  NFGS_REQUIRE(value >= 0, NFGS_ERR_ARG, "plan_set_tuning: value must be >= 0");
+  if (knob == NFGS_TUNE_ZCHUNKS) {
+    NFGS_REQUIRE(value == 0 || nfgs::shape::zchunks_fit(p->d.nz, value), NFGS_ERR_ARG,
+                 "plan_set_tuning: z-chunk count does not fit the grid");
+    const int old = p->zchunks;
+    p->zchunks = value;
+    if (nfgs::tiled_blocks(p) > p->pblocks) {  // not for counts zchunks_fit admits
+      p->zchunks = old;
+      nfgs::set_error("plan_set_tuning: more blocks than the partials hold");
+      return NFGS_ERR_ARG;
+    }
+    return NFGS_OK;
+  }
+  if (knob == NFGS_TUNE_TRANSFORM_BLOCKS_PER_CU) {
+    for (PairedMatrix* m : {&p->dct_x, &p->dct_y, &p->dst_x, &p->dst_y}) m->blocks_per_cu = value;
+    return NFGS_OK;
+  }
+  nfgs::set_error("plan_set_tuning: unknown knob");
+  return NFGS_ERR_ARG;
+}
+
+extern "C" int nfgs_plan_get_tuning(const nfgs_plan* p, int knob) {
+  if (!p) return -1;
+  if (knob == NFGS_TUNE_ZCHUNKS) return p->zchunks;
+  if (knob == NFGS_TUNE_TRANSFORM_BLOCKS_PER_CU) return p->dct_x.blocks_per_cu;
+  return -1;
 }

@@ -10,7 +10,7 @@
 namespace nfgs {
 namespace {
 
-constexpr int TX = 32, TY = 8, HH = 2;
+constexpr int TX = shape::TX, TY = shape::TY, HH = 2;
 constexpr int PW = TX + 2 * HH, PH = TY + 2 * HH, PS = PW * PH;
 
 // J.B / |B|^2 of the alpha projection: FP32 takes the hardware reciprocal
@@ -30,41 +30,10 @@ struct Grid {
   int kc;        // planes per z-chunk
 };
 
-int cu_count_stencil() {
-  static int cus = 0;
-  if (cus == 0) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        cus <= 0)
-      cus = 256;
-  }
-  return cus;
-}
-
-// Planes per z-chunk of the tiled stencils.  A chunk re-stages about four
-// planes (its Phi ring and the face below it), and the blocks of a launch
-// share the CUs evenly, so the launch takes about
-// ceil(blocks / CUs) * (chunk + 4) plane-steps on the busiest CU; the
-// chunk count minimises that.  The last chunk keeps at least two planes:
-// the top node's one-sided dBz/dz takes Bz of planes nz-1, nz-2 and nz-3
-// from the chunk's own z-march.
+// Planes per z-chunk of the tiled stencils: the plan's tuned count or the
+// cost model (launch_shape.h).
 int z_chunk(const nfgs_plan* p) {
-  const int nx = p->d.nx, ny = p->d.ny, nz = p->d.nz;
-  const long xy = (long)((nx + (nx % TX == 1) + TX - 1) / TX) * ((ny + (ny % TY == 1) + TY - 1) / TY);
-  const long cus = cu_count_stencil();
-  long best = -1;
-  int kc = nz;
-  for (int nch = 1; nch <= (nz + 3) / 4; ++nch) {
-    const int c = (nz + nch - 1) / nch;
-    if (nch > 1 && nz - (nch - 1) * c < 2) continue;
-    const long cost = ((xy * nch + cus - 1) / cus) * (c + 4);
-    if (best < 0 || cost < best) {
-      best = cost;
-      kc = c;
-    }
-  }
-  return kc;
+  return shape::chunk_len(p->d.nx, p->d.ny, p->d.nz, p->zchunks, cu_count_stencil());
 }
 
 template <typename T>
@@ -1042,14 +1011,17 @@ __global__ __launch_bounds__(256) void sources_kernel(Grid<T> g, const T* __rest
 // to kRedBatch * 1024 blocks); the sum order is fixed.
 constexpr int kRedBatch = 4;
 static_assert(kRedBatch == 4, "the batch is summed pairwise below");
+// Sums slot q's partials of the `active` blocks of the last launch (stride
+// pstride between slots; the zero-mode slot holds one value).
 __global__ __launch_bounds__(1024) void reduce_kernel(const double* __restrict__ partials,
-                                                      long pstride, double* __restrict__ metrics) {
+                                                      long pstride, long active,
+                                                      double* __restrict__ metrics) {
   __shared__ double red[NFGS_NQ * 16];
   double acc[NFGS_NQ];
   const long nt = blockDim.x;
 #pragma unroll
   for (int q = 0; q < NFGS_NQ; ++q) {
-    const long cnt = q == NFGS_Q_ZERO_MODE ? 1 : pstride;
+    const long cnt = q == NFGS_Q_ZERO_MODE ? 1 : active;
     const double* src = partials + q * pstride;
     double v[kRedBatch];
 #pragma unroll
@@ -1075,9 +1047,18 @@ __global__ void fill_kernel(T* __restrict__ p, long n, T v) {
 }
 
 dim3 tiled_grid(const nfgs_plan* p) {
-  const int sx = p->d.nx % TX == 1 ? 1 : 0, sy = p->d.ny % TY == 1 ? 1 : 0;
-  return dim3((unsigned)(((p->d.nx + sx + TX - 1) / TX) * ((p->d.ny + sy + TY - 1) / TY) *
-                         ((p->d.nz + z_chunk(p) - 1) / z_chunk(p))));
+  return dim3((unsigned)shape::blocks(p->d.nx, p->d.ny, p->d.nz, p->zchunks, cu_count_stencil()));
+}
+
+// A launch that writes monitor partials must fit the plan's partials (one
+// value per block and slot); nfgs_plan_set_tuning admits only chunkings that
+// do, so this fails only on a plan whose layout was changed behind it.
+int check_partials(const nfgs_plan* p, const double* partials) {
+  if (partials && (long)tiled_grid(p).x > p->pblocks) {
+    set_error("tiled launch has more blocks than the plan's partials hold");
+    return NFGS_ERR_STATE;
+  }
+  return NFGS_OK;
 }
 
 template <typename T>
@@ -1094,6 +1075,7 @@ int reconstruct_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi,
 template <typename T>
 int face_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* b, const void* az,
            const void* jc, void* jp, void* jout, double* partials, hipStream_t s) {
+  if (const int rc = check_partials(p, partials)) return rc;
   const Grid<T> g = make_grid<T>(p);
   const Window w{p->d.win_z0, p->d.win_z1, p->d.win_y0, p->d.win_y1, p->d.win_x0, p->d.win_x1};
   hipLaunchKernelGGL((face_pass_kernel<T>), tiled_grid(p), dim3(TX * TY), 0, s, g,
@@ -1109,6 +1091,7 @@ int face_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* b, const vo
 template <typename T>
 int state_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi, const void* az,
             void* b, void* jp, void* jout, double* partials, hipStream_t s) {
+  if (const int rc = check_partials(p, partials)) return rc;
   const Grid<T> g = make_grid<T>(p);
   const Window w{p->d.win_z0, p->d.win_z1, p->d.win_y0, p->d.win_y1, p->d.win_x0, p->d.win_x1};
   auto kern = (jp && !jout && partials) ? state_pass_kernel<T, true> : state_pass_kernel<T, false>;
@@ -1125,6 +1108,7 @@ int state_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi, const
 template <typename T>
 int sources_t(const nfgs_plan* p, const void* jp, void* sb, void* sj, double* partials,
               hipStream_t s) {
+  if (const int rc = check_partials(p, partials)) return rc;
   const Grid<T> g = make_grid<T>(p);
   hipLaunchKernelGGL((sources_kernel<T>), tiled_grid(p), dim3(TX * TY), 0, s, g,
                      static_cast<const T*>(jp), static_cast<T*>(sb), static_cast<T*>(sj),
@@ -1135,6 +1119,18 @@ int sources_t(const nfgs_plan* p, const void* jp, void* sb, void* sj, double* pa
 }
 
 }  // namespace
+
+int cu_count_stencil() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+  }
+  return cus;
+}
 
 long tiled_blocks(const nfgs_plan* p) {
   const dim3 d = tiled_grid(p);
@@ -1219,8 +1215,9 @@ extern "C" int nfgs_sources(const nfgs_plan* p, const void* jp, void* s_bcz, voi
 extern "C" int nfgs_reduce(const nfgs_plan* p, const double* partials, double* metrics,
                            void* stream) {
   NFGS_REQUIRE(p && partials && metrics, NFGS_ERR_ARG, "reduce: null argument");
+  if (const int rc = check_partials(p, partials)) return rc;
   hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, as_stream(stream), partials,
-                     (long)p->pblocks, metrics);
+                     (long)p->pblocks, tiled_blocks(p), metrics);
   NFGS_LAUNCH_CHECK();
   return NFGS_OK;
 }

@@ -1022,7 +1022,8 @@ int launch_i8_t(const PairedMatrix* m, int dir, const float* in, long ies, long 
     per_cu = nb;
   }
   const int ntiles = (total + 15) / 16;
-  const int grid = std::min(ntiles, per_cu * cu_count());
+  const int pcu = m->blocks_per_cu > 0 ? std::min(m->blocks_per_cu, per_cu) : per_cu;
+  const int grid = std::min(ntiles, pcu * cu_count());
   hipLaunchKernelGGL((paired_transform_i8<EC, NCH, TAIL, REM, FULL>), dim3(grid), dim3(512), 0, s,
                      in, ies, ivs, ips, out, oes, ovs, ops,
                      static_cast<const i32x4*>(m->dig[dir]),

@@ -217,6 +217,121 @@ class _Options:
             _unsupported("bottom_jperp_zero=False")
 
 
+class _Driver:
+    """Per-step bookkeeping of the reference driver at mu = 1
+    (nfgs_solver.py:3639-3814): the history lists, the best state, the
+    improvement and stall counters, the fixed-point streak and the stop
+    rules, in the reference's order.  iterate() and the batched runner share
+    it, so both stop exactly where the reference does."""
+
+    KEYS = ("sigma", "divB", "energy", "mu", "rollback", "sigma_relative_change",
+            "energy_relative_change", "fixed_point_criteria_met")
+
+    def __init__(self, sigma0, E0, tol, patience, sigma_improvement_tol, stop_mode, fixed_point):
+        self.history = {k: [] for k in self.KEYS}
+        self.sigma0, self.E0 = sigma0, E0
+        self.tol, self.patience = float(tol), int(patience)
+        self.need = max(int(patience), 1)
+        self.use_imp = sigma_improvement_tol is not None and float(sigma_improvement_tol) > 0.0
+        self.eps = float(sigma_improvement_tol) if self.use_imp else 0.0
+        self.stop_mode = stop_mode
+        self.fp = fixed_point
+        self.best_sigma, self.best_iter = sigma0, 0
+        self.prev = sigma0
+        self.stall = self.flat = 0
+        self.last_imp = None
+        self.reason = "max_iter"
+        self.final_iter = 0
+        self.fp_streak = 0
+        self.fp_converged = False
+        self.map_residual = self.pde_residual = None
+
+    def record(self, n, sigma, divB, energy, diagnostics=None):
+        """History of step n and its fixed-point criteria (:3697-3734)."""
+        h = self.history
+        pe = h["energy"][-1] if h["energy"] else self.E0
+        src = abs(sigma - self.prev) / max(abs(self.prev), 1.0e-30)
+        erc = abs(energy - pe) / max(abs(pe), 1.0e-30)
+        h["sigma"].append(sigma)
+        h["divB"].append(divB)
+        h["energy"].append(energy)
+        h["mu"].append(1.0)
+        h["rollback"].append(0)
+        h["sigma_relative_change"].append(src)
+        h["energy_relative_change"].append(erc)
+        d = diagnostics or {}
+        self.map_residual = max(float(d.get("phi_map_residual_relative", math.inf)),
+                                float(d.get("az_map_residual_relative", math.inf)),
+                                float(d.get("h_k0_map_residual_relative", 0.0)))
+        self.pde_residual = max(float(d.get("poloidal_pde_residual_relative", math.inf)),
+                                float(d.get("toroidal_pde_residual_relative", math.inf)),
+                                float(d.get("mixed_row_residual_relative", 0.0)))
+        met = (self.stop_mode == "fixed_point" and math.isfinite(self.map_residual)
+               and math.isfinite(self.pde_residual)
+               and self.map_residual <= self.fp["update"] and self.pde_residual <= self.fp["pde"]
+               and src <= self.fp["observable"] and erc <= self.fp["observable"])
+        h["fixed_point_criteria_met"].append(int(met))
+        self.fp_streak = self.fp_streak + 1 if met else 0
+        self.final_iter = n
+
+    def accept(self, n, sigma) -> bool:
+        """Improvement counters and the best state (:3739-3758); True when
+        step n is the new best."""
+        imp = self.prev - sigma
+        self.last_imp = imp
+        self.prev = sigma
+        self.flat = self.flat + 1 if imp < self.eps else 0
+        if sigma < self.best_sigma:
+            self.best_sigma, self.best_iter = sigma, n
+            self.stall = 0
+            return True
+        self.stall += 1
+        return False
+
+    def check_stop(self, n, verbose=False) -> bool:
+        """The stop rules in the reference's order (:3766-3814)."""
+        sigma = self.history["sigma"][-1]
+        if self.stop_mode == "fixed_point":
+            if self.fp_streak >= self.fp["patience"]:
+                self.reason = "fixed_point"
+                self.fp_converged = True
+                if verbose:
+                    print(f"Fixed point attained at iteration {n}: map={self.map_residual:.4e}, "
+                          f"pde={self.pde_residual:.4e}, observable drift<="
+                          f"{self.fp['observable']:.4e} for {self.fp_streak} accepted steps")
+                return True
+            return False
+        if sigma < self.tol:
+            self.reason = "tol"
+            if verbose:
+                print(f"Converged at iteration {n}")
+            return True
+        if self.use_imp and self.flat >= self.need:
+            self.reason = "sigma_improvement_tol"
+            if verbose:
+                print(f"Early stop at iter {n}: sigma improvement flat for {self.flat} steps "
+                      f"(last {self.last_imp:.4e} < {self.eps:.4e}; best sigma="
+                      f"{self.best_sigma:.4e} at iter {self.best_iter})")
+            return True
+        if self.patience > 0 and self.stall >= self.patience:
+            self.reason = "sigma_stall"
+            if verbose:
+                print(f"Early stop at iter {n}: sigma not improving (best sigma="
+                      f"{self.best_sigma:.4e} at iter {self.best_iter})")
+            return True
+        return False
+
+    def summary(self) -> dict:
+        return dict(best_sigma=self.best_sigma, best_iter=self.best_iter, rollbacks=0,
+                    stop_reason=self.reason, stop_mode=self.stop_mode,
+                    selected_state=("final_accepted_iterate" if self.stop_mode == "fixed_point"
+                                    else "best_sigma"),
+                    final_iter=self.final_iter, fixed_point_converged=self.fp_converged,
+                    fixed_point_streak=self.fp_streak, fixed_point_tolerances=dict(self.fp),
+                    sigma_improvement_tol=self.eps if self.use_imp else None,
+                    auto_rbsl_improvement_grace=0, last_sigma_improvement=self.last_imp)
+
+
 class NFGSSolverGPU:
     """Gaussian-separated NFPT solver on MI355X (nfgs_solver.py:3905).
 
@@ -605,14 +720,53 @@ class NFGSSolverGPU:
         self._ws = L.Workspace(*(L.ptr(self._ws_t[k]) for k in
                                  ("jp", "s_bcz", "s_jz", "hat0", "hat1", "cscr")),
                                L.ptr(self._partials), L.ptr(self._metrics))
-        self._slots = []
-        for _ in range(3):
-            phi = self._zeros(Nz, Ny, Nx)
-            az = self._zeros(Nz, Ny, Nx)
-            phi[0].copy_(self.Phi_above_bottom)
-            az[0].copy_(self.Az_bottom)
-            self._slots.append((phi, az))
+        self._slots = [self._new_slot() for _ in range(3)]
         self._Bwork = self._empty(3, Nz, Ny, Nx)
+
+    def _new_slot(self):
+        """A (Phi_c, Az) state slot with the planes the iteration never
+        rewrites already set: Phi_c[0] = Phi_above_bottom, Az[0] = Az_bottom,
+        Az zero on the top plane and the lateral walls."""
+        phi = self._zeros(self.Nz, self.Ny, self.Nx)
+        az = self._zeros(self.Nz, self.Ny, self.Nx)
+        phi[0].copy_(self.Phi_above_bottom)
+        az[0].copy_(self.Az_bottom)
+        return phi, az
+
+    # attributes the Gaussian separation and the bottom-current setup leave;
+    # all of them are independent of bottom_divergence_robin_weight
+    _BOUNDARY_ATTRS = ("Bx_obs", "By_obs", "Bz_obs", "Jz_obs", "Az_bottom", "Jz_toroidal_boundary",
+                       "B_background", "B_pot", "Bz_above", "Phi_above_bottom",
+                       "_bottom_dphi_c_target", "D0", "_phi_above_hat", "_H_k0",
+                       "_H_k0_bottom_target", "_k0_diagnostics", "gaussian_separation_diagnostics",
+                       "toroidal_boundary_diagnostics", "B_mean", "Bz_open_mean",
+                       "fov_slice_yx", "_observed_shape_zyx", "_bottom_current_grid_mode")
+
+    def share_boundary(self, other: "NFGSSolverGPU"):
+        """Take the boundary setup of another solver of the same grid, dtype
+        and separation options instead of recomputing it: the Gaussian
+        separation and the bottom current do not depend on the
+        bottom-divergence Robin weight (nfgs_solver.py:4671-4919; the weight
+        enters only the Phi_c solve, :5657-5667), so a scan over that weight
+        sets the boundary once.  The arrays are shared, not copied; this
+        solver builds its own plan (its weight) and modal tables.  An
+        extension with no counterpart in the reference."""
+        if other.B_background is None:
+            raise RuntimeError("the other solver has no boundary set")
+        if (other.Nz, other.Ny, other.Nx) != (self.Nz, self.Ny, self.Nx) or \
+                other.dtype != self.dtype or other.device != self.device or \
+                (other.dx, other.dy, other.dz) != (self.dx, self.dy, self.dz):
+            raise ValueError("share_boundary needs the same grid, spacing, dtype and device")
+        for key in ("mean_bz_policy", "horizontal_zero_mode_policy", "d0_zero_mode_policy",
+                    "harmonic_residual_policy", "gaussian_horizontal_operator",
+                    "weak_field_floor_b2", "alpha_limit_factor", "mean_mode_tolerance"):
+            if getattr(other.opts, key) != getattr(self.opts, key):
+                raise ValueError(f"share_boundary needs the same {key}")
+        self._reset_state()
+        for key in self._BOUNDARY_ATTRS:
+            setattr(self, key, getattr(other, key))
+        self._make_plan()
+        self._bind_boundary()
 
     def _dct2_plane(self, src, dst, tmp):
         lib, s, plan = self._lib, self._stream, self._plan
@@ -676,9 +830,10 @@ class NFGSSolverGPU:
                                          self.source_zero_mode_policy,
                                          "The Phi_c nonlinear source has an incompatible zero mode")
 
-    def _state_fields(self, slot: int, B_out=None, J_out=None, face_diag: bool = False):
-        """Rebuild B and J of a slot (identical to the iteration's own)."""
-        phi, az = self._slots[slot]
+    def _state_fields(self, slot, B_out=None, J_out=None, face_diag: bool = False):
+        """Rebuild B and J of a slot index or a (Phi_c, Az) pair (identical to
+        the iteration's own)."""
+        phi, az = self._slots[slot] if isinstance(slot, int) else slot
         B = self._empty(3, self.Nz, self.Ny, self.Nx) if B_out is None else B_out
         J = self._empty(3, self.Nz, self.Ny, self.Nx) if J_out is None else J_out
         L.check(self._lib.nfgs_state_pass(self._plan, L.C.byref(self._bnd), L.ptr(phi), L.ptr(az),
@@ -1224,19 +1379,10 @@ class NFGSSolverGPU:
             am_text = f"{am:.3e}" if am is not None else "off"
             print(f"Initial  sigma={sigma0:.4e}  divB={divB0:.4e}  E={E0:.4e} "
                   f"(mu0={mu}, adaptive={adaptive_mu}, alpha_max={am_text}, toroidal_bc=True)")
-        history = {k: [] for k in ("sigma", "divB", "energy", "mu", "rollback",
-                                   "sigma_relative_change", "energy_relative_change",
-                                   "fixed_point_criteria_met")}
-        best_sigma, best_iter, best_slot = sigma0, 0, 0
-        final_slot = 0
-        stall = flat = 0
-        need = max(int(patience), 1)
-        prev = sigma0
-        use_imp = sigma_improvement_tol is not None and float(sigma_improvement_tol) > 0.0
-        eps = float(sigma_improvement_tol) if use_imp else 0.0
-        last_imp = None
-        reason = "max_iter"
-        final_iter = 0
+        drv = _Driver(sigma0, E0, tol, patience, sigma_improvement_tol, stop_mode,
+                      dict(update=update_tol, pde=float(fixed_point_pde_tol),
+                           observable=observable_tol, patience=int(fixed_point_patience)))
+        best_slot = final_slot = 0
         # The loop keeps one step in flight: step n+1 is enqueued before the
         # monitors of step n are read, so the device never waits on the host.
         # Step n+1 goes into a slot that is neither the best state nor step
@@ -1260,16 +1406,7 @@ class NFGSSolverGPU:
             sigma, divB, energy, ratio = self._take_metrics(n & 1)
             self._check_zero_mode(ratio)
             final_slot = slot
-            pe = history["energy"][-1] if history["energy"] else E0
-            history["sigma"].append(sigma)
-            history["divB"].append(divB)
-            history["energy"].append(energy)
-            history["mu"].append(1.0)
-            history["rollback"].append(0)
-            history["sigma_relative_change"].append(abs(sigma - prev) / max(abs(prev), 1.0e-30))
-            history["energy_relative_change"].append(abs(energy - pe) / max(abs(pe), 1.0e-30))
-            history["fixed_point_criteria_met"].append(0)
-            final_iter = n
+            drv.record(n, sigma, divB, energy)
             if per_step_callback is not None:
                 # the callback owns fresh B and J of this step, as the
                 # reference hands it the new state tensors (:3736-3737)
@@ -1279,40 +1416,22 @@ class NFGSSolverGPU:
                                                  L.ptr(B_cb), L.ptr(self._slots[slot][1]),
                                                  None, L.ptr(J_cb), None, self._stream), "J")
                 per_step_callback(n, sigma, B_cb, J_cb)
-            imp = prev - sigma
-            last_imp = imp
-            prev = sigma
-            flat = flat + 1 if imp < eps else 0
-            if sigma < best_sigma:
-                best_sigma, best_iter, best_slot = sigma, n, slot
-                stall = 0
-            else:
-                stall += 1
+            if drv.accept(n, sigma):
+                best_slot = slot
             if verbose and (n == 1 or n % 10 == 0):
                 print(f"Iter {n:4d} | sigma={sigma:.4e} | divB={divB:.4e} | E={energy:.4e} | "
                       f"mu={1.0:.3f}")
-            if sigma < tol:
-                reason = "tol"
-                if verbose:
-                    print(f"Converged at iteration {n}")
+            if drv.check_stop(n, verbose):
                 break
-            if use_imp and flat >= need:
-                reason = "sigma_improvement_tol"
-                if verbose:
-                    print(f"Early stop at iter {n}: sigma improvement flat for {flat} steps "
-                          f"(last {imp:.4e} < {eps:.4e}; best sigma={best_sigma:.4e} at iter "
-                          f"{best_iter})")
-                break
-            if patience > 0 and stall >= patience:
-                reason = "sigma_stall"
-                if verbose:
-                    print(f"Early stop at iter {n}: sigma not improving (best sigma="
-                          f"{best_sigma:.4e} at iter {best_iter})")
-                break
+        return self._finish(drv, self._slots[best_slot], self._slots[final_slot],
+                            best_slot == final_slot)
 
-        # selected (best) state, final-vs-best and the staggered diagnostic
-        final_sigma = history["sigma"][-1] if history["sigma"] else sigma0
-        B_best, J_best = self._state_fields(best_slot, face_diag=True)
+    def _finish(self, drv, best_state, final_state, same: bool):
+        """Selected state, final-vs-best, the staggered diagnostic and the
+        history of the reference driver (nfgs_solver.py:3816-3850,
+        :6956-7008).  best_state and final_state are (Phi_c, Az) pairs."""
+        selected = final_state if drv.stop_mode == "fixed_point" else best_state
+        B_sel, J_sel = self._state_fields(selected, face_diag=True)
         m = self._metrics.cpu().numpy()
         N = float(self.Nz * self.Ny * self.Nx)
         self._last_staggered_face_diagnostics = {
@@ -1321,28 +1440,24 @@ class NFGSSolverGPU:
             "projected_cell_current_rms": math.sqrt(m[L.Q_CELL_J2] / (3.0 * N)),
             "face_planes": self.Nz - 1,
         }
-        if final_slot != best_slot:
-            B_fin, _ = self._state_fields(final_slot)
+        if not same:
+            B_fin, _ = self._state_fields(final_state)
+            B_best = B_sel if selected is best_state else self._state_fields(best_state)[0]
             d = torch.zeros(2, device=self.device, dtype=torch.float64)
             L.check(self._lib.nfgs_sq_diff(self._plan, L.ptr(B_fin), L.ptr(B_best), 3 * int(N),
                                            L.ptr(d), self._stream), "final vs best")
             dh = d.cpu().numpy()
             fvb = math.sqrt(dh[0] / (3 * N)) / max(math.sqrt(dh[1] / (3 * N)), 1.0e-30)
-            del B_fin
+            del B_fin, B_best
         else:
             fvb = 0.0
-        self._B, self._J = B_best, J_best
-        self._Phi_c, self._Az = (t.clone() for t in self._slots[best_slot])
+        self._B, self._J = B_sel, J_sel
+        self._Phi_c, self._Az = (t.clone() for t in selected)
+        history = drv.history
+        final_sigma = history["sigma"][-1] if history["sigma"] else drv.sigma0
+        history.update(drv.summary())
         history.update(
-            best_sigma=best_sigma, best_iter=best_iter, rollbacks=0, stop_reason=reason,
-            stop_mode="sigma_best", selected_state="best_sigma", final_iter=final_iter,
             final_sigma=float(final_sigma), final_vs_best_B_relative=float(fvb),
-            fixed_point_converged=False, fixed_point_streak=0,
-            fixed_point_tolerances={"update": update_tol, "pde": float(fixed_point_pde_tol),
-                                    "observable": observable_tol,
-                                    "patience": int(fixed_point_patience)},
-            sigma_improvement_tol=eps if use_imp else None, auto_rbsl_improvement_grace=0,
-            last_sigma_improvement=last_imp,
             init_diagnostics=dict(self._init_diagnostics),
             source_partition_mode=self.source_partition_mode,
             bottom_current_grid_mode=self._bottom_current_grid_mode,

@@ -30,6 +30,22 @@ def test_library_exports_every_declared_function():
     assert all(f'"{n}"' in src for n in names)
 
 
+def test_library_was_built_from_these_sources():
+    from nfgs_amd import _lib as L
+    from nfgs_amd.build import source_digest
+    assert L.load().nfgs_source_digest().decode() == source_digest()
+
+
+def test_loader_refuses_a_library_built_from_other_sources(monkeypatch):
+    """A library older than its sources (a failed rebuild) must not load."""
+    from nfgs_amd import _lib as L
+    from nfgs_amd import build
+    monkeypatch.setattr(build, "source_digest", lambda: "0" * 64)
+    monkeypatch.setattr(L, "_lib", None)
+    with pytest.raises(RuntimeError, match="built from other sources"):
+        L.load()
+
+
 def test_last_error_is_empty_string_initially():
     from nfgs_amd import _lib as L
     assert isinstance(L.load().nfgs_last_error(), bytes)
