@@ -393,6 +393,56 @@ int nfgs_source_diagnostics(const nfgs_plan* plan, const nfgs_boundary* bnd, con
                             const void* d0, const void* dphi_target, const void* bz_above,
                             void* workspace, double* out, void* stream);
 
+/* ---- per-iteration diagnostics (record_iteration_diagnostics) ---------- *
+ * The reference's iteration_step diagnostics at mu = 1 (nfgs_solver.py:
+ * 6660-6680, :6806-6931).  Each function writes FP64 sums on the device; the
+ * host forms relative RMS values from them (sqrt(num / n) / max(sqrt(den / n),
+ * 1e-30), n the element count of the slice) in the reference's terms:
+ *   phi/az map and update residuals (whole volumes),
+ *   B_c,z residual (d_zz + lap_xy) lap_xy(Phi_c) - S_Bcz on planes 2..nz-2
+ *     (1..nz-2 when nz <= 4), J_z residual -(d_zz + lap_xy) lap_xy(Az) - S_Jz
+ *     on planes 1..nz-2,
+ *   poloidal (d_zz + lap_xy) Phi_c - source_phi over the B_c,z planes,
+ *     source_phi = IDCT2(DCT2(S_Bcz) / lambda), zero mode 0; its scale is the
+ *     largest RMS of the three terms,
+ *   toroidal (d_zz + lap_xy) Az - IDST2(-DST2(S_Jz) / lambda_D) inside,
+ *   the rho-mixed first row (1-rho) poloidal[1] - rho (dPhi_c/dz(0) -
+ *     target) / dz against its two terms and source_phi[1]. */
+enum {
+  NFGS_DR_PHI_MAP = 0, NFGS_DR_PHI_REF = 1, NFGS_DR_AZ_MAP = 2, NFGS_DR_AZ_REF = 3,
+  NFGS_DR_BCZ_RES = 4, NFGS_DR_BCZ_SRC = 5, NFGS_DR_JZ_RES = 6, NFGS_DR_JZ_SRC = 7,
+  NFGS_DR_POL_RES = 8, NFGS_DR_POL_D2Z = 9, NFGS_DR_POL_LAP = 10, NFGS_DR_POL_SRC = 11,
+  NFGS_DR_TOR_RES = 12, NFGS_DR_TOR_D2Z = 13, NFGS_DR_TOR_LAP = 14, NFGS_DR_TOR_SRC = 15,
+  NFGS_DR_MIX_RES = 16, NFGS_DR_MIX_PDE = 17, NFGS_DR_MIX_DER = 18, NFGS_DR_MIX_SRC = 19,
+  NFGS_DR_COUNT = 20
+};
+/* Face fractions of a state and the bottom imprint of its B: counts of face
+ * points with |B_f|^2 <= weak_field_floor_b2 and with |J_f.B_f| /
+ * max(|B_f|^2, floor) > alpha_max over the (nz-1)*ny*nx vertical faces
+ * (the state's own J_z), and sum |B(z=0) - B_obs|^2, sum |B_obs|^2. */
+enum { NFGS_DF_WEAK = 0, NFGS_DF_CLIP = 1, NFGS_DF_IMPRINT = 2, NFGS_DF_OBS = 3,
+       NFGS_DF_COUNT = 4 };
+/* Bytes of the workspace of nfgs_iteration_residuals / nfgs_face_fractions
+ * (five volumes, the interior volume of S_Jz and the FP64 partials). */
+long nfgs_iteration_diagnostics_workspace_bytes(const nfgs_plan* plan);
+/* S_Jz on the lateral walls of every plane (a volume; only wall nodes are
+ * written) from the parallel current a step starts from (ws.jp before
+ * nfgs_advance overwrites it). */
+int nfgs_diag_source_edges(const nfgs_plan* plan, const void* jp, void* sj_walls, void* stream);
+/* The NFGS_DR_* sums of one step from (phi_old, az_old) to (phi_new, az_new)
+ * with the step's sources: s_bcz (volume) and s_jz (interior planes) as
+ * nfgs_advance left them in ws, sj_walls from nfgs_diag_source_edges. */
+int nfgs_iteration_residuals(const nfgs_plan* plan, const void* phi_old, const void* az_old,
+                             const void* phi_new, const void* az_new, const void* s_bcz,
+                             const void* s_jz, const void* sj_walls, const void* dphi_target,
+                             void* workspace, double* out, void* stream);
+/* The NFGS_DF_* sums of the state (b, az), or of (b, jz) with a caller's
+ * centre J_z volume jz (az is then not read); bx_obs, by_obs the observed
+ * planes on the calculation grid (bz_obs is the boundary's). */
+int nfgs_face_fractions(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* b,
+                        const void* az, const void* jz, const void* bx_obs, const void* by_obs,
+                        void* workspace, double* out, void* stream);
+
 /* ---- common scoring of output fields (FP64, any grid, no plan) --------- */
 
 /* Second-order curl and divergence of b (3*nz*ny*nx, component-first z-y-x):

@@ -30,6 +30,11 @@ SD_COUNT = 10
 SC_COUNT = 11
 # nfgs_advance_ex poloidal update modes (NFGS_POLOIDAL_*)
 POLOIDAL_DIRECT, POLOIDAL_TWO_STAGE = 0, 1
+# iteration diagnostics sums (NFGS_DR_*, NFGS_DF_*)
+(DR_PHI_MAP, DR_PHI_REF, DR_AZ_MAP, DR_AZ_REF, DR_BCZ_RES, DR_BCZ_SRC, DR_JZ_RES, DR_JZ_SRC,
+ DR_POL_RES, DR_POL_D2Z, DR_POL_LAP, DR_POL_SRC, DR_TOR_RES, DR_TOR_D2Z, DR_TOR_LAP, DR_TOR_SRC,
+ DR_MIX_RES, DR_MIX_PDE, DR_MIX_DER, DR_MIX_SRC, DR_COUNT) = range(21)
+DF_WEAK, DF_CLIP, DF_IMPRINT, DF_OBS, DF_COUNT = range(5)
 # nfgs_plan_set_tuning knobs (NFGS_TUNE_*)
 TUNE_ZCHUNKS, TUNE_TRANSFORM_BLOCKS_PER_CU = 0, 1
 
@@ -69,6 +74,10 @@ def _declare(lib):
         "nfgs_plan_table": (vp, [vp, C.c_int]),
         "nfgs_plan_table_size": (C.c_long, [vp, C.c_int]),
         "nfgs_source_digest": (C.c_char_p, []),
+        "nfgs_iteration_diagnostics_workspace_bytes": (C.c_long, [vp]),
+        "nfgs_diag_source_edges": (C.c_int, [vp, vp, vp, vp]),
+        "nfgs_iteration_residuals": (C.c_int, [vp] + [vp] * 8 + [vp, vp, vp]),
+        "nfgs_face_fractions": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp, vp]),
         "nfgs_plan_set_tuning": (C.c_int, [vp, C.c_int, C.c_int]),
         "nfgs_plan_get_tuning": (C.c_int, [vp, C.c_int]),
         "nfgs_copy_plan_table": (C.c_int, [vp, C.c_int, vp, vp]),

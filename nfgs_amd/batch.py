@@ -191,6 +191,9 @@ def solve_batch(cases: Sequence[Case], *, max_iter=500, tol=1e-5, patience=40,
     roots = {}
     for c in cases:
         s = _make_solver(c, device)
+        if s.record_iteration_diagnostics:
+            raise NotImplementedError("solve_batch does not record per-iteration diagnostics; "
+                                      "run such a case with NFGSSolverGPU.iterate()")
         key = (id(c.boundary) if c.boundary is not None else id(c.bz),
                (s.Nz, s.Ny, s.Nx), c.spacing, c.dtype)
         if key in roots:

@@ -209,8 +209,9 @@ class _Options:
             _unsupported("bottom_divergence_projection_weight > 0")
         if self.bottom_parallel_current_mode != "observed_full":
             _unsupported("bottom_parallel_current_mode='face_parallel_exact_jz'")
-        if self.record_iteration_diagnostics:
-            _unsupported("record_iteration_diagnostics=True")
+        if self.record_iteration_diagnostics and self.poloidal_update_mode == "two_stage":
+            # the two-stage update reuses the source volume for its tables
+            _unsupported("record_iteration_diagnostics=True with poloidal_update_mode='two_stage'")
         if self.t_filter_scale != 0.0 or self.bottom_imprint_robin_weight != 0.0:
             _unsupported("Stable-layer T filtering or bottom imprint")
         if not self.bottom_jperp_zero:
@@ -332,6 +333,141 @@ class _Driver:
                     auto_rbsl_improvement_grace=0, last_sigma_improvement=self.last_imp)
 
 
+class _IterationDiagnostics:
+    """Per-iteration diagnostics of record_iteration_diagnostics=True
+    (nfgs_solver.py:6660-6680, :6806-6931) at mu = 1: device buffers and the
+    host assembly of the reference's relative RMS values from the library's
+    FP64 sums.
+
+    A step's dict holds the face fractions of the state it starts from and
+    the residuals and bottom imprint of the state it makes; the library
+    computes the fractions of each new state, so they are carried to the
+    next step.  With the frozen horizontal zero mode the H_k0 entries are 0,
+    and the bottom J_z imprint is 0 because the state carries the bottom
+    current exactly (J[2, 0] is set to it, :5920-5925)."""
+
+    KEYS = ("alpha_clip_fraction", "az_map_residual_relative", "az_update_relative",
+            "bcz_pde_residual_relative", "bottom_field_imprint_relative",
+            "bottom_jz_imprint_relative", "h_k0_map_residual_relative", "h_k0_update_relative",
+            "jz_pde_residual_relative", "mixed_row_residual_relative",
+            "phi_map_residual_relative", "phi_update_relative",
+            "poloidal_pde_residual_relative", "toroidal_pde_residual_relative",
+            "weak_field_floor_fraction")
+
+    def __init__(self, solver: "NFGSSolverGPU"):
+        self.s = solver
+        Nz, Ny, Nx = solver.Nz, solver.Ny, solver.Nx
+        lib = solver._lib
+        self.walls = solver._zeros(Nz, Ny, Nx)
+        self.ws = torch.empty(int(lib.nfgs_iteration_diagnostics_workspace_bytes(solver._plan)),
+                              device=solver.device, dtype=torch.uint8)
+        width = L.DR_COUNT + L.DF_COUNT
+        self.dev = torch.zeros(width, device=solver.device, dtype=torch.float64)
+        self.ring = [torch.zeros(width, dtype=torch.float64).pin_memory() for _ in range(2)]
+        self.face_in = None  # fractions of the state the next step starts from
+        n2 = Ny * Nx
+        zb0 = 2 if Nz > 4 else 1
+        self.n_all = float(Nz * n2)
+        self.n_b = float((Nz - 1 - zb0) * n2)
+        self.n_j = float((Nz - 2) * n2)
+        self.n_t = float((Nz - 2) * (Ny - 2) * (Nx - 2))
+        self.n_m = float(n2)
+        self.n_f = float((Nz - 1) * n2)
+        self.n_o = float(3 * n2)
+
+    def _df_ptr(self):
+        return L.ptr(self.dev) + 8 * L.DR_COUNT
+
+    def face(self, b, az=None, jz=None, out=None):
+        """Fractions and imprint of the state (b, az) or (b, J_z) into out
+        (default: this step's row)."""
+        s = self.s
+        L.check(s._lib.nfgs_face_fractions(s._plan, L.C.byref(s._bnd), L.ptr(b),
+                                           L.ptr(az) if az is not None else None,
+                                           L.ptr(jz) if jz is not None else None,
+                                           L.ptr(s.Bx_obs), L.ptr(s.By_obs), L.ptr(self.ws),
+                                           out if out is not None else self._df_ptr(),
+                                           s._stream), "face fractions")
+
+    def start(self, b, az=None, jz=None):
+        """Fractions of the state the first step starts from."""
+        self.face(b, az, jz)
+        self.face_in = self.dev[L.DR_COUNT:].cpu().numpy().copy()
+
+    def edges(self):
+        """S_Jz on the walls from the parallel current a step starts from."""
+        s = self.s
+        L.check(s._lib.nfgs_diag_source_edges(s._plan, L.ptr(s._ws_t["jp"]), L.ptr(self.walls),
+                                              s._stream), "source edges")
+
+    def after_step(self, old, new, b_new):
+        """Residual sums of the step old -> new and the fractions and imprint
+        of its new state (B in b_new)."""
+        s = self.s
+        (phi0, az0), (phi, az) = old, new
+        L.check(s._lib.nfgs_iteration_residuals(
+            s._plan, L.ptr(phi0), L.ptr(az0), L.ptr(phi), L.ptr(az), L.ptr(s._ws_t["s_bcz"]),
+            L.ptr(s._ws_t["s_jz"]), L.ptr(self.walls), L.ptr(s._bottom_dphi_c_target),
+            L.ptr(self.ws), L.ptr(self.dev), s._stream), "iteration residuals")
+        self.face(b_new, az)
+
+    def post(self, k):
+        self.ring[k].copy_(self.dev, non_blocking=True)
+
+    def take(self, k):
+        """The step's dict from host row k (after its event); the new
+        state's fractions become the next step's input fractions."""
+        row = self.ring[k].numpy().copy()
+        d = self.assemble(row[:L.DR_COUNT], self.face_in, row[L.DR_COUNT:])
+        self.face_in = row[L.DR_COUNT:]
+        return d
+
+    def assemble(self, dr, face_in, face_new):
+        def rel(num, den, n):
+            return math.sqrt(num / n) / max(math.sqrt(den / n), 1.0e-30)
+
+        def rel_op(res, terms, n):
+            return math.sqrt(res / n) / max(max(math.sqrt(t / n) for t in terms), 1.0e-30)
+
+        phi = rel(dr[L.DR_PHI_MAP], dr[L.DR_PHI_REF], self.n_all)
+        az = rel(dr[L.DR_AZ_MAP], dr[L.DR_AZ_REF], self.n_all)
+        return {
+            "phi_update_relative": phi,
+            "az_update_relative": az,
+            "phi_map_residual_relative": phi,
+            "az_map_residual_relative": az,
+            "h_k0_update_relative": 0.0,
+            "h_k0_map_residual_relative": 0.0,
+            "bcz_pde_residual_relative": rel(dr[L.DR_BCZ_RES], dr[L.DR_BCZ_SRC], self.n_b),
+            "jz_pde_residual_relative": rel(dr[L.DR_JZ_RES], dr[L.DR_JZ_SRC], self.n_j),
+            "poloidal_pde_residual_relative": rel_op(
+                dr[L.DR_POL_RES], (dr[L.DR_POL_D2Z], dr[L.DR_POL_LAP], dr[L.DR_POL_SRC]), self.n_b),
+            "toroidal_pde_residual_relative": rel_op(
+                dr[L.DR_TOR_RES], (dr[L.DR_TOR_D2Z], dr[L.DR_TOR_LAP], dr[L.DR_TOR_SRC]), self.n_t),
+            "mixed_row_residual_relative": rel_op(
+                dr[L.DR_MIX_RES], (dr[L.DR_MIX_PDE], dr[L.DR_MIX_DER], dr[L.DR_MIX_SRC]), self.n_m),
+            "bottom_field_imprint_relative": rel(face_new[L.DF_IMPRINT], face_new[L.DF_OBS],
+                                                 self.n_o),
+            "bottom_jz_imprint_relative": 0.0,
+            "alpha_clip_fraction": float(face_in[L.DF_CLIP]) / self.n_f,
+            "weak_field_floor_fraction": float(face_in[L.DF_WEAK]) / self.n_f,
+        }
+
+    def definition(self):
+        """history["iteration_diagnostics_definition"] (its keys and texts are
+        the reference's, :6972-6991)."""
+        return {
+            "state": ("attempted candidate from the proposed nonlinear map; accepted versus "
+                      "rollback status is stored in history.rollback"),
+            "map_residual": "unrelaxed nonlinear map T(x_n)-x_n divided by RMS state",
+            "pde_scope": ("scalar poloidal/toroidal interior rows; mixed first-interior row is "
+                          "reported separately"),
+            "relative_norm": "RMS residual divided by the largest RMS operator/source term",
+            "weak_field_floor_B2": self.s.weak_field_floor_b2,
+            "alpha_clip_scale": self.s.alpha_clip_scale,
+        }
+
+
 class NFGSSolverGPU:
     """Gaussian-separated NFPT solver on MI355X (nfgs_solver.py:3905).
 
@@ -383,7 +519,10 @@ class NFGSSolverGPU:
         self.source_zero_mode_tolerance = opts.source_zero_mode_tolerance
         self.weak_field_floor_b2 = opts.weak_field_floor_b2
         self.alpha_clip_scale = opts.alpha_limit_factor
-        self.record_iteration_diagnostics = False
+        self.record_iteration_diagnostics = opts.record_iteration_diagnostics
+        self._last_iteration_diagnostics = {}
+        self._iteration_diagnostics_history = []
+        self._diag = None
 
         self._input_shape_zyx = (int(Nz), int(Ny), int(Nx))
         use_pad = bool(padded_buffer_zone) if pad_calculation_domain is None \
@@ -602,6 +741,7 @@ class NFGSSolverGPU:
         self._metrics_ring = [torch.zeros(L.NQ, dtype=torch.float64).pin_memory()
                               for _ in range(2)]
         self._metrics_events = [None, None]
+        self._diag = None
 
     def _configure_gaussian_diagnostics(self, g, h_target):
         o = self.opts
@@ -798,6 +938,12 @@ class NFGSSolverGPU:
         self._metrics_host.copy_(self._metrics, non_blocking=True)
         torch.cuda.current_stream(self.device).synchronize()
         return self._decode_metrics(self._metrics_host.numpy())
+
+    def _diagnostics(self) -> _IterationDiagnostics:
+        """The per-iteration diagnostics buffers of this plan."""
+        if self._diag is None:
+            self._diag = _IterationDiagnostics(self)
+        return self._diag
 
     def _post_metrics(self, k: int):
         """Queue a copy of the monitors of the step just enqueued into host
@@ -997,14 +1143,29 @@ class NFGSSolverGPU:
                              f"{tuple(J.shape)}")
         Bc = B.to(device=self.device, dtype=self.dtype).contiguous()
         Jc = J.to(device=self.device, dtype=self.dtype).contiguous()
+        diag = self._diagnostics() if self.record_iteration_diagnostics else None
+        if diag is not None:
+            # the fractions of the caller's state, with its J_z (:6660-6680)
+            diag.start(Bc, jz=Jc[2])
         L.check(self._lib.nfgs_face_pass_j(self._plan, L.C.byref(self._bnd), L.ptr(Bc), L.ptr(Jc),
                                            L.ptr(self._ws_t["jp"]), None, self._stream),
                 "face pass")
+        if diag is not None:
+            diag.edges()
         self._advance(1, zero_source=False)
         _, _, _, ratio = self._read_metrics()
         self._check_zero_mode(ratio)
         Bn, Jn = self._state_fields(1)
         phi, az = self._slots[1]
+        if diag is not None:
+            shape = (self.Nz, self.Ny, self.Nx)
+            old = (self._dev(Phi_c, shape, "Phi_c"), self._dev(Az, shape, "Az"))
+            diag.after_step(old, (phi, az), Bn)
+            diag.post(0)
+            torch.cuda.current_stream(self.device).synchronize()
+            dg = diag.take(0)
+            self._last_iteration_diagnostics = dg
+            self._iteration_diagnostics_history.append(dg)
         return Bn, Jn, phi.clone(), az.clone(), None
 
     # ------------------------------------------------------------------
@@ -1345,7 +1506,7 @@ class NFGSSolverGPU:
             raise ValueError("NFGS supports init_mode='gs_cfcf', 'cfcf', or 'soft_cfcf' "
                              "(and 'field' for a caller's starting field)")
         stop_mode = _norm(stop_mode)
-        if stop_mode == "fixed_point":
+        if stop_mode == "fixed_point" and not self.record_iteration_diagnostics:
             raise ValueError("stop_mode='fixed_point' requires record_iteration_diagnostics=True")
         if stop_mode not in {"sigma_best", "fixed_point"}:
             raise ValueError("stop_mode must be 'sigma_best' or 'fixed_point'")
@@ -1365,15 +1526,22 @@ class NFGSSolverGPU:
         self._require_boundary()
         mode = self._init_mode(init_mode)
 
+        diag = self._diagnostics() if self.record_iteration_diagnostics else None
+        self._iteration_diagnostics_history = []
+        self._last_iteration_diagnostics = {}
         # initial state in slot 0 (gs_cfcf, or a caller's field) and its monitors
         if mode == "field":
             self._init_from_field(init_field, init_az, init_current)
             sigma0, divB0, E0, _ = self._read_metrics()  # no source zero mode at the start
+            if diag is not None:
+                self._state_fields(0, B_out=self._Bwork)
         else:
-            self._advance(0, zero_source=True)
+            self._advance(0, zero_source=True, want_b=diag is not None)
             sigma0, divB0, E0, ratio0 = self._read_metrics()
             self._check_zero_mode(ratio0)
             self._init_diagnostics = self._gs_cfcf_diagnostics()
+        if diag is not None:
+            diag.start(self._Bwork, self._slots[0][1])
         if verbose:
             am = self.alpha_max()
             am_text = f"{am:.3e}" if am is not None else "off"
@@ -1388,25 +1556,40 @@ class NFGSSolverGPU:
         # Step n+1 goes into a slot that is neither the best state nor step
         # n's, so whichever the stop rules keep survives it.  A callback
         # needs B and J of the step it sees, so then steps run one at a time.
+        # With diagnostics a step also needs the state it started from, so the
+        # slot last written is kept as well.
         ahead = per_step_callback is None
+        last = [0]
 
-        def launch(n_step, avoid):
-            s_new = next(k for k in range(3) if k not in avoid)
-            self._advance(s_new, zero_source=False, want_b=not ahead)
+        def launch(n_step, best):
+            old = last[0]
+            s_new = next(k for k in range(3) if k not in (best, old))
+            if diag is not None:
+                diag.edges()
+            self._advance(s_new, zero_source=False, want_b=not ahead or diag is not None)
+            if diag is not None:
+                diag.after_step(self._slots[old], self._slots[s_new], self._Bwork)
+                diag.post(n_step & 1)
             self._post_metrics(n_step & 1)
+            last[0] = s_new
             return s_new
 
-        slot_next = launch(1, {best_slot}) if int(max_iter) >= 1 else best_slot
+        slot_next = launch(1, best_slot) if int(max_iter) >= 1 else best_slot
         for n in range(1, int(max_iter) + 1):
             if not ahead and n > 1:
-                slot_next = launch(n, {best_slot})
+                slot_next = launch(n, best_slot)
             slot = slot_next
             if ahead and n < int(max_iter):
-                slot_next = launch(n + 1, {best_slot, slot})
+                slot_next = launch(n + 1, best_slot)
             sigma, divB, energy, ratio = self._take_metrics(n & 1)
             self._check_zero_mode(ratio)
             final_slot = slot
-            drv.record(n, sigma, divB, energy)
+            dg = None
+            if diag is not None:
+                dg = diag.take(n & 1)
+                self._last_iteration_diagnostics = dg
+                self._iteration_diagnostics_history.append(dg)
+            drv.record(n, sigma, divB, energy, dg)
             if per_step_callback is not None:
                 # the callback owns fresh B and J of this step, as the
                 # reference hands it the new state tensors (:3736-3737)
@@ -1456,6 +1639,12 @@ class NFGSSolverGPU:
         history = drv.history
         final_sigma = history["sigma"][-1] if history["sigma"] else drv.sigma0
         history.update(drv.summary())
+        if self.record_iteration_diagnostics:
+            # one list per diagnostic, as the reference adds them (:6964-6991)
+            rows = self._iteration_diagnostics_history[:len(history["sigma"])]
+            for key in _IterationDiagnostics.KEYS:
+                history[key] = [float(r[key]) for r in rows]
+            history["iteration_diagnostics_definition"] = self._diagnostics().definition()
         history.update(
             final_sigma=float(final_sigma), final_vs_best_B_relative=float(fvb),
             init_diagnostics=dict(self._init_diagnostics),

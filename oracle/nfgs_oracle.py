@@ -1042,11 +1042,112 @@ class OracleNFGS:
         return B2, J2, phi, az
 
     # ------------------------------------------------------------------
+    # Per-iteration diagnostics (record_iteration_diagnostics=True,
+    # nfgs_solver.py:6660-6680 and :6806-6931), at mu = 1 with the frozen
+    # horizontal zero mode and no bottom projection
+    # ------------------------------------------------------------------
+    DIAGNOSTIC_KEYS = ("alpha_clip_fraction", "az_map_residual_relative",
+                       "az_update_relative", "bcz_pde_residual_relative",
+                       "bottom_field_imprint_relative", "bottom_jz_imprint_relative",
+                       "h_k0_map_residual_relative", "h_k0_update_relative",
+                       "jz_pde_residual_relative", "mixed_row_residual_relative",
+                       "phi_map_residual_relative", "phi_update_relative",
+                       "poloidal_pde_residual_relative", "toroidal_pde_residual_relative",
+                       "weak_field_floor_fraction")
+
+    def step_diagnostics(self, B, J, Phi_c, Az):
+        """One step and its diagnostics: returns (B2, J2, phi, az, diag)."""
+        dx, dy, dz = self.dx, self.dy, self.dz
+        f64 = lambda v: np.asarray(v, dtype=np.float64)
+        rms = lambda v: float(np.sqrt(np.mean(f64(v) ** 2)))
+
+        def rel(delta, scale):
+            return rms(delta) / max(rms(scale), 1.0e-30)
+
+        def rel_op(residual, *terms):
+            return rms(residual) / max(max(rms(t) for t in terms), 1.0e-30)
+
+        # face counts of the input state (:6660-6680)
+        Bf, Jf = self.face_state(B, J[2])
+        b2 = np.sum(Bf * Bf, axis=0)
+        weak = float(np.mean(f64(b2 <= self.floor_b2)))
+        alpha = np.sum(Jf * Bf, axis=0) / np.maximum(b2, self.dtype.type(self.floor_b2))
+        clip = float(np.mean(f64(np.abs(alpha) > self.alpha_max)))
+
+        _, s_bcz, s_jz = self.sources(B, J)
+        B2, J2, phi, az = self.step(B, J, Phi_c, Az)
+        lap = lambda f: lap_xy(f, dx, dy)
+        d2z = lambda f: d2_dz2(f, dz)
+        bcz = lap(phi)
+        bcz_res = d2z(bcz) + lap(bcz) - s_bcz
+        jzs = -lap(az)
+        jz_res = d2z(jzs) + lap(jzs) - s_jz
+        zb = slice(2, -1) if self.Nz > 4 else slice(1, -1)
+        zj = slice(1, -1)
+
+        lam = self.lam_neumann().astype(self.dtype)
+        ls = lam.copy()
+        ls[0, 0] = 1.0
+        sph = dct2(s_bcz) / ls[None]
+        sph[:, 0, 0] = 0.0
+        src_phi = idct2(sph)
+        phi_res = d2z(phi) + lap(phi) - src_phi
+
+        src_az = np.zeros_like(az)
+        sjh = dst2(s_jz[:, 1:-1, 1:-1])
+        src_az[:, 1:-1, 1:-1] = idst2(-sjh / self.lam_dirichlet().astype(self.dtype)[None])
+        az_res = d2z(az) + lap(az) - src_az
+        inner = (slice(1, -1), slice(1, -1), slice(1, -1))
+
+        rho = self.rho
+        dphi = (-3.0 * phi[0] + 4.0 * phi[1] - phi[2]) / (2.0 * dz)
+        mixed_pde = (1.0 - rho) * phi_res[1]
+        mixed_der = -rho * (dphi - self.dphi_target) / dz
+        mixed = mixed_pde + mixed_der
+
+        obs = np.stack([self.Bx_obs, self.By_obs, self.Bz_obs])
+        diag = {
+            "phi_update_relative": rel(phi - Phi_c, Phi_c),
+            "az_update_relative": rel(az - Az, Az),
+            "phi_map_residual_relative": rel(phi - Phi_c, Phi_c),
+            "az_map_residual_relative": rel(az - Az, Az),
+            "h_k0_update_relative": 0.0,  # the frozen profile does not change in a step
+            "h_k0_map_residual_relative": 0.0,
+            "bcz_pde_residual_relative": rel(bcz_res[zb], s_bcz[zb]),
+            "jz_pde_residual_relative": rel(jz_res[zj], s_jz[zj]),
+            "poloidal_pde_residual_relative": rel_op(phi_res[zb], d2z(phi)[zb], lap(phi)[zb],
+                                                     src_phi[zb]),
+            "toroidal_pde_residual_relative": rel_op(az_res[inner], d2z(az)[inner],
+                                                     lap(az)[inner], src_az[inner]),
+            "mixed_row_residual_relative": rel_op(mixed, mixed_pde, mixed_der, src_phi[1]),
+            "bottom_field_imprint_relative": rel(B2[:, 0] - obs, obs),
+            "bottom_jz_imprint_relative": rel(J2[2, 0] - self.Jz_T, self.Jz_T),
+            "alpha_clip_fraction": clip,
+            "weak_field_floor_fraction": weak,
+        }
+        return B2, J2, phi, az, diag
+
+    # ------------------------------------------------------------------
     # Driver (nfgs_solver.py:3522-3850, :6937-7009)
     # ------------------------------------------------------------------
     def iterate(self, max_iter=200, tol=1e-5, patience=1, sigma_improvement_tol=1e-5,
-                callback=None, start=None):
-        """start=(B, Az): begin from project_field(B, Az) instead of gs_cfcf."""
+                callback=None, start=None, record_diagnostics=False, stop_mode="sigma_best",
+                fixed_point_update_tol=None, fixed_point_pde_tol=1.0e-8,
+                fixed_point_observable_tol=None, fixed_point_patience=10):
+        """start=(B, Az): begin from project_field(B, Az) instead of gs_cfcf.
+        record_diagnostics: the per-step diagnostics as history lists;
+        stop_mode="fixed_point" (needs them) stops after fixed_point_patience
+        consecutive steps with the map residual, the PDE residual and the
+        sigma/energy drift within their tolerances and selects the final
+        iterate (nfgs_solver.py:3565-3582, :3712-3733, :3766-3776, :3816)."""
+        if stop_mode == "fixed_point" and not record_diagnostics:
+            raise ValueError("stop_mode='fixed_point' requires record_iteration_diagnostics=True")
+        upd = float(tol if fixed_point_update_tol is None else fixed_point_update_tol)
+        obs_tol = float(tol if fixed_point_observable_tol is None else fixed_point_observable_tol)
+        pde_tol = float(fixed_point_pde_tol)
+        streak = 0
+        fp_converged = False
+        diags = []
         if start is None:
             B, J, phi, az = self.initial_state()
         else:
@@ -1065,18 +1166,34 @@ class OracleNFGS:
         reason = "max_iter"
         final_iter = 0
         for n in range(1, max_iter + 1):
-            B, J, phi, az = self.step(B, J, phi, az)
+            if record_diagnostics:
+                B, J, phi, az, dg = self.step_diagnostics(B, J, phi, az)
+                diags.append(dg)
+            else:
+                B, J, phi, az = self.step(B, J, phi, az)
             s = self.sigma(B, J)
             d, e = self.divB(B), self.energy(B)
             pe = h["energy"][-1] if h["energy"] else e0
+            src = abs(s - prev) / max(abs(prev), 1e-30)
+            erc = abs(e - pe) / max(abs(pe), 1e-30)
             h["sigma"].append(s)
             h["divB"].append(d)
             h["energy"].append(e)
             h["mu"].append(1.0)
             h["rollback"].append(0)
-            h["sigma_relative_change"].append(abs(s - prev) / max(abs(prev), 1e-30))
-            h["energy_relative_change"].append(abs(e - pe) / max(abs(pe), 1e-30))
-            h["fixed_point_criteria_met"].append(0)
+            h["sigma_relative_change"].append(src)
+            h["energy_relative_change"].append(erc)
+            met = False
+            if stop_mode == "fixed_point":
+                dg = diags[-1]
+                mres = max(dg["phi_map_residual_relative"], dg["az_map_residual_relative"],
+                           dg["h_k0_map_residual_relative"])
+                pres = max(dg["poloidal_pde_residual_relative"],
+                           dg["toroidal_pde_residual_relative"], dg["mixed_row_residual_relative"])
+                met = (np.isfinite(mres) and np.isfinite(pres) and mres <= upd and pres <= pde_tol
+                       and src <= obs_tol and erc <= obs_tol)
+            h["fixed_point_criteria_met"].append(int(met))
+            streak = streak + 1 if met else 0
             final_iter = n
             if callback is not None:
                 callback(n, s, B, J)
@@ -1090,6 +1207,12 @@ class OracleNFGS:
                 stall = 0
             else:
                 stall += 1
+            if stop_mode == "fixed_point":
+                if streak >= int(fixed_point_patience):
+                    reason = "fixed_point"
+                    fp_converged = True
+                    break
+                continue
             if s < tol:
                 reason = "tol"
                 break
@@ -1100,10 +1223,19 @@ class OracleNFGS:
                 reason = "sigma_stall"
                 break
         final = (B, J)
+        final_state = (B, J, phi, az)
         fvb = np.sqrt(np.mean((final[0].astype(np.float64) - best[0].astype(np.float64)) ** 2)) / \
             max(np.sqrt(np.mean(best[0].astype(np.float64) ** 2)), 1e-30)
+        if record_diagnostics:
+            for k in self.DIAGNOSTIC_KEYS:
+                h[k] = [float(dg[k]) for dg in diags]
+        if stop_mode == "fixed_point":
+            best = tuple(a.copy() for a in final_state)
+        h.update(fixed_point_converged=fp_converged, fixed_point_streak=streak)
         h.update(best_sigma=best_sigma, best_iter=best_iter, rollbacks=0, stop_reason=reason,
-                 stop_mode="sigma_best", selected_state="best_sigma", final_iter=final_iter,
+                 stop_mode=stop_mode,
+                 selected_state="final_accepted_iterate" if stop_mode == "fixed_point"
+                 else "best_sigma", final_iter=final_iter,
                  final_sigma=float(self.sigma(*final)), final_vs_best_B_relative=float(fvb),
                  sigma_improvement_tol=eps if use_imp else None, last_sigma_improvement=last_imp,
                  initial_sigma=s0, initial_divB=d0, initial_energy=e0)
