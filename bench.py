@@ -61,6 +61,9 @@ def parse():
     p.add_argument("--batch-rhos", type=int, default=11)
     p.add_argument("--no-two-stage", action="store_true",
                    help="skip the two-stage poloidal update solve")
+    p.add_argument("--steps-only", action="store_true",
+                   help="profiling workload: set up the 258x258x257 FP32 solve and run only "
+                        "--warmup + --steps plain iterations (no solves, no legs)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run a 256 MiB device copy for counter calibration")
     return p.parse_args()
@@ -399,6 +402,22 @@ def main():
     # measure the per-solve work only
     log("warm-up solve fp32 rho=0.5, 2 iterations")
     warm = solve(torch, NFGSSolverGPU, planes, n, 0.5, torch.float32, 2)
+    if a.steps_only:
+        s = warm[0]
+        for _ in range(a.warmup):
+            s._advance(1, zero_source=False)
+            s._read_metrics()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            s._advance(1, zero_source=False)
+            s._read_metrics()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / max(a.steps, 1) * 1e3
+        log(f"steps only: {ms:.4f} ms per step over {a.steps}")
+        print(json.dumps({"workload": "steps only (profiling)", "ms_per_step": ms,
+                          "steps": a.steps, "grid_zyx": [s.Nz, s.Ny, s.Nx]}))
+        return
     del warm
 
     # headline: plain method at rho 0, 0.5 and 1 (value = rho 0.5)
