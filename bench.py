@@ -61,6 +61,10 @@ def parse():
     p.add_argument("--batch-rhos", type=int, default=11)
     p.add_argument("--no-two-stage", action="store_true",
                    help="skip the two-stage poloidal update solve")
+    p.add_argument("--no-precision", action="store_true",
+                   help="skip the precision leg (512x512x256: FP32 exact and split-bf16 "
+                        "against FP64)")
+    p.add_argument("--precision-n", type=int, default=512)
     p.add_argument("--steps-only", action="store_true",
                    help="profiling workload: set up the 258x258x257 FP32 solve and run only "
                         "--warmup + --steps plain iterations (no solves, no legs)")
@@ -83,14 +87,17 @@ ITER_KW = dict(init_mode="gs_cfcf", tol=1e-5, mu=1, patience=40, sigma_improveme
                stop_mode="sigma_best", verbose=False)
 
 
-def solve(torch, NFGSSolverGPU, planes, n, rho, dtype, max_iter, start=None, extra=None):
-    """One timed solve with the boundary already resident on the device.
+def solve(torch, NFGSSolverGPU, planes, n, rho, dtype, max_iter, start=None, extra=None,
+          nz=None):
+    """One timed solve with the boundary already resident on the device
+    (n x n observed columns over nz planes, default n).
     start=(B, Az): init_mode="field" from that state instead of gs_cfcf;
     extra: more constructor keywords."""
     bx, by, bz = (p.to(dtype) for p in planes)
     kw = solver_kwargs(rho)
     kw.update(extra or {})
-    s = NFGSSolverGPU(n, n, n, 1.0, 1.0, 1.0, device="cuda", dtype=dtype, **kw)
+    s = NFGSSolverGPU(n, n, n if nz is None else nz, 1.0, 1.0, 1.0, device="cuda", dtype=dtype,
+                      **kw)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     s.set_bottom_vector_boundary(bx, by, bz, zero_net_current=False, smooth_passes=0, ijzbcor=3)
@@ -264,23 +271,29 @@ def batched_leg(torch, a):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     seq = []
+    seq_phases = {"construct_and_setup": 0.0, "iterate": 0.0}
     for c in cases:
+        ta = time.perf_counter()
         nyo, nxo = c.bz.shape
         s = NFGSSolverGPU(nxo, nyo, c.nz, 1.0, 1.0, 1.0, device="cuda", dtype=c.dtype,
                           **c.kwargs)
         s.set_bottom_vector_boundary(c.bx, c.by, c.bz, zero_net_current=False,
                                      smooth_passes=0, ijzbcor=3)
+        tb = time.perf_counter()
         h = s.iterate(init_mode="gs_cfcf", mu=1, stop_mode="sigma_best", verbose=False, **it)
-        seq.append((h["best_sigma"], h["best_iter"], h["final_iter"], s.fields_to_numpy()))
-        del s
+        seq_phases["construct_and_setup"] += tb - ta
+        seq_phases["iterate"] += time.perf_counter() - tb
+        seq.append((h, s))  # the selected fields stay on the device until after timing
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
+    phases = {}
     t0 = time.perf_counter()
-    res = solve_batch(cases, **it)
+    res = solve_batch(cases, stats=phases, **it)
     torch.cuda.synchronize()
     t_bat = time.perf_counter() - t0
     dsig, dB, same_stop = 0.0, 0.0, 0
-    for (hb, sb), (bs, bi, fi, fs) in zip(res, seq):
+    for (hb, sb), (hs, ss) in zip(res, seq):
+        bs, bi, fi, fs = hs["best_sigma"], hs["best_iter"], hs["final_iter"], ss.fields_to_numpy()
         dsig = max(dsig, abs(hb["best_sigma"] - bs) / bs)
         fb = sb.fields_to_numpy()
         num = sum(float(np.sum((fb[k].astype(np.float64) - fs[k]) ** 2)) for k in ("Bx", "By", "Bz"))
@@ -296,10 +309,64 @@ def batched_leg(torch, a):
                 batched_solves_per_hour=round(n / t_bat * 3600, 1),
                 speedup=round(t_seq / t_bat, 3), total_iterations=iters,
                 max_best_sigma_rel_diff=dsig, max_B_rel_diff=dB,
-                same_best_and_final_iter=f"{same_stop}/{n}",
+                same_best_and_final_iter=f"{same_stop}/{n}", batched_phases_s=phases,
+                sequential_phases_s={k: round(v, 4) for k, v in seq_phases.items()},
                 note="batched: boundary setup shared across Robin weights, 8-step HIP graphs, "
                      "4 streams, autotuned z-chunks and transform blocks; tuning changes "
                      "only the monitors' summation order")
+
+
+def field_errors(f, ref):
+    """Relative L2 of every field of f against ref (fields_to_numpy dicts)."""
+    import numpy as np
+    return {k: float(np.linalg.norm(f[k].astype(np.float64) - ref[k]) /
+                     max(np.linalg.norm(ref[k]), 1e-300)) for k in f}
+
+
+def precision_leg(torch, a):
+    """The precision configuration: a 512x512 observed multipole boundary
+    (seed a.seed) over 256 planes (calculation grid 514x514x257), rho 0.5,
+    solved in FP64 (the reference of the errors), in FP32 with the exact
+    lateral arithmetic (FP64 products at this length; the int8 digit form
+    covers lengths up to 258) and in FP32 with the split-bf16 x3 products.
+    Per-field relative L2 and the best sigma_J deviation against FP64, and
+    the reference's precision-comparison metrics (common scores)."""
+    import numpy as np
+    from nfgs_amd import NFGSSolverGPU
+    from nfgs_amd import scores as S
+    from nfgs_amd.boundary import multipole_boundary
+    n = a.precision_n
+    bx, by, bz, _ = multipole_boundary(n, seed=a.seed)
+    planes = tuple(torch.tensor(p, dtype=torch.float64, device="cuda") for p in (bx, by, bz))
+    out = {"grid_observed_xyz": [n, n, n // 2]}
+    runs = {}
+    for tag, dtype, extra in (("fp64", torch.float64, None),
+                              ("fp32_exact", torch.float32, None),
+                              ("fp32_bf16x3", torch.float32,
+                               {"lateral_transform_arithmetic": "bf16x3"})):
+        log(f"  precision: {tag} solve at {n}x{n}x{n // 2}")
+        s, h, setup_s, total_s = solve(torch, NFGSSolverGPU, planes, n, 0.5, dtype, a.max_iter,
+                                       extra=extra, nz=n // 2)
+        out["grid_calc_zyx"] = [s.Nz, s.Ny, s.Nx]
+        runs[tag] = (h, s.fields_to_numpy())
+        out[tag] = dict(sec_to_converge=round(total_s, 3), iterations=h["final_iter"],
+                        best_iter=h["best_iter"], best_sigma=h["best_sigma"],
+                        stop_reason=h["stop_reason"])
+        del s
+        torch.cuda.empty_cache()
+    h64, f64 = runs["fp64"]
+    box = (slice(None),) * 3
+    sj64 = S.sigma_j(f64, 1.0, box)
+    for tag in ("fp32_exact", "fp32_bf16x3"):
+        h, f = runs[tag]
+        e = field_errors(f, f64)
+        e["best_sigma_rel"] = abs(h["best_sigma"] - h64["best_sigma"]) / h64["best_sigma"]
+        m = S.compute_reference_metrics(f64, f, box)
+        e.update({k: m[k] for k in ("C_vec", "Eprime_n", "Eprime_m", "epsilon")})
+        e["sigma_J_common_fd2"] = S.sigma_j(f, 1.0, box)
+        e["sigma_J_common_fd2_fp64"] = sj64
+        out[tag]["vs_fp64"] = e
+    return out
 
 
 def cpu_threads():
@@ -601,6 +668,13 @@ def main():
         err["common_scores"] = cs
         log(f"  common scores FP32 vs FP64: {cs}")
 
+    precision = None
+    if not a.no_precision:
+        log(f"precision leg: {a.precision_n}^2 x {a.precision_n // 2}, FP64 / FP32 exact / "
+            "FP32 split-bf16")
+        precision = precision_leg(torch, a)
+        log(f"  {precision}")
+
     cpu = None
     if not a.no_cpu:
         threads = cpu_threads()
@@ -646,6 +720,7 @@ def main():
         "kernels": kernels,
         "warm_start": warm,
         "batched": batched,
+        "precision": precision,
         "two_stage": two,
         "err_vs_fp64": err,
         "cpu_baseline": cpu,

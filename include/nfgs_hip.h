@@ -125,6 +125,18 @@ int nfgs_plan_set_tuning(nfgs_plan* plan, int knob, int value);
 /* Current value of a knob (-1 for an unknown knob). */
 int nfgs_plan_get_tuning(const nfgs_plan* plan, int knob);
 
+/* Arithmetic of an FP32 plan's lateral transforms.  NFGS_LATERAL_EXACT (the
+ * default): exact integer digit products (lengths up to 258) or FP64 products,
+ * one FP32 rounding per output.  NFGS_LATERAL_BF16X3: the precision form,
+ * every input and matrix entry split into two bf16 terms, three bf16 MFMA
+ * products per term, FP32 accumulation (relative error near 2^-16 per
+ * product).  FP64 plans take only NFGS_LATERAL_EXACT.  No counterpart in the
+ * reference (its transforms are FFTs in the solver dtype). */
+enum { NFGS_LATERAL_EXACT = 0, NFGS_LATERAL_BF16X3 = 1 };
+int nfgs_plan_set_lateral_arithmetic(nfgs_plan* plan, int arith);
+/* The current NFGS_LATERAL_* value (-1 for a null plan). */
+int nfgs_plan_get_lateral_arithmetic(const nfgs_plan* plan);
+
 /* Unnormalised type-I transform (forward) or its inverse along one axis of a
  * batch of vectors: vector g = p*nvec + q starts at p*plane_stride +
  * q*vec_stride and element j sits at j*elem_stride.  Either both element

@@ -35,6 +35,8 @@ POLOIDAL_DIRECT, POLOIDAL_TWO_STAGE = 0, 1
  DR_POL_RES, DR_POL_D2Z, DR_POL_LAP, DR_POL_SRC, DR_TOR_RES, DR_TOR_D2Z, DR_TOR_LAP, DR_TOR_SRC,
  DR_MIX_RES, DR_MIX_PDE, DR_MIX_DER, DR_MIX_SRC, DR_COUNT) = range(21)
 DF_WEAK, DF_CLIP, DF_IMPRINT, DF_OBS, DF_COUNT = range(5)
+# lateral transform arithmetic of FP32 plans (NFGS_LATERAL_*)
+LATERAL_EXACT, LATERAL_BF16X3 = 0, 1
 # nfgs_plan_set_tuning knobs (NFGS_TUNE_*)
 TUNE_ZCHUNKS, TUNE_TRANSFORM_BLOCKS_PER_CU = 0, 1
 
@@ -79,6 +81,8 @@ def _declare(lib):
         "nfgs_iteration_residuals": (C.c_int, [vp] + [vp] * 8 + [vp, vp, vp]),
         "nfgs_face_fractions": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp, vp]),
         "nfgs_plan_set_tuning": (C.c_int, [vp, C.c_int, C.c_int]),
+        "nfgs_plan_set_lateral_arithmetic": (C.c_int, [vp, C.c_int]),
+        "nfgs_plan_get_lateral_arithmetic": (C.c_int, [vp]),
         "nfgs_plan_get_tuning": (C.c_int, [vp, C.c_int]),
         "nfgs_copy_plan_table": (C.c_int, [vp, C.c_int, vp, vp]),
         "nfgs_modal_robin_solve": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, C.c_int, vp, vp, vp,

@@ -26,6 +26,7 @@ fields.
 """
 from __future__ import annotations
 
+import time
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -127,8 +128,9 @@ class _Run:
             self.event = torch.cuda.Event()
             self.event.record(self.stream)
 
-    def collect(self):
+    def collect(self, clock=None):
         """Apply the stop rules to the K monitors of the last launch."""
+        t = time.perf_counter()
         self.event.synchronize()
         m = self.host_m.numpy()
         best_i = None
@@ -145,10 +147,15 @@ class _Run:
                 break
         with torch.cuda.stream(self.stream):
             if best_i is not None:
-                for t, u in zip(self.best, self.ring[best_i]):
-                    t.copy_(u)
+                for a, b in zip(self.best, self.ring[best_i]):
+                    a.copy_(b)
+        if clock is not None:
+            clock["collect"] += time.perf_counter() - t
         if self.final is not None:
+            t = time.perf_counter()
             self._finish()
+            if clock is not None:
+                clock["finish"] += time.perf_counter() - t
 
     def _finish(self):
         s = self.s
@@ -173,7 +180,8 @@ def _make_solver(case: Case, device) -> NFGSSolverGPU:
 
 def solve_batch(cases: Sequence[Case], *, max_iter=500, tol=1e-5, patience=40,
                 sigma_improvement_tol: Optional[float] = 1e-5, graph_steps: int = 8,
-                streams: int = 4, tune: bool = True, device="cuda", max_concurrent: int = 12):
+                streams: int = 4, tune: bool = True, device="cuda", max_concurrent: int = 12,
+                stats: Optional[dict] = None):
     """Solve every case; returns [(history, solver)] in the order of `cases`.
 
     Solves of one boundary share its setup; at most `max_concurrent` solves
@@ -181,7 +189,12 @@ def solve_batch(cases: Sequence[Case], *, max_iter=500, tol=1e-5, patience=40,
     streams (default priority).  tune=True applies the autotuner's launch
     shapes per grid.  The stop rules are the reference driver's sigma_best
     rules with the given max_iter, tol, patience and sigma_improvement_tol.
+    stats (a dict) receives the host seconds of each phase: setup (solvers,
+    boundaries, tuning), start (gs_cfcf starts and graph captures), launch,
+    collect (waiting for and reading monitors) and finish (selected fields).
     """
+    clock = {k: 0.0 for k in ("setup", "tune", "start", "launch", "collect", "finish")}
+    t_setup = time.perf_counter()
     if graph_steps < 1 or streams < 1 or max_concurrent < 1:
         raise ValueError("graph_steps, streams and max_concurrent must be >= 1")
     iterate_kw = dict(max_iter=max_iter, tol=tol, patience=patience,
@@ -203,13 +216,17 @@ def solve_batch(cases: Sequence[Case], *, max_iter=500, tol=1e-5, patience=40,
                                          smooth_passes=0, ijzbcor=3)
             roots[key] = s
         if tune:
+            t = time.perf_counter()
             autotune.tuned(s)
+            clock["tune"] += time.perf_counter() - t
         solvers.append(s)
     torch.cuda.synchronize()
+    clock["setup"] = time.perf_counter() - t_setup - clock["tune"]
     results = [None] * len(solvers)
     pending = list(range(len(solvers)))
     active = []
     while pending or active:
+        t = time.perf_counter()
         while pending and len(active) < max_concurrent:
             idx = pending.pop(0)
             run = _Run(solvers[idx], graph_steps, pool[len(active) % streams], iterate_kw)
@@ -218,15 +235,20 @@ def solve_batch(cases: Sequence[Case], *, max_iter=500, tol=1e-5, patience=40,
                 results[idx] = (run.history, solvers[idx])
             else:
                 active.append((idx, run))
+        clock["start"] += time.perf_counter() - t
+        t = time.perf_counter()
         for _, run in active:
             run.launch()
+        clock["launch"] += time.perf_counter() - t
         still = []
         for idx, run in active:
-            run.collect()
+            run.collect(clock)
             if run.done:
                 results[idx] = (run.history, solvers[idx])
             else:
                 still.append((idx, run))
         active = still
     torch.cuda.synchronize()
+    if stats is not None:
+        stats.update({k: round(v, 4) for k, v in clock.items()})
     return results

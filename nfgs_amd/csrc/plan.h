@@ -27,6 +27,13 @@ struct PairedMatrix {
   void* dig[2] = {nullptr, nullptr};    // [direction] int8 [half][digit][chunk][block][lane][16]
   void* mtail[2] = {nullptr, nullptr};  // [direction] double [half][16*hb] tail row
   void* mrem[2] = {nullptr, nullptr};   // [direction] double [n] row 16*hb of T
+  // Split-bf16 form of E and O for the BF16x3 kernel (FP32 plans): each
+  // entry as hi + lo bf16, laid out as the 16x16x16 MFMA operand of every
+  // (16-row K chunk, 16-column output block): bfr[dir][part][chunk][block]
+  // [lane] holds 4 bf16 (paired rows 4*(lane/16)..+3 of output lane%16);
+  // part 0..3 = E hi, E lo, O hi, O lo.
+  int bchunks = 0, bblocks = 0;
+  void* bfr[2] = {nullptr, nullptr};
 };
 
 struct nfgs_plan {
@@ -42,6 +49,7 @@ struct nfgs_plan {
   void* ax = nullptr;    // nx   2 sin(pi k / (2(nx-1))) / dx
   void* ay = nullptr;    // ny   2 sin(pi k / (2(ny-1))) / dy
   int zchunks = 0;       // z-chunk count of the tiled stencils (0: the cost model)
+  int lateral_arith = 0; // NFGS_LATERAL_* arithmetic of the FP32 lateral transforms
   int pblocks = 0;       // blocks of the reducing kernels (partials per slot)
   double* red = nullptr; // kRedScratch doubles of reduction scratch (not re-entrant)
 };
@@ -54,6 +62,9 @@ int cu_count_stencil();
 // Workgroups of a tiled stencil launch with the plan's current z-chunking.
 long tiled_blocks(const nfgs_plan* p);
 int fill_zero_f64(double* ptr, long n, hipStream_t s);
+// The split-bf16 x3 lateral transform (FP32 data and accumulation).
+int launch_bf16x3(const PairedMatrix* m, int dir, const float* in, long ies, long ivs, long ips,
+                  float* out, long oes, long ovs, long ops, int nvec, long total, hipStream_t s);
 int launch_transform(const nfgs_plan* p, const PairedMatrix* m, int direction, const void* in,
                      long ies, long ivs, long ips, void* out, long oes, long ovs, long ops,
                      int nvec, int nplanes, hipStream_t s);

@@ -131,6 +131,51 @@ int build_digits(int kind, int n, const Type1& T1, PairedMatrix* m) {
   return NFGS_OK;
 }
 
+// bf16 of a value, round to nearest even (finite values)
+uint16_t bf16_of(double v) {
+  const float f = static_cast<float>(v);
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+double bf16_value(uint16_t h) {
+  const uint32_t u = static_cast<uint32_t>(h) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// Split-bf16 operand fragments of E and O (PairedMatrix::bfr) for one
+// direction: entry (K, out) of E or O as hi + lo, hi = bf16(t), lo = bf16(t - hi).
+int build_bf16(const std::vector<double>& et, const std::vector<double>& ot, int dir,
+               PairedMatrix* m) {
+  const int nc = (m->rows + 15) / 16, nb = (m->h + 15) / 16;
+  m->bchunks = nc;
+  m->bblocks = nb;
+  std::vector<uint16_t> f((size_t)4 * nc * nb * 64 * 4, 0);
+  for (int eo = 0; eo < 2; ++eo) {
+    const std::vector<double>& t = eo == 0 ? et : ot;
+    for (int c = 0; c < nc; ++c)
+      for (int b = 0; b < nb; ++b)
+        for (int l = 0; l < 64; ++l)
+          for (int q = 0; q < 4; ++q) {
+            const int K = c * 16 + 4 * (l / 16) + q, o = b * 16 + (l % 16);
+            const double v = (K < m->rows && o < m->h) ? t[(size_t)K * m->cols_pad + o] : 0.0;
+            const uint16_t hi = bf16_of(v);
+            const uint16_t lo = bf16_of(v - bf16_value(hi));
+            // fragment index ((part * nc + c) * nb + b) * 64 + l, 4 bf16 each
+            const size_t base = ((size_t)c * nb + b) * 64 + l, part = (size_t)nc * nb * 64;
+            f[((size_t)(2 * eo) * part + base) * 4 + q] = hi;
+            f[((size_t)(2 * eo + 1) * part + base) * 4 + q] = lo;
+          }
+  }
+  NFGS_HIP_CHECK(hipMalloc(&m->bfr[dir], f.size() * sizeof(uint16_t)));
+  NFGS_HIP_CHECK(hipMemcpy(m->bfr[dir], f.data(), f.size() * sizeof(uint16_t),
+                           hipMemcpyHostToDevice));
+  return NFGS_OK;
+}
+
 int build_paired(int dtype, int kind, int n, PairedMatrix* m) {
   m->n = n;
   m->h = (n + 1) / 2;
@@ -156,6 +201,10 @@ int build_paired(int dtype, int kind, int n, PairedMatrix* m) {
     if (rc) return rc;
     rc = upload<double>(ot, &m->ot[dir]);
     if (rc) return rc;
+    if (dtype == NFGS_F32) {
+      rc = build_bf16(et, ot, dir, m);
+      if (rc) return rc;
+    }
   }
   return dtype == NFGS_F32 ? build_digits(kind, n, T1, m) : NFGS_OK;
 }
@@ -167,7 +216,8 @@ void free_paired(PairedMatrix* m) {
     if (m->dig[d]) (void)hipFree(m->dig[d]);
     if (m->mtail[d]) (void)hipFree(m->mtail[d]);
     if (m->mrem[d]) (void)hipFree(m->mrem[d]);
-    m->et[d] = m->ot[d] = m->dig[d] = m->mtail[d] = m->mrem[d] = nullptr;
+    if (m->bfr[d]) (void)hipFree(m->bfr[d]);
+    m->et[d] = m->ot[d] = m->dig[d] = m->mtail[d] = m->mrem[d] = m->bfr[d] = nullptr;
   }
 }
 
@@ -356,6 +406,21 @@ extern "C" int nfgs_plan_set_tuning(nfgs_plan* p, int knob, int value) {
   }
   nfgs::set_error("plan_set_tuning: unknown knob");
   return NFGS_ERR_ARG;
+}
+
+extern "C" int nfgs_plan_set_lateral_arithmetic(nfgs_plan* p, int arith) {
+  NFGS_REQUIRE(p, NFGS_ERR_ARG, "plan_set_lateral_arithmetic: null plan");
+  NFGS_REQUIRE(arith == NFGS_LATERAL_EXACT || arith == NFGS_LATERAL_BF16X3, NFGS_ERR_ARG,
+               "plan_set_lateral_arithmetic: arithmetic must be NFGS_LATERAL_EXACT or "
+               "NFGS_LATERAL_BF16X3");
+  NFGS_REQUIRE(arith == NFGS_LATERAL_EXACT || p->d.dtype == NFGS_F32, NFGS_ERR_ARG,
+               "plan_set_lateral_arithmetic: the split-bf16 products are for FP32 plans");
+  p->lateral_arith = arith;
+  return NFGS_OK;
+}
+
+extern "C" int nfgs_plan_get_lateral_arithmetic(const nfgs_plan* p) {
+  return p ? p->lateral_arith : -1;
 }
 
 extern "C" int nfgs_plan_get_tuning(const nfgs_plan* p, int knob) {

@@ -1133,6 +1133,12 @@ int launch_transform(const nfgs_plan* p, const PairedMatrix* m, int direction, c
   NFGS_REQUIRE(in != nullptr && out != nullptr && in != out, NFGS_ERR_ARG,
                "transform: in and out must be distinct non-null buffers");
   NFGS_REQUIRE(nvec >= 0 && nplanes >= 0, NFGS_ERR_ARG, "transform: negative counts");
+  if (p->d.dtype == NFGS_F32 && p->lateral_arith == NFGS_LATERAL_BF16X3) {
+    const long total = (long)nvec * nplanes;
+    if (total == 0) return NFGS_OK;
+    return launch_bf16x3(m, direction, static_cast<const float*>(in), ies, ivs, ips,
+                         static_cast<float*>(out), oes, ovs, ops, nvec, total, s);
+  }
   if (p->d.dtype == NFGS_F32)
     return launch_t<double, float>(m, direction, in, ies, ivs, ips, out, oes, ovs, ops, nvec,
                                    nplanes, s);

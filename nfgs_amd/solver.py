@@ -77,6 +77,9 @@ class _Options:
         self.rho = float(pop("bottom_divergence_robin_weight", 0.0))
         self.poloidal_update_mode = _norm(pop("poloidal_update_mode", "direct"))
         self.record_iteration_diagnostics = bool(pop("record_iteration_diagnostics", False))
+        # extension: arithmetic of the FP32 lateral transforms ("exact" digit
+        # or FP64 products; "bf16x3" the split-bf16 precision form)
+        self.lateral_transform_arithmetic = _norm(pop("lateral_transform_arithmetic", "exact"))
         self.soft_cfcf_epsilon = pop("soft_cfcf_epsilon", None)
         self.soft_cfcf_mode_batch_size = int(pop("soft_cfcf_mode_batch_size", 1024))
         self.soft_cfcf_normal_ridge = float(pop("soft_cfcf_normal_ridge", 0.0))
@@ -178,6 +181,8 @@ class _Options:
                              "mutually exclusive")
         if self.poloidal_update_mode not in {"direct", "two_stage"}:
             raise ValueError("poloidal_update_mode must be 'direct' or 'two_stage'")
+        if self.lateral_transform_arithmetic not in {"exact", "bf16x3"}:
+            raise ValueError("lateral_transform_arithmetic must be 'exact' or 'bf16x3'")
         if self.soft_cfcf_epsilon is not None and float(self.soft_cfcf_epsilon) < 0.0:
             raise ValueError("soft_cfcf_epsilon must be non-negative")
         if self.soft_cfcf_mode_batch_size <= 0:
@@ -493,6 +498,9 @@ class NFGSSolverGPU:
                     _unsupported(f"{key}={val!r}")
         if kwargs:
             raise TypeError(f"unexpected keyword arguments: {sorted(kwargs)}")
+        if opts.lateral_transform_arithmetic == "bf16x3" and dtype != torch.float32:
+            raise ValueError("lateral_transform_arithmetic='bf16x3' is the precision form of "
+                             "FP32 solves; use dtype=torch.float32")
         if buffer_thickness not in (None, 0):
             _unsupported("buffer_thickness > 0")
         lw = _norm(lateral_wall)
@@ -732,6 +740,9 @@ class NFGSSolverGPU:
         plan = L.C.c_void_p()
         L.check(self._lib.nfgs_plan_create(L.C.byref(desc), L.C.byref(plan)), "plan")
         self._plan = plan
+        if self.opts.lateral_transform_arithmetic == "bf16x3":
+            L.check(self._lib.nfgs_plan_set_lateral_arithmetic(plan, L.LATERAL_BF16X3),
+                    "lateral_transform_arithmetic")
         self._window = win
         self._partials = torch.zeros(int(self._lib.nfgs_partials_count(plan)), device=self.device,
                                      dtype=torch.float64)
