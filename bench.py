@@ -65,6 +65,9 @@ def parse():
                    help="skip the precision leg (512x512x256: FP32 exact and split-bf16 "
                         "against FP64)")
     p.add_argument("--precision-n", type=int, default=512)
+    p.add_argument("--transforms-only", action="store_true",
+                   help="time one DCT-I pass along x and y at 258x258x257, 386x386x129 and "
+                        "514x514x257 for FP32 exact, FP32 split-bf16 and FP64 plans, and exit")
     p.add_argument("--steps-only", action="store_true",
                    help="profiling workload: set up the 258x258x257 FP32 solve and run only "
                         "--warmup + --steps plain iterations (no solves, no legs)")
@@ -369,6 +372,44 @@ def precision_leg(torch, a):
     return out
 
 
+def transform_timings(torch, L, reps=10):
+    """Average device time (HIP events, torch's stream) of one forward DCT-I
+    pass along x and along y on a random volume, per grid and arithmetic."""
+    import ctypes as C
+    lib = L.load()
+    out = {}
+    for n, nz in ((258, 257), (386, 129), (514, 257)):
+        for tag, f32, arith in (("fp32_exact", True, L.LATERAL_EXACT),
+                                ("fp32_bf16x3", True, L.LATERAL_BF16X3),
+                                ("fp64", False, L.LATERAL_EXACT)):
+            desc = L.GridDesc(n, n, nz, 1.0, 1.0, 1.0, L.NFGS_F32 if f32 else L.NFGS_F64, 0.0,
+                              1.0, 1e-20, 1, 1, 0, 0, 0, 0)
+            plan = C.c_void_p()
+            L.check(lib.nfgs_plan_create(C.byref(desc), C.byref(plan)))
+            L.check(lib.nfgs_plan_set_lateral_arithmetic(plan, arith))
+            dt = torch.float32 if f32 else torch.float64
+            x = torch.randn(nz, n, n, dtype=dt, device="cuda")
+            y = torch.empty_like(x)
+            s = torch.cuda.current_stream().cuda_stream
+            pl = n * n
+            for axis, (es, vs) in (("x", (1, n)), ("y", (n, 1))):
+                def run():
+                    L.check(lib.nfgs_transform(plan, L.NFGS_DCT1, L.NFGS_FORWARD, x.data_ptr(), es,
+                                               vs, pl, y.data_ptr(), es, vs, pl, n, n, nz, s))
+                run()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    run()
+                e1.record()
+                torch.cuda.synchronize()
+                out[f"{n}x{n}x{nz} {tag} DCT {axis}"] = round(e0.elapsed_time(e1) / reps * 1e3, 1)
+            lib.nfgs_plan_destroy(plan)
+            del x, y
+    return out
+
+
 def cpu_threads():
     """Thread counts of the CPU baseline: the host's cores (at most 16, this
     process's share of a one-GPU box) and 8, the reference runner's
@@ -449,6 +490,12 @@ def main():
     from nfgs_amd import _lib as L
     from nfgs_amd.boundary import multipole_boundary
 
+    if a.transforms_only:
+        t = transform_timings(torch, L)
+        for k, v in t.items():
+            log(f"{k}: {v} us")
+        print(json.dumps({"workload": "lateral transform passes", "us": t}))
+        return
     n = a.n
     bx, by, bz, alpha = multipole_boundary(n, seed=a.seed)
     dev = torch.device("cuda")

@@ -1,21 +1,25 @@
 // Lateral type-I transforms (DCT-I / DST-I and inverses) as paired MFMA GEMMs.
 //
-// Both plan dtypes compute in FP64 (v_mfma_f64_16x16x4); FP32 data is widened
-// when staged in LDS and rounded once when stored (see launch_t).
+// The transform matrix is split into its even-column half E and odd-column
+// half O (see PairedMatrix), so each vector costs two (h x n/2) products
+// instead of one (n x n) product, finished with out[k] = E+O,
+// out[n-1-k] = E-O.  Three kernels share that form:
 //
-// A workgroup of 256 lanes (4 wavefronts) takes 64 vectors of length n.  The
-// transform matrix is split into its even-column half E and odd-column half O
-// (see PairedMatrix), so each vector costs two (h x n/2) products instead of
-// one (n x n) product.  The contraction dimension is streamed through LDS in
-// chunks of 16 even + 16 odd elements; each wavefront owns 16 vectors and all
-// output rows of the current group, accumulating with v_mfma 16x16x4 (f32 or
-// f64) and finishing with out[k] = E+O, out[n-1-k] = E-O.
+// * FP32 plans with n <= 258: fixed-point digit products summed exactly on
+//   the int8 MFMA (launch_i8), as accurate as FP64 products for FP32 output;
+// * vectors whose contraction fits the registers (Q <= 33 paired rows per
+//   lane group): the matrix-stationary persistent kernel (launch_v2);
+// * longer vectors: 128 vectors per workgroup, K chunks of the data and of
+//   nine 16-row output blocks of the matrix staged in LDS per step, the next
+//   chunk's loads in flight during the current chunk's MFMAs
+//   (paired_transform_wide).
 //
-// Two layouts share the kernel: ELEM_CONTIG (elements of a vector are
-// adjacent, the x axis) computes C = X * T^T so that lanes hold consecutive
-// output elements; otherwise (vectors adjacent, the y axis) it computes
-// C = T * X so that lanes hold consecutive vectors.  Either way stores are
-// 64-byte runs.
+// The last two compute in FP64 (v_mfma_f64_16x16x4) for both plan dtypes:
+// FP32 data is widened when staged and each output is rounded once when
+// stored.  Two layouts share each kernel: ELEM_CONTIG (elements of a vector
+// are adjacent, the x axis) computes C = X * T^T so that lanes hold
+// consecutive output elements; otherwise (vectors adjacent, the y axis) it
+// computes C = T * X so that lanes hold consecutive vectors.
 
 #include <algorithm>
 #include <type_traits>
@@ -50,150 +54,172 @@ struct Mfma<double> {
   static __device__ __forceinline__ int row(int l4, int r) { return l4 + 4 * r; }
 };
 
-constexpr int kVec = 64;    // vectors per workgroup
-constexpr int kChunk = 16;  // paired rows (j') per LDS chunk
-constexpr int kNB = 9;      // 16-row output blocks per group (144 outputs)
+// Exact-arithmetic (FP64 product) paired transform for long vectors.  A
+// workgroup of 512 lanes (8 wavefronts) takes 128 vectors and one group of
+// NB 16-row output blocks; each wavefront owns 16 vectors.  The contraction
+// runs over chunks of 16 paired rows: the chunk's data (widened to FP64) and
+// the group's 16 x 16*NB matrix entries of E and O are staged in LDS, in two
+// buffers with one barrier per chunk: after its MFMAs on chunk c a wave
+// writes chunk c+1 (held in registers) to the other buffer and issues the
+// loads of chunk c+2 (lds_barrier leaves them in flight).  For f64 16x16x4 lane l
+// supplies A[l%16][l/16] and B[l/16][l%16]; accumulator r holds
+// D[l/16 + 4r][l%16].  Every output is rounded once to the storage type.
+constexpr int kWVec = 128;  // vectors per workgroup
+constexpr int kWNB = 9;     // most 16-row output blocks per group
+constexpr int kWXS = 17;    // LDS stride (doubles) of a vector's 16 paired elements
 
-template <typename T, bool ELEM_CONTIG, typename TS = T>
-__global__ __launch_bounds__(256) void paired_transform_kernel(
+template <typename TS, bool ELEM_CONTIG, int NB>
+__global__ __launch_bounds__(512) void paired_transform_wide(
     const TS* __restrict__ in, long ies, long ivs, long ips, TS* __restrict__ out, long oes,
-    long ovs, long ops, const T* __restrict__ et, const T* __restrict__ ot, int n, int h,
-    int rows, int cols_pad, int nvec, long nvec_total) {
-  using A = typename Mfma<T>::acc;
-  // Data chunk: ELEM_CONTIG -> [vector][j'] (row padded to 17);
-  //             otherwise   -> [j'][vector].
-  constexpr int XS = ELEM_CONTIG ? kVec * (kChunk + 1) : kChunk * kVec;
-  __shared__ T xe[XS];
-  __shared__ T xo[XS];
-  __shared__ T me[kChunk * kNB * 16];
-  __shared__ T mo[kChunk * kNB * 16];
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int l16 = lane & 15;
-  const int l4 = lane >> 4;
-  const long g0 = (long)blockIdx.x * kVec;
-
-  // Base offsets of the 8 (vector, j) slots this thread loads per chunk.
+    long ovs, long ops, const double* __restrict__ et, const double* __restrict__ ot, int n,
+    int h, int rows, int cols_pad, int nvec, long nvec_total, int ngroups) {
+  // LDS row stride (doubles) of a matrix chunk: 16 NB = 16 mod 32, so the
+  // four rows a wavefront reads at once (k = 4 ks + lane/16) fall on two
+  // disjoint halves of the 64 banks, each bank hit twice (the minimum for
+  // 64 eight-byte reads)
+  constexpr int MC = NB * 16;
+  static_assert(NB % 2 == 1, "odd block counts keep the matrix rows conflict-free");
+  constexpr int MS = (16 * NB * 16 + 511) / 512;  // matrix entries per lane per chunk
+  // two buffers: chunk c is read from buffer c&1 while chunk c+1 is written
+  // to the other
+  __shared__ double xe[2][kWVec * kWXS];
+  __shared__ double xo[2][kWVec * kWXS];
+  __shared__ double me[2][16 * MC];
+  __shared__ double mo[2][16 * MC];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, l4 = lane >> 4;
+  // workgroup -> (vector tile, output group): the groups of one tile are
+  // adjacent logical workgroups, so they read the tile's data through one L2
+  const int lt = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = lt / ngroups, grp = lt - tile * ngroups;
+  const long g0 = (long)tile * kWVec;
+  const int b0 = grp * NB;
+  const int nchunks = (rows + 15) >> 4;
+  // loader slot i of a lane: vector sv[i] of the tile, element sj[i] of a
+  // chunk's 32 (paired row sj/2, parity sj&1)
   long ibase[8];
-  int islot_v[8], islot_j[8];
+  int sv[8], sj[8];
+  unsigned vmask = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    const int e = tid + 256 * i;
-    const int v = ELEM_CONTIG ? (e >> 5) : (e & 63);
-    const int jj = ELEM_CONTIG ? (e & 31) : (e >> 6);
-    islot_v[i] = v;
-    islot_j[i] = jj;
-    const long g = g0 + v;
+    const int e = tid + 512 * i;
+    sv[i] = ELEM_CONTIG ? (e >> 5) : (e & (kWVec - 1));
+    sj[i] = ELEM_CONTIG ? (e & 31) : (e >> 7);
+    const long g = g0 + sv[i];
+    ibase[i] = 0;
     if (g < nvec_total) {
       const long p = g / nvec, q = g - p * nvec;
       ibase[i] = p * ips + q * ivs;
-    } else {
-      ibase[i] = -1;
+      vmask |= 1u << i;
     }
   }
-
-  const int hblocks = (h + 15) >> 4;
-  const int ngroups = (hblocks + kNB - 1) / kNB;
-  const int nchunks = (rows + kChunk - 1) / kChunk;
-
-  for (int grp = 0; grp < ngroups; ++grp) {
-    const int b0 = grp * kNB;
-    const int nbg = min(kNB, hblocks - b0);
-    A accE[kNB], accO[kNB];
+  // matrix slot i of a lane: entry (mrow, column) of the group's chunk at
+  // element offset moff + 16*c*cols_pad (the matrices are far below 4 GiB)
+  int moff[MS], mrow[MS];
+  unsigned mmask = 0;
 #pragma unroll
-    for (int b = 0; b < kNB; ++b) {
-      accE[b] = A{0, 0, 0, 0};
-      accO[b] = A{0, 0, 0, 0};
+  for (int i = 0; i < MS; ++i) {
+    const int e = tid + 512 * i;
+    const int r = e / (NB * 16), col = e - r * (NB * 16);
+    mrow[i] = r;
+    moff[i] = r * cols_pad + b0 * 16 + col;
+    if (e < 16 * NB * 16 && b0 * 16 + col < cols_pad) mmask |= 1u << i;
+  }
+  TS rd[8];
+  double rme[MS], rmo[MS];
+  auto load_chunk = [&](int c) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int j = c * 32 + sj[i];
+      rd[i] = ((vmask >> i) & 1) && j < n ? in[ibase[i] + (long)j * ies] : TS(0);
     }
-
-    for (int c = 0; c < nchunks; ++c) {
-      lds_barrier();  // previous chunk fully consumed
-      // ---- stage the data chunk: original j in [32c, 32c+32) ----
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int j = c * 2 * kChunk + islot_j[i];
-        T val = T(0);
-        if (ibase[i] >= 0 && j < n) val = T(in[ibase[i] + (long)j * ies]);
-        const int jp = islot_j[i] >> 1;
-        const int v = islot_v[i];
-        const int idx = ELEM_CONTIG ? v * (kChunk + 1) + jp : jp * kVec + v;
-        if (islot_j[i] & 1)
-          xo[idx] = val;
-        else
-          xe[idx] = val;
-      }
-      // ---- stage the matrix chunk: rows [16c, 16c+16), cols of this group ----
-      const int mcols = kNB * 16;
-      for (int e = tid; e < kChunk * mcols; e += 256) {
-        const int r = e / mcols, col = e - r * mcols;
-        const int gr = c * kChunk + r, gc = b0 * 16 + col;
-        T ve = T(0), vo = T(0);
-        if (gr < rows && gc < cols_pad) {
-          ve = et[(long)gr * cols_pad + gc];
-          vo = ot[(long)gr * cols_pad + gc];
-        }
-        me[e] = ve;
-        mo[e] = vo;
-      }
-      lds_barrier();
-
-      const int valid = min(kChunk, rows - c * kChunk);
-      const int ksteps = (valid + 3) >> 2;
-      for (int ks = 0; ks < ksteps; ++ks) {
-        const int jl = ks * 4 + l4;
-        if (ELEM_CONTIG) {
-          const T aE = xe[(wave * 16 + l16) * (kChunk + 1) + jl];
-          const T aO = xo[(wave * 16 + l16) * (kChunk + 1) + jl];
+    for (int i = 0; i < MS; ++i) {
+      const bool ok = ((mmask >> i) & 1) && mrow[i] < rows - c * 16;
+      const unsigned o = (unsigned)(moff[i] + c * 16 * cols_pad) * 8u;
+      rme[i] = ok ? ld(et, o) : 0.0;
+      rmo[i] = ok ? ld(ot, o) : 0.0;
+    }
+  };
+  auto store_chunk = [&](int buf) {
 #pragma unroll
-          for (int b = 0; b < kNB; ++b) {
-            if (b < nbg) {
-              const T bE = me[jl * mcols + b * 16 + l16];
-              const T bO = mo[jl * mcols + b * 16 + l16];
-              accE[b] = Mfma<T>::mma(aE, bE, accE[b]);
-              accO[b] = Mfma<T>::mma(aO, bO, accO[b]);
-            }
-          }
-        } else {
-          const T bE = xe[jl * kVec + wave * 16 + l16];
-          const T bO = xo[jl * kVec + wave * 16 + l16];
+    for (int i = 0; i < 8; ++i) {
+      const int idx = sv[i] * kWXS + (sj[i] >> 1);
+      if (sj[i] & 1)
+        xo[buf][idx] = (double)rd[i];
+      else
+        xe[buf][idx] = (double)rd[i];
+    }
 #pragma unroll
-          for (int b = 0; b < kNB; ++b) {
-            if (b < nbg) {
-              const T aE = me[jl * mcols + b * 16 + l16];
-              const T aO = mo[jl * mcols + b * 16 + l16];
-              accE[b] = Mfma<T>::mma(aE, bE, accE[b]);
-              accO[b] = Mfma<T>::mma(aO, bO, accO[b]);
-            }
-          }
-        }
+    for (int i = 0; i < MS; ++i) {
+      const int e = tid + 512 * i;
+      if (e < 16 * NB * 16) {
+        const int r = e / (NB * 16), col = e - r * (NB * 16);
+        me[buf][r * MC + col] = rme[i];
+        mo[buf][r * MC + col] = rmo[i];
       }
     }
-
-    // ---- epilogue: out[k] = E + O, out[n-1-k] = E - O ----
+  };
+  f64x4 accE[NB], accO[NB];
 #pragma unroll
-    for (int b = 0; b < kNB; ++b) {
-      if (b >= nbg) continue;
+  for (int b = 0; b < NB; ++b) {
+    accE[b] = f64x4{0, 0, 0, 0};
+    accO[b] = f64x4{0, 0, 0, 0};
+  }
+  load_chunk(0);
+  store_chunk(0);
+  if (nchunks > 1) load_chunk(1);
+  lds_barrier();
+  const int v0 = (wave * 16 + l16) * kWXS;
+  for (int c = 0; c < nchunks; ++c) {
+    const int buf = c & 1;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int k, v;
+    for (int ks = 0; ks < 4; ++ks) {
+      const int k = ks * 4 + l4;
+      const double de = xe[buf][v0 + k], dod = xo[buf][v0 + k];
+      // every block of the group runs (blocks past h multiply zero matrix
+      // entries or rows the epilogue drops), so the loop has no branches
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const double m0 = me[buf][k * MC + b * 16 + l16];
+        const double m1 = mo[buf][k * MC + b * 16 + l16];
         if (ELEM_CONTIG) {
-          v = wave * 16 + Mfma<T>::row(l4, r);
-          k = (b0 + b) * 16 + l16;
+          accE[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(de, m0, accE[b], 0, 0, 0);
+          accO[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(dod, m1, accO[b], 0, 0, 0);
         } else {
-          k = (b0 + b) * 16 + Mfma<T>::row(l4, r);
-          v = wave * 16 + l16;
+          accE[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(m0, de, accE[b], 0, 0, 0);
+          accO[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(m1, dod, accO[b], 0, 0, 0);
         }
-        const long g = g0 + v;
-        if (k < h && g < nvec_total) {
-          const long p = g / nvec, q = g - p * nvec;
-          TS* o = out + p * ops + q * ovs;
-          const T e = accE[b][r], od = accO[b][r];
-          o[(long)k * oes] = TS(e + od);
-          const int k2 = n - 1 - k;
-          if (k2 != k) o[(long)k2 * oes] = TS(e - od);
-        }
+      }
+    }
+    // chunk c+1 into the other buffer (its last readers finished before the
+    // previous barrier), then the loads of chunk c+2
+    if (c + 1 < nchunks) {
+      store_chunk(buf ^ 1);
+      if (c + 2 < nchunks) load_chunk(c + 2);
+    }
+    lds_barrier();
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int k, v;
+      if (ELEM_CONTIG) {
+        v = wave * 16 + l4 + 4 * r;
+        k = (b0 + b) * 16 + l16;
+      } else {
+        k = (b0 + b) * 16 + l4 + 4 * r;
+        v = wave * 16 + l16;
+      }
+      const long g = g0 + v;
+      if (k < h && g < nvec_total) {
+        const long p = g / nvec, q = g - p * nvec;
+        TS* o = out + p * ops + q * ovs;
+        const double e = accE[b][r], od = accO[b][r];
+        o[(long)k * oes] = TS(e + od);
+        const int k2 = n - 1 - k;
+        if (k2 != k) o[(long)k2 * oes] = TS(e - od);
       }
     }
   }
@@ -1069,6 +1095,48 @@ int launch_i8(const PairedMatrix* m, int dir, const float* in, long ies, long iv
 #undef NFGS_I8
 }
 
+template <typename TS, int NB>
+int launch_wide_nb(const PairedMatrix* m, int dir, const TS* in, long ies, long ivs, long ips,
+                   TS* out, long oes, long ovs, long ops, int nvec, long total, int ngroups,
+                   hipStream_t s) {
+  const long nblk = (total + kWVec - 1) / kWVec * ngroups;
+  NFGS_REQUIRE(nblk < (1L << 31), NFGS_ERR_ARG, "transform: too many vectors");
+  const double* et = static_cast<const double*>(m->et[dir]);
+  const double* ot = static_cast<const double*>(m->ot[dir]);
+  if (ies == 1 && oes == 1) {
+    hipLaunchKernelGGL((paired_transform_wide<TS, true, NB>), dim3((unsigned)nblk), dim3(512), 0,
+                       s, in, ies, ivs, ips, out, oes, ovs, ops, et, ot, m->n, m->h, m->rows,
+                       m->cols_pad, nvec, total, ngroups);
+  } else if (ivs == 1 && ovs == 1) {
+    hipLaunchKernelGGL((paired_transform_wide<TS, false, NB>), dim3((unsigned)nblk), dim3(512), 0,
+                       s, in, ies, ivs, ips, out, oes, ovs, ops, et, ot, m->n, m->h, m->rows,
+                       m->cols_pad, nvec, total, ngroups);
+  } else {
+    set_error("transform: need unit element strides or unit vector strides on both sides");
+    return NFGS_ERR_ARG;
+  }
+  NFGS_LAUNCH_CHECK();
+  return NFGS_OK;
+}
+
+// Output blocks split into the fewest groups of at most nine, each group as
+// even as the instantiated widths (5, 7, 9 blocks) allow.
+template <typename TS>
+int launch_wide(const PairedMatrix* m, int dir, const TS* in, long ies, long ivs, long ips,
+                TS* out, long oes, long ovs, long ops, int nvec, long total, hipStream_t s) {
+  const int hblocks = (m->h + 15) / 16;
+  const int ngroups = (hblocks + kWNB - 1) / kWNB;
+  const int need = (hblocks + ngroups - 1) / ngroups;
+  if (need <= 5)
+    return launch_wide_nb<TS, 5>(m, dir, in, ies, ivs, ips, out, oes, ovs, ops, nvec, total,
+                                 ngroups, s);
+  if (need <= 7)
+    return launch_wide_nb<TS, 7>(m, dir, in, ies, ivs, ips, out, oes, ovs, ops, nvec, total,
+                                 ngroups, s);
+  return launch_wide_nb<TS, 9>(m, dir, in, ies, ivs, ips, out, oes, ovs, ops, nvec, total,
+                               ngroups, s);
+}
+
 // Storage type TS (the plan dtype) with arithmetic type T.  FP32 plans run
 // the products in FP64 (T = double): the matrices are FP64, elements are
 // widened on their way into LDS and each output is rounded once when it is
@@ -1080,12 +1148,8 @@ int launch_t(const PairedMatrix* m, int dir, const void* in, long ies, long ivs,
              void* out, long oes, long ovs, long ops, int nvec, int nplanes, hipStream_t s) {
   const long total = (long)nvec * nplanes;
   if (total == 0) return NFGS_OK;
-  const long nblk = (total + kVec - 1) / kVec;
-  NFGS_REQUIRE(nblk < (1L << 31), NFGS_ERR_ARG, "transform: too many vectors");
   const TS* ti = static_cast<const TS*>(in);
   TS* to = static_cast<TS*>(out);
-  const T* et = static_cast<const T*>(m->et[dir]);
-  const T* ot = static_cast<const T*>(m->ot[dir]);
   if constexpr (std::is_same<TS, float>::value) {
     // the digit kernels address both arrays with 32-bit byte offsets
     const long ispan = ((long)(nplanes - 1) * ips + (long)(nvec - 1) * ivs + (long)(m->n - 1) * ies + 1) * 4;
@@ -1107,20 +1171,8 @@ int launch_t(const PairedMatrix* m, int dir, const void* in, long ies, long ivs,
         launch_v2<T, false, TS>(m, dir, ti, ies, ivs, ips, to, oes, ovs, ops, nvec, total, s);
     if (rc >= 0) return rc;
   }
-  if (ies == 1 && oes == 1) {
-    hipLaunchKernelGGL((paired_transform_kernel<T, true, TS>), dim3((unsigned)nblk), dim3(256), 0,
-                       s, ti, ies, ivs, ips, to, oes, ovs, ops, et, ot, m->n, m->h, m->rows,
-                       m->cols_pad, nvec, total);
-  } else if (ivs == 1 && ovs == 1) {
-    hipLaunchKernelGGL((paired_transform_kernel<T, false, TS>), dim3((unsigned)nblk), dim3(256), 0,
-                       s, ti, ies, ivs, ips, to, oes, ovs, ops, et, ot, m->n, m->h, m->rows,
-                       m->cols_pad, nvec, total);
-  } else {
-    set_error("transform: need unit element strides or unit vector strides on both sides");
-    return NFGS_ERR_ARG;
-  }
-  NFGS_LAUNCH_CHECK();
-  return NFGS_OK;
+  // long vectors: 128 vectors per workgroup, nine 16-row output blocks per pass
+  return launch_wide<TS>(m, dir, ti, ies, ivs, ips, to, oes, ovs, ops, nvec, total, s);
 }
 
 }  // namespace

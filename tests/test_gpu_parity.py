@@ -39,7 +39,7 @@ def _plan(nx, ny, nz, dtype):
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("shape", [(9, 14, 14), (5, 13, 17), (3, 66, 66), (3, 258, 258),
-                                   (2, 130, 98), (2, 33, 257)])
+                                   (2, 130, 98), (2, 33, 257), (2, 301, 386)])
 def test_transforms_match_oracle(dtype, shape):
     from nfgs_amd import _lib as L
     nz, ny, nx = shape
