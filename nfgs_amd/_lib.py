@@ -37,6 +37,9 @@ POLOIDAL_DIRECT, POLOIDAL_TWO_STAGE = 0, 1
 DF_WEAK, DF_CLIP, DF_IMPRINT, DF_OBS, DF_COUNT = range(5)
 # lateral transform arithmetic of FP32 plans (NFGS_LATERAL_*)
 LATERAL_EXACT, LATERAL_BF16X3 = 0, 1
+# learned start (NFGS_FNO_*)
+NFGS_FNO_IDENTITY, NFGS_FNO_GELU = 0, 1
+NFGS_FNO_PROJECT, NFGS_FNO_EXPAND = 0, 1
 # nfgs_plan_set_tuning knobs (NFGS_TUNE_*)
 TUNE_ZCHUNKS, TUNE_TRANSFORM_BLOCKS_PER_CU = 0, 1
 
@@ -132,6 +135,29 @@ def _declare(lib):
                                           C.c_int, P(C.c_double), C.c_int, C.c_double, vp, vp]),
         "nfgs_source_diagnostics_workspace_bytes": (C.c_long, [vp]),
         "nfgs_source_diagnostics": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp]),
+        # learned start (nfgs_fno_*)
+        "nfgs_fno_pointwise": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, vp, vp]),
+        "nfgs_fno_pointwise_backward": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int,
+                                                  vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]),
+        "nfgs_fno_outer_sum_workspace_bytes": (C.c_long, [C.c_int, C.c_int]),
+        "nfgs_fno_outer_sum": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_long, vp, vp, C.c_int, vp,
+                                         vp]),
+        "nfgs_fno_axis": (C.c_int, [vp, vp, vp, C.c_long, C.c_int, C.c_long, C.c_int, vp]),
+        "nfgs_fno_mix": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),
+        "nfgs_fno_mix_grad": (C.c_int, [vp, vp, vp, C.c_int, vp]),
+        "nfgs_fno_loss_workspace_bytes": (C.c_long, []),
+        "nfgs_fno_loss": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),
+        "nfgs_fno_adam": (C.c_int, [vp, vp, vp, vp, vp, C.c_long, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, C.c_int, C.c_double, vp]),
+        "nfgs_fno_to_bf16": (C.c_int, [vp, vp, C.c_long, vp]),
+        "nfgs_fno_features": (C.c_int, [C.c_int, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
+                                        C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, vp, vp]),
+        "nfgs_fno_targets": (C.c_int, [C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
+                                       C.c_double, C.c_double, vp, vp]),
+        "nfgs_fno_apply": (C.c_int, [C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                     C.c_double, vp, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -21,6 +21,7 @@ rejects raise its ``ValueError``.
 from __future__ import annotations
 
 import math
+import time
 import warnings
 from typing import Callable, Optional
 
@@ -1003,7 +1004,7 @@ class NFGSSolverGPU:
         return B, J
 
     def compute_initial_conditions(self, *, init_mode="gs_cfcf", init_field=None, init_az=None,
-                                   init_current=None, **_unused):
+                                   init_current=None, learned_start=None, **_unused):
         """gs_cfcf start (nfgs_solver.py:6613-6645); returns (B, J, Phi_c, Az, None).
 
         init_mode="field" (an extension) starts from a caller's field instead:
@@ -1012,11 +1013,18 @@ class NFGSSolverGPU:
         the way the cfcf handoff projects the Stable solver's state
         (nfgs_solver.py:6080-6147).  init_current, when given, is compared
         with the projected state's current.
+
+        init_mode="learned" (an extension) adds a learned correction to the
+        gs_cfcf state (learned_start, an nfgs_amd.learned.SpectralWarmStart)
+        and keeps it only when its sigma_J is below the gs_cfcf state's.
         """
         mode = self._init_mode(init_mode)
         self._require_boundary()
         if mode == "field":
             B, J = self._init_from_field(init_field, init_az, init_current, want_fields=True)
+        elif mode == "learned":
+            self._init_learned(learned_start)
+            B, J = self._state_fields(0)
         else:
             self._advance(0, zero_source=True)
             sigma, _, _, ratio = self._read_metrics()
@@ -1030,10 +1038,11 @@ class NFGSSolverGPU:
         mode = _norm(init_mode)
         if mode == "yi_soft":
             mode = "soft_cfcf"
-        if mode not in ("gs_cfcf", "cfcf", "curl_free_current", "soft_cfcf", "field"):
+        if mode not in ("gs_cfcf", "cfcf", "curl_free_current", "soft_cfcf", "field", "learned"):
             raise ValueError("NFGS supports init_mode='gs_cfcf', 'cfcf', or 'soft_cfcf' "
-                             "(and 'field' for a caller's starting field)")
-        if mode not in ("gs_cfcf", "field"):
+                             "(and 'field' for a caller's starting field, 'learned' for the "
+                             "learned start)")
+        if mode not in ("gs_cfcf", "field", "learned"):
             _unsupported(f"init_mode={init_mode!r}")
         return mode
 
@@ -1109,6 +1118,43 @@ class NFGSSolverGPU:
             return B, J
         del B, J
         return None, None
+
+    def _init_learned(self, net):
+        """Slot 0 from the learned start or gs_cfcf, whichever has the lower
+        sigma_J: the gs_cfcf state goes to slot 0, the network's correction of
+        it to slot 1, each is evaluated by the state pass, and the loser is
+        discarded.  Leaves J_par and the monitors of the kept state."""
+        if net is None or not hasattr(net, "predict_start"):
+            raise ValueError("init_mode='learned' needs learned_start= a "
+                             "nfgs_amd.learned.SpectralWarmStart")
+        lib, plan, s = self._lib, self._plan, self._stream
+        self._advance(0, zero_source=True)
+        sigma_gs, _, _, ratio = self._read_metrics()
+        self._check_zero_mode(ratio)
+        t0 = time.perf_counter()
+        (phi0, az0), (phi1, az1) = self._slots[0], self._slots[1]
+        net.predict_start(self, phi0, az0, phi1, az1)
+
+        def evaluate(phi, az):
+            L.check(lib.nfgs_state_pass(plan, L.C.byref(self._bnd), L.ptr(phi), L.ptr(az), None,
+                                        L.ptr(self._ws_t["jp"]), None, L.ptr(self._partials), s),
+                    "state pass")
+            L.check(lib.nfgs_reduce(plan, L.ptr(self._partials), L.ptr(self._metrics), s), "reduce")
+            return self._read_metrics()[0]
+
+        sigma_l = evaluate(phi1, az1)
+        seconds = time.perf_counter() - t0
+        use = math.isfinite(sigma_l) and sigma_l < sigma_gs
+        if use:
+            self._slots[0], self._slots[1] = self._slots[1], self._slots[0]
+        else:
+            evaluate(phi0, az0)
+        self._init_diagnostics = {
+            "init_mode": "learned", "learned_start_used": bool(use),
+            "sigma_gs_cfcf": float(sigma_gs), "sigma_learned": float(sigma_l),
+            "learned_start_seconds": float(seconds),
+            "gaussian_separation": self.gaussian_separation_diagnostics,
+            "coupled_k0": self._k0_diagnostics}
 
     def warm_start_state(self):
         """(B, Az, J) of the selected state on the calculation grid, for
@@ -1509,13 +1555,14 @@ class NFGSSolverGPU:
                 divergence_factor=2.0, max_rollbacks=12, stop_mode: str = "sigma_best",
                 fixed_point_update_tol=None, fixed_point_pde_tol=1.0e-8,
                 fixed_point_observable_tol=None, fixed_point_patience=10,
-                init_field=None, init_az=None, init_current=None):
+                init_field=None, init_az=None, init_current=None, learned_start=None):
         mode = _norm(init_mode)
         if mode == "yi_soft":
             mode = "soft_cfcf"
-        if mode not in ("gs_cfcf", "cfcf", "curl_free_current", "soft_cfcf", "field"):
+        if mode not in ("gs_cfcf", "cfcf", "curl_free_current", "soft_cfcf", "field", "learned"):
             raise ValueError("NFGS supports init_mode='gs_cfcf', 'cfcf', or 'soft_cfcf' "
-                             "(and 'field' for a caller's starting field)")
+                             "(and 'field' for a caller's starting field, 'learned' for the "
+                             "learned start)")
         stop_mode = _norm(stop_mode)
         if stop_mode == "fixed_point" and not self.record_iteration_diagnostics:
             raise ValueError("stop_mode='fixed_point' requires record_iteration_diagnostics=True")
@@ -1544,6 +1591,11 @@ class NFGSSolverGPU:
         if mode == "field":
             self._init_from_field(init_field, init_az, init_current)
             sigma0, divB0, E0, _ = self._read_metrics()  # no source zero mode at the start
+            if diag is not None:
+                self._state_fields(0, B_out=self._Bwork)
+        elif mode == "learned":
+            self._init_learned(learned_start)
+            sigma0, divB0, E0, _ = self._read_metrics()
             if diag is not None:
                 self._state_fields(0, B_out=self._Bwork)
         else:
