@@ -527,17 +527,19 @@ int nfgs_outward_errors(const double* b, const double* c, int ny, int nx, int y0
 /* ------------------------------------------------------------------------
  * Learned warm start.  No counterpart in the reference: its iterate() starts
  * only from gs_cfcf (nfgs_solver.py:6613-6645).  These are the kernels of a
- * small cosine-basis spectral neural operator that maps the gs_cfcf scalars,
- * the bottom planes and rho to a correction of (Phi_c, Az); the host
- * (nfgs_amd.learned) composes them into the network's forward and backward
- * passes and its optimiser step.  FP32 activations (C, nz, ny, nx) with x
- * fastest; bfloat16 weights (uint16_t bit patterns); every grid axis >= 16.
+ * small cosine-basis spectral neural operator that maps the gs_cfcf state
+ * (B and the parallel current J_par) and rho to a correction of J_par; one
+ * solver step (nfgs_advance_ex) from the corrected current gives the start.
+ * The host (nfgs_amd.learned) composes them into the network's forward and
+ * backward passes and its optimiser step.  FP32 activations (C, nz, ny, nx)
+ * with x fastest; bfloat16 weights (uint16_t bit patterns); every grid axis
+ * >= 16.
  * ------------------------------------------------------------------------ */
 #define NFGS_FNO_WIDTH 32
 #define NFGS_FNO_MODES 16
 #define NFGS_FNO_FEATURES 8
 #define NFGS_FNO_HIDDEN 64
-#define NFGS_FNO_OUTPUTS 2
+#define NFGS_FNO_OUTPUTS 3
 #define NFGS_FNO_IDENTITY 0
 #define NFGS_FNO_GELU 1
 #define NFGS_FNO_PROJECT 0
@@ -546,7 +548,7 @@ int nfgs_outward_errors(const double* b, const double* c, int ny, int nx, int y0
 /* out[o] = act(W in + b + E_x spec), W (cout, cin) bfloat16, spec (cout,
  * nz*ny, 16) or NULL, basis_x (nx, 16).  Shapes (cin, cout, act, spec):
  * (8,32,id,-) lift, (32,32,gelu|id,spec) spectral layer, (32,64,gelu,-) and
- * (64,2,id,-) head. */
+ * (64,3,id,-) head. */
 int nfgs_fno_pointwise(const float* in, int cin, const uint16_t* weight, const float* bias,
                        const float* spec, const float* basis_x, int cout, int activation,
                        int nx, int ny, int nz, float* out, void* stream);
@@ -559,7 +561,7 @@ int nfgs_fno_pointwise_backward(const float* in, int cin, const uint16_t* weight
                                 const float* grad_out_spec, int nx, int ny, int nz,
                                 float* grad_z, float* grad_in, void* stream);
 /* grad_w[o][i] (+)= sum_p a[o][p] b[i][p], grad_b[o] (+)= sum_p a[o][p];
- * (ca, cb) in {(32,8), (32,32), (64,32), (2,64)}. */
+ * (ca, cb) in {(32,8), (32,32), (64,32), (3,64)}. */
 long nfgs_fno_outer_sum_workspace_bytes(int ca, int cb);
 int nfgs_fno_outer_sum(const float* a, int ca, const float* b, int cb, long npoints,
                        float* grad_w, float* grad_b, int accumulate, void* workspace,
@@ -574,33 +576,25 @@ int nfgs_fno_mix(const uint16_t* weight, const float* in, float* out, int nmodes
 /* grad_weight[m][o][i] += grad_out[o][m] in[i][m]. */
 int nfgs_fno_mix_grad(const float* grad_out, const float* in, float* grad_weight, int nmodes,
                       void* stream);
-/* Mean squared error over the free nodes (Phi_c above the bottom plane, Az
- * inside the walls, bottom and top) of the two output channels, and its
- * gradient; loss is one device double. */
+/* Mean squared error of the n output values and its gradient; loss is one
+ * device double. */
 long nfgs_fno_loss_workspace_bytes(void);
-int nfgs_fno_loss(const float* out, const float* target, int nx, int ny, int nz, float* grad_out,
-                  double* loss, void* workspace, void* stream);
+int nfgs_fno_loss(const float* out, const float* target, long n, float* grad_out, double* loss,
+                  void* workspace, void* stream);
 /* Adam step on FP32 master weights; writes the bfloat16 copy and clears grad. */
 int nfgs_fno_adam(float* master, uint16_t* weight, float* m, float* v, float* grad, long n,
                   double lr, double beta1, double beta2, double eps, int step, double grad_scale,
                   void* stream);
 int nfgs_fno_to_bf16(const float* src, uint16_t* dst, long n, void* stream);
-/* The eight input channels: Phi_c c_phi, Az c_az, the bottom planes Bx, By,
- * Bz (c_b) and Jz (c_jz) repeated over z, rho, z / (nz - 1).  dtype is the
- * solver's (NFGS_F32 or NFGS_F64). */
-int nfgs_fno_features(int dtype, const void* phi, const void* az, const void* bx, const void* by,
-                      const void* bz, const void* jz, int nx, int ny, int nz, double c_phi,
-                      double c_az, double c_b, double c_jz, double rho, float* feat,
-                      void* stream);
-/* Training targets: (Phi_c - gs Phi_c) inv_scale_phi and (Az - gs Az)
- * inv_scale_az on the free nodes, zero elsewhere. */
-int nfgs_fno_targets(int dtype, const void* gs_phi, const void* gs_az, const void* phi,
-                     const void* az, int nx, int ny, int nz, double inv_scale_phi,
-                     double inv_scale_az, float* target, void* stream);
-/* Start state: gs + scale * net on the free nodes; the fixed planes keep the
- * gs values (the slots' boundary planes). */
-int nfgs_fno_apply(int dtype, const void* gs_phi, const void* gs_az, const float* net, int nx,
-                   int ny, int nz, double scale_phi, double scale_az, void* phi, void* az,
+/* The eight input channels: B (3*N) times c_b, J_par (3*N) times c_j, rho,
+ * z / (nz - 1).  dtype is the solver's (NFGS_F32 or NFGS_F64). */
+int nfgs_fno_features(int dtype, const void* b, const void* jp, int nx, int ny, int nz, double c_b,
+                      double c_j, double rho, float* feat, void* stream);
+/* Training target (jp - jp_gs) inv_scale over n values. */
+int nfgs_fno_targets(int dtype, const void* jp_gs, const void* jp, long n, double inv_scale,
+                     float* target, void* stream);
+/* Start current jp = jp_gs + scale * net over n values (jp may be jp_gs). */
+int nfgs_fno_apply(int dtype, const void* jp_gs, const float* net, long n, double scale, void* jp,
                    void* stream);
 
 #ifdef __cplusplus

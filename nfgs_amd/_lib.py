@@ -147,17 +147,14 @@ def _declare(lib):
         "nfgs_fno_mix": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),
         "nfgs_fno_mix_grad": (C.c_int, [vp, vp, vp, C.c_int, vp]),
         "nfgs_fno_loss_workspace_bytes": (C.c_long, []),
-        "nfgs_fno_loss": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),
+        "nfgs_fno_loss": (C.c_int, [vp, vp, C.c_long, vp, vp, vp, vp]),
         "nfgs_fno_adam": (C.c_int, [vp, vp, vp, vp, vp, C.c_long, C.c_double, C.c_double,
                                     C.c_double, C.c_double, C.c_int, C.c_double, vp]),
         "nfgs_fno_to_bf16": (C.c_int, [vp, vp, C.c_long, vp]),
-        "nfgs_fno_features": (C.c_int, [C.c_int, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
-                                        C.c_double, C.c_double, C.c_double, C.c_double,
-                                        C.c_double, vp, vp]),
-        "nfgs_fno_targets": (C.c_int, [C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
-                                       C.c_double, C.c_double, vp, vp]),
-        "nfgs_fno_apply": (C.c_int, [C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_double,
-                                     C.c_double, vp, vp, vp]),
+        "nfgs_fno_features": (C.c_int, [C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                        C.c_double, C.c_double, vp, vp]),
+        "nfgs_fno_targets": (C.c_int, [C.c_int, vp, vp, C.c_long, C.c_double, vp, vp]),
+        "nfgs_fno_apply": (C.c_int, [C.c_int, vp, vp, C.c_long, C.c_double, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -1,16 +1,17 @@
 // Learned warm start: the kernels of a small cosine-basis spectral neural
-// operator (forward, backward and its optimiser step) that maps a boundary,
-// its gs_cfcf state and rho to a correction of the NFGS scalars (Phi_c, Az).
-// No counterpart in the reference, whose iterate() only starts from gs_cfcf
-// (nfgs_solver.py:6613-6645); the result enters the iteration as its slot-0
-// state.
+// operator (forward, backward and its optimiser step) that maps the gs_cfcf
+// state (its B and parallel current J_par) and rho to a correction of J_par
+// towards the converged state's.  One solver step from the corrected current
+// (the exact Poisson solves) then gives the start state.  No counterpart in
+// the reference, whose iterate() only starts from gs_cfcf
+// (nfgs_solver.py:6613-6645).
 //
 // Network (fixed shape): lift 8 -> 32 channels; four spectral layers
 //   v' = act(W v + b + E_x E_y E_z (R_mode (P_z P_y P_x v)))
 // with P_a the truncated orthonormal DCT-II along axis a (16 modes, a dense
 // n x 16 basis) and E_a = P_a^T, R a 32 x 32 channel mix per retained mode
 // (16^3 modes), act = GELU except after the last layer; head 32 -> 64 (GELU)
-// -> 2.  Weights are bfloat16 (FP32 master copies in the optimiser),
+// -> 3.  Weights are bfloat16 (FP32 master copies in the optimiser),
 // activations and accumulation FP32.
 //
 // Layouts: a volume of C channels is (C, nz, ny, nx), x fastest.  The
@@ -416,33 +417,21 @@ __global__ __launch_bounds__(256) void mix_grad_kernel(const float* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Loss: the mean over the free nodes of both channels of (out - target)^2.
-// Channel 0 (Phi_c) is free above the bottom plane; channel 1 (Az) inside
-// the walls, the bottom and the top plane.  gout = d loss / d out.
+// Loss: the mean of (out - target)^2 over every node of the output channels;
+// gout = d loss / d out.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool free_phi(int z) { return z >= 1; }
-__device__ __forceinline__ bool free_az(int z, int y, int x, int nz, int ny, int nx) {
-  return z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2;
-}
-
 constexpr int kLossBlocks = 256;
 
 __global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ out,
-                                                   const float* __restrict__ tgt, int nx, int ny,
-                                                   int nz, float scale, float* __restrict__ gout,
+                                                   const float* __restrict__ tgt, long n,
+                                                   float scale, float* __restrict__ gout,
                                                    double* __restrict__ part) {
   __shared__ double sh[256];
-  const long pl = (long)nx * ny, N = pl * nz;
   double acc = 0.0;
-  for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < N; p += (long)gridDim.x * 256) {
-    const int z = (int)(p / pl);
-    const long r = p - (long)z * pl;
-    const int y = (int)(r / nx), x = (int)(r - (long)y * nx);
-    const float d0 = free_phi(z) ? out[p] - tgt[p] : 0.0f;
-    const float d1 = free_az(z, y, x, nz, ny, nx) ? out[N + p] - tgt[N + p] : 0.0f;
-    gout[p] = 2.0f * scale * d0;
-    gout[N + p] = 2.0f * scale * d1;
-    acc += (double)d0 * d0 + (double)d1 * d1;
+  for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long)gridDim.x * 256) {
+    const float d = out[p] - tgt[p];
+    gout[p] = 2.0f * scale * d;
+    acc += (double)d * d;
   }
   sh[threadIdx.x] = acc;
   __syncthreads();
@@ -494,50 +483,33 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(const float* __restrict__ 
 // Inputs, targets and the start state.
 // ---------------------------------------------------------------------------
 template <typename T>
-int features_t(const T* phi, const T* az, const T* bx, const T* by, const T* bz, const T* jz,
-               int nx, int ny, int nz, float cphi, float caz, float cb, float cj, float rho,
+int features_t(const T* b, const T* jp, int nx, int ny, int nz, float cb, float cj, float rho,
                float* feat, hipStream_t s) {
   const long pl = (long)nx * ny, N = pl * nz;
   const float zs = nz > 1 ? 1.0f / (float)(nz - 1) : 0.0f;
   return launch_map(N, [=] __device__(long p) {
     const int z = (int)(p / pl);
-    const long r = p - (long)z * pl;
-    feat[p] = (float)phi[p] * cphi;
-    feat[N + p] = (float)az[p] * caz;
-    feat[2 * N + p] = (float)bx[r] * cb;
-    feat[3 * N + p] = (float)by[r] * cb;
-    feat[4 * N + p] = (float)bz[r] * cb;
-    feat[5 * N + p] = (float)jz[r] * cj;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      feat[c * N + p] = (float)b[c * N + p] * cb;
+      feat[(3 + c) * N + p] = (float)jp[c * N + p] * cj;
+    }
     feat[6 * N + p] = rho;
     feat[7 * N + p] = (float)z * zs;
   }, s);
 }
 
 template <typename T>
-int targets_t(const T* gphi, const T* gaz, const T* phi, const T* az, int nx, int ny, int nz,
-              double sphi, double saz, float* tgt, hipStream_t s) {
-  const long pl = (long)nx * ny, N = pl * nz;
-  return launch_map(N, [=] __device__(long p) {
-    const int z = (int)(p / pl);
-    const long r = p - (long)z * pl;
-    const int y = (int)(r / nx), x = (int)(r - (long)y * nx);
-    tgt[p] = free_phi(z) ? (float)(((double)phi[p] - (double)gphi[p]) * sphi) : 0.0f;
-    tgt[N + p] = free_az(z, y, x, nz, ny, nx)
-                     ? (float)(((double)az[p] - (double)gaz[p]) * saz) : 0.0f;
+int targets_t(const T* jp_gs, const T* jp, long n, double inv_scale, float* tgt, hipStream_t s) {
+  return launch_map(n, [=] __device__(long p) {
+    tgt[p] = (float)(((double)jp[p] - (double)jp_gs[p]) * inv_scale);
   }, s);
 }
 
 template <typename T>
-int apply_t(const T* gphi, const T* gaz, const float* net, int nx, int ny, int nz, double sphi,
-            double saz, T* phi, T* az, hipStream_t s) {
-  const long pl = (long)nx * ny, N = pl * nz;
-  return launch_map(N, [=] __device__(long p) {
-    const int z = (int)(p / pl);
-    const long r = p - (long)z * pl;
-    const int y = (int)(r / nx), x = (int)(r - (long)y * nx);
-    phi[p] = free_phi(z) ? (T)((double)gphi[p] + sphi * (double)net[p]) : gphi[p];
-    az[p] = free_az(z, y, x, nz, ny, nx) ? (T)((double)gaz[p] + saz * (double)net[N + p])
-                                         : gaz[p];
+int apply_t(const T* jp_gs, const float* net, long n, double scale, T* jp, hipStream_t s) {
+  return launch_map(n, [=] __device__(long p) {
+    jp[p] = (T)((double)jp_gs[p] + scale * (double)net[p]);
   }, s);
 }
 
@@ -575,7 +547,7 @@ int nfgs_fno_pointwise(const float* in, int cin, const uint16_t* weight, const f
   else if (cin == kW && cout == kW && activation == NFGS_FNO_GELU && sp) PWF(kW, kW, 1, true);
   else if (cin == kW && cout == kW && activation == NFGS_FNO_IDENTITY && sp) PWF(kW, kW, 0, true);
   else if (cin == kW && cout == 64 && activation == NFGS_FNO_GELU && !sp) PWF(kW, 64, 1, false);
-  else if (cin == 64 && cout == 2 && activation == NFGS_FNO_IDENTITY && !sp) PWF(64, 2, 0, false);
+  else if (cin == 64 && cout == 3 && activation == NFGS_FNO_IDENTITY && !sp) PWF(64, 3, 0, false);
   else NFGS_REQUIRE(false, NFGS_ERR_ARG, "fno pointwise: unsupported layer shape");
 #undef PWF
   NFGS_LAUNCH_CHECK();
@@ -603,7 +575,7 @@ int nfgs_fno_pointwise_backward(const float* in, int cin, const uint16_t* weight
 #define PWB(CI, CO, A, S, GS)                                                                  \
   hipLaunchKernelGGL((pw_backward<CI, CO, A, S, GS>), grid, blk, 0, s, in, weight, bias, spec,   \
                      basis_x, nx, rows, grad_out, grad_out_spec, grad_z, grad_in)
-  if (cin == 64 && cout == 2 && !gelu_act && !sp && !gs) PWB(64, 2, 0, false, false);
+  if (cin == 64 && cout == 3 && !gelu_act && !sp && !gs) PWB(64, 3, 0, false, false);
   else if (cin == kW && cout == 64 && gelu_act && !sp && !gs) PWB(kW, 64, 1, false, false);
   else if (cin == kW && cout == kW && !gelu_act && !gs) PWB(kW, kW, 0, false, false);
   else if (cin == kW && cout == kW && gelu_act && sp && gs) PWB(kW, kW, 1, true, true);
@@ -632,7 +604,7 @@ int nfgs_fno_outer_sum(const float* a, int ca, const float* b, int cb, long npoi
   if (ca == kW && cb == 8) OS(kW, 8);
   else if (ca == kW && cb == kW) OS(kW, kW);
   else if (ca == 64 && cb == kW) OS(64, kW);
-  else if (ca == 2 && cb == 64) OS(2, 64);
+  else if (ca == 3 && cb == 64) OS(3, 64);
   else NFGS_REQUIRE(false, NFGS_ERR_ARG, "fno outer sum: unsupported shape");
 #undef OS
   NFGS_LAUNCH_CHECK();
@@ -702,16 +674,14 @@ int nfgs_fno_mix_grad(const float* grad_out, const float* in, float* grad_weight
 
 long nfgs_fno_loss_workspace_bytes(void) { return (long)kLossBlocks * (long)sizeof(double); }
 
-int nfgs_fno_loss(const float* out, const float* target, int nx, int ny, int nz, float* grad_out,
-                  double* loss, void* workspace, void* stream) {
-  NFGS_REQUIRE(out && target && grad_out && loss && workspace, NFGS_ERR_ARG,
-               "fno loss: null pointer");
-  NFGS_REQUIRE(nx >= 3 && ny >= 3 && nz >= 3, NFGS_ERR_ARG, "fno loss: grid too small");
-  const long count = (long)(nz - 1) * ny * nx + (long)(nz - 2) * (ny - 2) * (nx - 2);
-  const double scale = 1.0 / (double)count;
+int nfgs_fno_loss(const float* out, const float* target, long n, float* grad_out, double* loss,
+                  void* workspace, void* stream) {
+  NFGS_REQUIRE(out && target && grad_out && loss && workspace && n > 0, NFGS_ERR_ARG,
+               "fno loss: null pointer or empty output");
+  const double scale = 1.0 / (double)n;
   hipStream_t s = FNO_STREAM(stream);
   double* part = static_cast<double*>(workspace);
-  hipLaunchKernelGGL(loss_kernel, dim3(kLossBlocks), dim3(256), 0, s, out, target, nx, ny, nz,
+  hipLaunchKernelGGL(loss_kernel, dim3(kLossBlocks), dim3(256), 0, s, out, target, n,
                      (float)scale, grad_out, part);
   NFGS_LAUNCH_CHECK();
   hipLaunchKernelGGL(loss_sum_kernel, dim3(1), dim3(64), 0, s, part, kLossBlocks, scale, loss);
@@ -740,52 +710,37 @@ int nfgs_fno_to_bf16(const float* src, uint16_t* dst, long n, void* stream) {
   return NFGS_OK;
 }
 
-int nfgs_fno_features(int dtype, const void* phi, const void* az, const void* bx, const void* by,
-                      const void* bz, const void* jz, int nx, int ny, int nz, double c_phi,
-                      double c_az, double c_b, double c_jz, double rho, float* feat,
-                      void* stream) {
-  NFGS_REQUIRE(phi && az && bx && by && bz && jz && feat, NFGS_ERR_ARG,
-               "fno features: null pointer");
+int nfgs_fno_features(int dtype, const void* b, const void* jp, int nx, int ny, int nz, double c_b,
+                      double c_j, double rho, float* feat, void* stream) {
+  NFGS_REQUIRE(b && jp && feat, NFGS_ERR_ARG, "fno features: null pointer");
   FNO_GRID_CHECK(nx, ny, nz);
   hipStream_t s = FNO_STREAM(stream);
   if (dtype == NFGS_F32)
-    return features_t<float>((const float*)phi, (const float*)az, (const float*)bx,
-                             (const float*)by, (const float*)bz, (const float*)jz, nx, ny, nz,
-                             (float)c_phi, (float)c_az, (float)c_b, (float)c_jz, (float)rho, feat,
-                             s);
+    return features_t<float>((const float*)b, (const float*)jp, nx, ny, nz, (float)c_b,
+                             (float)c_j, (float)rho, feat, s);
   NFGS_REQUIRE(dtype == NFGS_F64, NFGS_ERR_ARG, "fno features: dtype must be F32 or F64");
-  return features_t<double>((const double*)phi, (const double*)az, (const double*)bx,
-                            (const double*)by, (const double*)bz, (const double*)jz, nx, ny, nz,
-                            (float)c_phi, (float)c_az, (float)c_b, (float)c_jz, (float)rho, feat,
-                            s);
+  return features_t<double>((const double*)b, (const double*)jp, nx, ny, nz, (float)c_b,
+                            (float)c_j, (float)rho, feat, s);
 }
 
-int nfgs_fno_targets(int dtype, const void* gs_phi, const void* gs_az, const void* phi,
-                     const void* az, int nx, int ny, int nz, double inv_scale_phi,
-                     double inv_scale_az, float* target, void* stream) {
-  NFGS_REQUIRE(gs_phi && gs_az && phi && az && target, NFGS_ERR_ARG, "fno targets: null pointer");
-  FNO_GRID_CHECK(nx, ny, nz);
+int nfgs_fno_targets(int dtype, const void* jp_gs, const void* jp, long n, double inv_scale,
+                     float* target, void* stream) {
+  NFGS_REQUIRE(jp_gs && jp && target && n > 0, NFGS_ERR_ARG, "fno targets: null pointer");
   hipStream_t s = FNO_STREAM(stream);
   if (dtype == NFGS_F32)
-    return targets_t<float>((const float*)gs_phi, (const float*)gs_az, (const float*)phi,
-                            (const float*)az, nx, ny, nz, inv_scale_phi, inv_scale_az, target, s);
+    return targets_t<float>((const float*)jp_gs, (const float*)jp, n, inv_scale, target, s);
   NFGS_REQUIRE(dtype == NFGS_F64, NFGS_ERR_ARG, "fno targets: dtype must be F32 or F64");
-  return targets_t<double>((const double*)gs_phi, (const double*)gs_az, (const double*)phi,
-                           (const double*)az, nx, ny, nz, inv_scale_phi, inv_scale_az, target, s);
+  return targets_t<double>((const double*)jp_gs, (const double*)jp, n, inv_scale, target, s);
 }
 
-int nfgs_fno_apply(int dtype, const void* gs_phi, const void* gs_az, const float* net, int nx,
-                   int ny, int nz, double scale_phi, double scale_az, void* phi, void* az,
+int nfgs_fno_apply(int dtype, const void* jp_gs, const float* net, long n, double scale, void* jp,
                    void* stream) {
-  NFGS_REQUIRE(gs_phi && gs_az && net && phi && az, NFGS_ERR_ARG, "fno apply: null pointer");
-  FNO_GRID_CHECK(nx, ny, nz);
+  NFGS_REQUIRE(jp_gs && net && jp && n > 0, NFGS_ERR_ARG, "fno apply: null pointer");
   hipStream_t s = FNO_STREAM(stream);
   if (dtype == NFGS_F32)
-    return apply_t<float>((const float*)gs_phi, (const float*)gs_az, net, nx, ny, nz, scale_phi,
-                          scale_az, (float*)phi, (float*)az, s);
+    return apply_t<float>((const float*)jp_gs, net, n, scale, (float*)jp, s);
   NFGS_REQUIRE(dtype == NFGS_F64, NFGS_ERR_ARG, "fno apply: dtype must be F32 or F64");
-  return apply_t<double>((const double*)gs_phi, (const double*)gs_az, net, nx, ny, nz, scale_phi,
-                         scale_az, (double*)phi, (double*)az, s);
+  return apply_t<double>((const double*)jp_gs, net, n, scale, (double*)jp, s);
 }
 
 }  // extern "C"

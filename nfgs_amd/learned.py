@@ -1,10 +1,12 @@
 """Learned warm start for the NFGS iterate (an extension; the reference starts
 only from gs_cfcf, nfgs_solver.py:6613-6645).
 
-A small cosine-basis spectral neural operator maps what the solver knows
-before its first step (the gs_cfcf scalars Phi_c and Az, the bottom planes Bx,
-By, Bz, Jz on the calculation grid and the Robin weight rho) to a correction
-of (Phi_c, Az) towards the converged state.  ``iterate(init_mode="learned",
+A small cosine-basis spectral neural operator maps what the solver has after
+its gs_cfcf start (that state's B and parallel current J_par, and the Robin
+weight rho) to a correction of J_par towards the converged state's.  One
+solver step from the corrected current (the exact Poisson solves of the
+iteration) gives the learned start, so prediction errors reach the state only
+through the inverse Laplacians.  ``iterate(init_mode="learned",
 learned_start=net)`` evaluates sigma_J of both starts and keeps the learned
 one only when its sigma_J is lower; otherwise the solve is the plain gs_cfcf
 solve.  The iteration itself is unchanged, so the converged result is judged
@@ -12,7 +14,7 @@ by the same monitors and stop rules.
 
 Network: lift 8 -> 32, four spectral layers (truncated orthonormal DCT-II of
 16 modes per axis, a 32 x 32 mix per mode, a pointwise 32 x 32 path, GELU
-between layers), head 32 -> 64 -> 2.  Every pass runs in the library's
+between layers), head 32 -> 64 -> 3.  Every pass runs in the library's
 kernels (nfgs_fno_*, nfgs_amd/csrc/fno.hip): bfloat16 weights, FP32
 activations and accumulation; Adam keeps FP32 master weights.  Training pairs
 come from the solver itself (``generate_pairs``).
@@ -29,7 +31,7 @@ import torch
 
 from . import _lib as L
 
-WIDTH, MODES, FEATURES, HIDDEN, OUTPUTS, LAYERS = 32, 16, 8, 64, 2, 4
+WIDTH, MODES, FEATURES, HIDDEN, OUTPUTS, LAYERS = 32, 16, 8, 64, 3, 4
 NMODES = MODES ** 3
 
 
@@ -69,17 +71,15 @@ def init_parameters(seed: int = 0) -> dict:
 
 @dataclass
 class Normalisation:
-    """Fixed channel scales, in units of the boundary's RMS Bz (s_B): the
-    gs_cfcf scalars are divided by s_B * phi / az, Jz by s_B * jz, the
-    targets by s_B * out_phi / out_az."""
-    phi: float = 1.0
-    az: float = 1.0
-    jz: float = 1.0
-    out_phi: float = 1.0
-    out_az: float = 1.0
+    """Fixed channel scales, in units of the boundary's RMS Bz (s_B): B is
+    divided by s_B * b, J_par by s_B * j and the target correction of J_par by
+    s_B * out."""
+    b: float = 1.0
+    j: float = 1.0
+    out: float = 1.0
 
     def as_array(self):
-        return np.array([self.phi, self.az, self.jz, self.out_phi, self.out_az])
+        return np.array([self.b, self.j, self.out])
 
 
 def _bz_scale(solver) -> float:
@@ -118,8 +118,8 @@ class _Buffers:
             self.gspec = [torch.empty(WIDTH * nz * ny * MODES, **f32) for _ in range(2)]
             self.gw = torch.empty(WIDTH, NMODES, **f32)
             self.guh = torch.empty(WIDTH, NMODES, **f32)
-            ws = max(int(L.load().nfgs_fno_outer_sum_workspace_bytes(a, b))
-                     for a, b in ((WIDTH, FEATURES), (WIDTH, WIDTH), (HIDDEN, WIDTH),
+            ws = max(int(L.load().nfgs_fno_outer_sum_workspace_bytes(a, c))
+                     for a, c in ((WIDTH, FEATURES), (WIDTH, WIDTH), (HIDDEN, WIDTH),
                                   (OUTPUTS, HIDDEN)))
             self.os_ws = torch.empty(ws, device=device, dtype=torch.uint8)
             self.loss_ws = torch.empty(int(L.load().nfgs_fno_loss_workspace_bytes()),
@@ -207,13 +207,13 @@ class SpectralWarmStart:
         return self._bufs[key]
 
     # -- passes ---------------------------------------------------------------
-    def features(self, b, dtype_code, phi, az, bx, by, bz, jz, rho, sb):
+    def features(self, b, dtype_code, B, jp, rho, sb):
+        """The network's inputs from a state's B and J_par (solver dtype)."""
         nz, ny, nx = b.grid
         n = self.norm
-        L.check(self._lib.nfgs_fno_features(
-            dtype_code, L.ptr(phi), L.ptr(az), L.ptr(bx), L.ptr(by), L.ptr(bz), L.ptr(jz),
-            nx, ny, nz, 1.0 / (sb * n.phi), 1.0 / (sb * n.az), 1.0 / sb, 1.0 / (sb * n.jz),
-            float(rho), L.ptr(b.feat), self._stream), "learned-start features")
+        L.check(self._lib.nfgs_fno_features(dtype_code, L.ptr(B), L.ptr(jp), nx, ny, nz,
+                                            1.0 / (sb * n.b), 1.0 / (sb * n.j), float(rho),
+                                            L.ptr(b.feat), self._stream), "learned-start features")
 
     def forward(self, b):
         """Network output into b.out from b.feat; with training buffers every
@@ -271,7 +271,7 @@ class SpectralWarmStart:
         P, E = L.NFGS_FNO_PROJECT, L.NFGS_FNO_EXPAND
         chk = L.check
         ws = L.ptr(b.os_ws)
-        chk(lib.nfgs_fno_loss(L.ptr(b.out), L.ptr(target), nx, ny, nz, L.ptr(b.gout),
+        chk(lib.nfgs_fno_loss(L.ptr(b.out), L.ptr(target), OUTPUTS * N, L.ptr(b.gout),
                               L.ptr(b.loss), L.ptr(b.loss_ws), s), "loss")
         # head
         chk(lib.nfgs_fno_pointwise_backward(L.ptr(b.h), HIDDEN, self._w("head2.w"),
@@ -337,20 +337,17 @@ class SpectralWarmStart:
                 "adam")
 
     # -- the solver-facing start ---------------------------------------------
-    def predict_start(self, solver, gs_phi, gs_az, phi_out, az_out):
-        """(phi_out, az_out) = the gs_cfcf state (gs_phi, gs_az) plus the
-        network's correction, on the solver's grid and dtype."""
+    def predict_current(self, solver, B, jp):
+        """jp (the gs_cfcf state's J_par, solver dtype, 3 x N) += the network's
+        correction; B is that state's field."""
         grid = (solver.Nz, solver.Ny, solver.Nx)
         b = self._buffers(grid, False)
         code = L.NFGS_F64 if solver.dtype == torch.float64 else L.NFGS_F32
         sb = _bz_scale(solver)
-        self.features(b, code, gs_phi, gs_az, solver.Bx_obs, solver.By_obs, solver.Bz_obs,
-                      solver.Jz_obs, solver.bottom_divergence_robin_weight, sb)
+        self.features(b, code, B, jp, solver.bottom_divergence_robin_weight, sb)
         out = self.forward(b)
-        nz, ny, nx = grid
-        L.check(self._lib.nfgs_fno_apply(code, L.ptr(gs_phi), L.ptr(gs_az), L.ptr(out), nx, ny, nz,
-                                         sb * self.norm.out_phi, sb * self.norm.out_az,
-                                         L.ptr(phi_out), L.ptr(az_out), self._stream),
+        L.check(self._lib.nfgs_fno_apply(code, L.ptr(jp), L.ptr(out), OUTPUTS * b.N,
+                                         sb * self.norm.out, L.ptr(jp), self._stream),
                 "learned start")
 
 
@@ -359,12 +356,11 @@ class SpectralWarmStart:
 # ---------------------------------------------------------------------------
 @dataclass
 class Pair:
-    """One (boundary, rho) -> converged-state pair on the device (FP32)."""
-    gs_phi: torch.Tensor
-    gs_az: torch.Tensor
-    conv_phi: torch.Tensor
-    conv_az: torch.Tensor
-    planes: tuple  # Bx, By, Bz, Jz on the calculation grid
+    """One (boundary, rho) -> converged-current pair on the device (FP32):
+    the gs_cfcf state's B and J_par and the selected state's J_par."""
+    b_gs: torch.Tensor
+    jp_gs: torch.Tensor
+    jp_conv: torch.Tensor
     rho: float
     sb: float
     iterations: int
@@ -374,42 +370,43 @@ class Pair:
 def solve_pair(solver, rho, iterate_kw) -> Pair:
     """Plain solve of a solver whose boundary is set; returns its pair."""
     t0 = time.perf_counter()
-    _, _, phi0, az0, _ = solver.compute_initial_conditions()
+    B, jp = solver.gs_cfcf_start()
+    f = lambda t: t.to(torch.float32).contiguous().clone()
+    b_gs, jp_gs = f(B), f(jp)
     h = solver.iterate(**iterate_kw)
+    jp_c = f(solver.parallel_current())
     torch.cuda.synchronize(solver.device)
-    f = lambda t: t.to(torch.float32).contiguous()
-    return Pair(f(phi0), f(az0), f(solver._Phi_c), f(solver._Az),
-                tuple(f(p) for p in (solver.Bx_obs, solver.By_obs, solver.Bz_obs, solver.Jz_obs)),
-                float(rho), _bz_scale(solver), len(h["sigma"]), time.perf_counter() - t0)
+    return Pair(b_gs, jp_gs, jp_c, float(rho), _bz_scale(solver), len(h["sigma"]),
+                time.perf_counter() - t0)
 
 
 def fit_normalisation(pairs) -> Normalisation:
     """Channel scales from the training pairs (RMS over the set, in units of
     each pair's s_B)."""
-    acc = np.zeros(5)
+    acc = np.zeros(3)
     for p in pairs:
-        d_phi = (p.conv_phi - p.gs_phi).double()
-        d_az = (p.conv_az - p.gs_az).double()
-        vals = [p.gs_phi.double(), p.gs_az.double(), p.planes[3].double(), d_phi, d_az]
+        vals = [p.b_gs.double(), p.jp_gs.double(), (p.jp_conv - p.jp_gs).double()]
         acc += np.array([float(torch.mean(v * v).item()) / p.sb ** 2 for v in vals])
     rms = np.sqrt(acc / max(len(pairs), 1))
     return Normalisation(*[float(max(r, 1e-12)) for r in rms])
 
 
 def pair_target(net: SpectralWarmStart, p: Pair, out: torch.Tensor):
-    nz, ny, nx = p.gs_phi.shape
-    n = net.norm
-    L.check(net._lib.nfgs_fno_targets(L.NFGS_F32, L.ptr(p.gs_phi), L.ptr(p.gs_az),
-                                      L.ptr(p.conv_phi), L.ptr(p.conv_az), nx, ny, nz,
-                                      1.0 / (p.sb * n.out_phi), 1.0 / (p.sb * n.out_az),
-                                      L.ptr(out), net._stream), "targets")
+    L.check(net._lib.nfgs_fno_targets(L.NFGS_F32, L.ptr(p.jp_gs), L.ptr(p.jp_conv),
+                                      p.jp_gs.numel(), 1.0 / (p.sb * net.norm.out), L.ptr(out),
+                                      net._stream), "targets")
+
+
+def _pair_forward(net, b, p):
+    net.features(b, L.NFGS_F32, p.b_gs, p.jp_gs, p.rho, p.sb)
+    net.forward(b)
 
 
 def train(net: SpectralWarmStart, pairs, epochs: int, batch: int = 4, lr: float = 1e-3,
           lr_final: float = 1e-5, seed: int = 0, time_limit: Optional[float] = None, log=None):
     """Adam over the pairs with a cosine learning-rate schedule.  Returns the
     mean loss per epoch (read once per epoch)."""
-    grid = tuple(pairs[0].gs_phi.shape)
+    grid = tuple(pairs[0].b_gs.shape[1:])
     b = net._buffers(grid, True)
     target = torch.empty(OUTPUTS, b.N, device=net.device, dtype=torch.float32)
     rng = np.random.default_rng(seed)
@@ -424,8 +421,7 @@ def train(net: SpectralWarmStart, pairs, epochs: int, batch: int = 4, lr: float 
             idx = order[k0:k0 + batch]
             for i in idx:
                 p = pairs[int(i)]
-                net.features(b, L.NFGS_F32, p.gs_phi, p.gs_az, *p.planes, p.rho, p.sb)
-                net.forward(b)
+                _pair_forward(net, b, p)
                 pair_target(net, p, target)
                 losses += net.loss_and_backward(b, target)
             lr_t = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * step / steps_total))
@@ -442,17 +438,158 @@ def train(net: SpectralWarmStart, pairs, epochs: int, batch: int = 4, lr: float 
 
 def evaluate_loss(net: SpectralWarmStart, pairs) -> float:
     """Mean loss of the network over pairs (no parameter change)."""
-    grid = tuple(pairs[0].gs_phi.shape)
+    grid = tuple(pairs[0].b_gs.shape[1:])
     b = net._buffers(grid, True)
     target = torch.empty(OUTPUTS, b.N, device=net.device, dtype=torch.float32)
     tot = torch.zeros(1, device=net.device, dtype=torch.float64)
-    lib = net._lib
     for p in pairs:
-        net.features(b, L.NFGS_F32, p.gs_phi, p.gs_az, *p.planes, p.rho, p.sb)
-        net.forward(b)
+        _pair_forward(net, b, p)
         pair_target(net, p, target)
-        nz, ny, nx = grid
-        L.check(lib.nfgs_fno_loss(L.ptr(b.out), L.ptr(target), nx, ny, nz, L.ptr(b.gout),
-                                  L.ptr(b.loss), L.ptr(b.loss_ws), net._stream), "loss")
+        L.check(net._lib.nfgs_fno_loss(L.ptr(b.out), L.ptr(target), OUTPUTS * b.N, L.ptr(b.gout),
+                                       L.ptr(b.loss), L.ptr(b.loss_ws), net._stream), "loss")
         tot += b.loss
     return float(tot.item()) / len(pairs)
+
+
+# ---------------------------------------------------------------------------
+# Solver-generated pairs of the multipole family and the plain-vs-learned
+# comparison
+# ---------------------------------------------------------------------------
+# The production profile of the reference's controlled runs
+# (scripts/run_controlled_native.py:25-31) and its paper stopping protocol.
+SOLVER_KW = dict(lateral_wall="paper_bc1", pin_potential_boundaries=False, buffer_thickness=0,
+                 padded_buffer_zone=True, pad_calculation_domain=True,
+                 calculation_domain_mode="mudpack", mean_bz_policy="uniform_open",
+                 horizontal_zero_mode_policy="frozen_linear_harmonic",
+                 d0_zero_mode_policy="project", source_zero_mode_policy="project",
+                 harmonic_residual_policy="diagnose", poloidal_update_mode="direct",
+                 vertical_current_operator="staggered_face")
+ITERATE_KW = dict(max_iter=500, tol=1e-5, mu=1, patience=40, sigma_improvement_tol=1e-5,
+                  stop_mode="sigma_best", verbose=False)
+
+
+def case(n: int, seed: int, rho: Optional[float] = None):
+    """Boundary planes (device FP64) and rho of one case of the family: the
+    seeded multipole boundary on n x n observed columns, rho ~ U[0, 1] from
+    the same seed unless given."""
+    from .boundary import multipole_boundary
+    bx, by, bz, _ = multipole_boundary(n, seed=seed)
+    if rho is None:
+        rho = float(np.random.default_rng(10_000 + seed).uniform(0.0, 1.0))
+    return tuple(torch.as_tensor(p, device="cuda") for p in (bx, by, bz)), float(rho)
+
+
+def make_solver(n: int, planes, rho: float, nz: Optional[int] = None,
+                dtype=torch.float32):
+    from .solver import NFGSSolverGPU
+    s = NFGSSolverGPU(n, n, n if nz is None else nz, 1.0, 1.0, 1.0, device="cuda", dtype=dtype,
+                      bottom_divergence_robin_weight=rho, **SOLVER_KW)
+    s.set_bottom_vector_boundary(*(p.to(dtype) for p in planes), zero_net_current=False,
+                                 smooth_passes=0, ijzbcor=3)
+    return s
+
+
+def generate_pairs(count: int, n: int = 128, seed0: int = 0, nz: Optional[int] = None,
+                   log=None):
+    """`count` solver-generated pairs of the multipole family (seeds seed0,
+    seed0 + 1, ...), each a plain FP32 solve to the sigma_best stop."""
+    pairs = []
+    t0 = time.perf_counter()
+    for i in range(count):
+        planes, rho = case(n, seed0 + i)
+        s = make_solver(n, planes, rho, nz)
+        pairs.append(solve_pair(s, rho, ITERATE_KW))
+        del s
+        if log and (i + 1) % max(1, count // 10) == 0:
+            log(f"pairs {i + 1}/{count} ({time.perf_counter() - t0:.1f} s)")
+    return pairs
+
+
+def compare_starts(net: SpectralWarmStart, n: int, seeds, nz: Optional[int] = None):
+    """Plain gs_cfcf solve and learned-start solve of each held-out case:
+    iterations, wall time (set-up, start and iteration, synchronised), best
+    sigma_J and the relative L2 distance of the selected B between them."""
+    rows = []
+    for seed in seeds:
+        planes, rho = case(n, seed)
+        res = {}
+        for mode in ("gs_cfcf", "learned"):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s = make_solver(n, planes, rho, nz)
+            kw = dict(ITERATE_KW, init_mode=mode)
+            if mode == "learned":
+                kw["learned_start"] = net
+            h = s.iterate(**kw)
+            torch.cuda.synchronize()
+            res[mode] = (s, h, time.perf_counter() - t0)
+        (sp, hp, tp), (sl, hl, tl) = res["gs_cfcf"], res["learned"]
+        db = float(torch.linalg.norm((sl._B - sp._B).double()) /
+                   torch.linalg.norm(sp._B.double()))
+        rows.append(dict(seed=int(seed), rho=rho, plain_iterations=len(hp["sigma"]),
+                         learned_iterations=len(hl["sigma"]), plain_seconds=tp,
+                         learned_seconds=tl, plain_best_sigma=hp["best_sigma"],
+                         learned_best_sigma=hl["best_sigma"],
+                         learned_used=hl["init_diagnostics"]["learned_start_used"],
+                         sigma_gs_cfcf=hl["init_diagnostics"]["sigma_gs_cfcf"],
+                         sigma_learned=hl["init_diagnostics"]["sigma_learned"],
+                         B_relative_to_plain=db))
+        del res
+    return rows
+
+
+def summarize(rows) -> dict:
+    it_p = np.array([r["plain_iterations"] for r in rows], float)
+    it_l = np.array([r["learned_iterations"] for r in rows], float)
+    t_p = np.array([r["plain_seconds"] for r in rows])
+    t_l = np.array([r["learned_seconds"] for r in rows])
+    sg = np.array([r["learned_best_sigma"] / r["plain_best_sigma"] - 1.0 for r in rows])
+    return dict(cases=len(rows), used=int(sum(r["learned_used"] for r in rows)),
+                plain_iterations_mean=float(it_p.mean()),
+                learned_iterations_mean=float(it_l.mean()),
+                wall_speedup_total=float(t_p.sum() / t_l.sum()),
+                wall_speedup_median=float(np.median(t_p / t_l)),
+                best_sigma_change_median=float(np.median(sg)),
+                best_sigma_change_max=float(sg.max()),
+                B_relative_to_plain_median=float(np.median([r["B_relative_to_plain"]
+                                                            for r in rows])))
+
+
+def main(argv=None):
+    """python -m nfgs_amd.learned --pairs P --n N --epochs E [--out model.npz]:
+    generate P solver pairs, train, and compare plain and learned starts on
+    held-out seeds (one JSON line)."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--pairs", type=int, default=200)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--holdout", type=int, default=16)
+    ap.add_argument("--seed0", type=int, default=0)
+    ap.add_argument("--time-limit", type=float, default=None)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)
+    t0 = time.perf_counter()
+    pairs = generate_pairs(a.pairs, a.n, a.seed0, log=log)
+    t_gen = time.perf_counter() - t0
+    net = SpectralWarmStart(seed=a.seed0)
+    net.norm = fit_normalisation(pairs)
+    t1 = time.perf_counter()
+    hist = train(net, pairs, a.epochs, batch=a.batch, lr=a.lr, time_limit=a.time_limit, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    if a.out:
+        net.save(a.out)
+    del pairs
+    rows = compare_starts(net, a.n, range(a.seed0 + 1_000_000, a.seed0 + 1_000_000 + a.holdout))
+    print(json.dumps(dict(pairs=a.pairs, n=a.n, epochs=len(hist), loss=hist,
+                          generate_seconds=t_gen, train_seconds=t_train,
+                          summary=summarize(rows), cases=rows)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

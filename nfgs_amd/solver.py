@@ -1121,40 +1121,57 @@ class NFGSSolverGPU:
 
     def _init_learned(self, net):
         """Slot 0 from the learned start or gs_cfcf, whichever has the lower
-        sigma_J: the gs_cfcf state goes to slot 0, the network's correction of
-        it to slot 1, each is evaluated by the state pass, and the loser is
-        discarded.  Leaves J_par and the monitors of the kept state."""
-        if net is None or not hasattr(net, "predict_start"):
+        sigma_J.  The gs_cfcf state goes to slot 0; the network corrects its
+        parallel current and one iteration step from the corrected current
+        (the same Poisson solves) fills slot 1; the state pass evaluates it
+        and the loser is discarded.  Leaves J_par and the monitors of the
+        kept state."""
+        if net is None or not hasattr(net, "predict_current"):
             raise ValueError("init_mode='learned' needs learned_start= a "
                              "nfgs_amd.learned.SpectralWarmStart")
         lib, plan, s = self._lib, self._plan, self._stream
-        self._advance(0, zero_source=True)
+        B, jp = self.gs_cfcf_start()
         sigma_gs, _, _, ratio = self._read_metrics()
         self._check_zero_mode(ratio)
         t0 = time.perf_counter()
-        (phi0, az0), (phi1, az1) = self._slots[0], self._slots[1]
-        net.predict_start(self, phi0, az0, phi1, az1)
-
-        def evaluate(phi, az):
-            L.check(lib.nfgs_state_pass(plan, L.C.byref(self._bnd), L.ptr(phi), L.ptr(az), None,
-                                        L.ptr(self._ws_t["jp"]), None, L.ptr(self._partials), s),
-                    "state pass")
-            L.check(lib.nfgs_reduce(plan, L.ptr(self._partials), L.ptr(self._metrics), s), "reduce")
-            return self._read_metrics()[0]
-
-        sigma_l = evaluate(phi1, az1)
+        net.predict_current(self, B, jp)
+        self._advance(1, zero_source=False)
+        sigma_l = self._read_metrics()[0]
         seconds = time.perf_counter() - t0
         use = math.isfinite(sigma_l) and sigma_l < sigma_gs
         if use:
             self._slots[0], self._slots[1] = self._slots[1], self._slots[0]
         else:
-            evaluate(phi0, az0)
+            phi0, az0 = self._slots[0]
+            L.check(lib.nfgs_state_pass(plan, L.C.byref(self._bnd), L.ptr(phi0), L.ptr(az0), None,
+                                        L.ptr(self._ws_t["jp"]), None, L.ptr(self._partials), s),
+                    "state pass")
+            L.check(lib.nfgs_reduce(plan, L.ptr(self._partials), L.ptr(self._metrics), s), "reduce")
         self._init_diagnostics = {
             "init_mode": "learned", "learned_start_used": bool(use),
             "sigma_gs_cfcf": float(sigma_gs), "sigma_learned": float(sigma_l),
             "learned_start_seconds": float(seconds),
             "gaussian_separation": self.gaussian_separation_diagnostics,
             "coupled_k0": self._k0_diagnostics}
+
+    def gs_cfcf_start(self):
+        """The gs_cfcf state into slot 0; returns views of its B and its
+        parallel current J_par (3 x N each; valid until the next step).  An
+        extension: the inputs of the learned start."""
+        self._require_boundary()
+        self._advance(0, zero_source=True, want_b=True)
+        return self._Bwork, self._ws_t["jp"]
+
+    def parallel_current(self):
+        """J_par (3, Nz, Ny, Nx) of the selected state: the current the next
+        iteration step would take (an extension; the learned start's target)."""
+        if self._Phi_c is None:
+            raise RuntimeError("No selected state yet. Call iterate() first.")
+        jp = self._empty(3, self.Nz, self.Ny, self.Nx)
+        L.check(self._lib.nfgs_state_pass(self._plan, L.C.byref(self._bnd), L.ptr(self._Phi_c),
+                                          L.ptr(self._Az), None, L.ptr(jp), None, None,
+                                          self._stream), "state pass")
+        return jp
 
     def warm_start_state(self):
         """(B, Az, J) of the selected state on the calculation grid, for

@@ -1,9 +1,10 @@
 """CPU oracle of the learned warm start's network — TEST INFRASTRUCTURE ONLY.
 
 A NumPy FP64 statement of the cosine-basis spectral neural operator that
-nfgs_amd.learned runs on the GPU (nfgs_amd/csrc/fno.hip): its forward pass,
-the analytic gradients of every weight, the masked mean-squared-error loss and
-the Adam step.  The reference has no learned start (its iterate() starts only
+nfgs_amd.learned runs on the GPU (nfgs_amd/csrc/fno.hip; it maps the gs_cfcf
+state's B and parallel current and rho to a correction of that current): its
+forward pass, the analytic gradients of every weight, the mean-squared-error
+loss and the Adam step.  The reference has no learned start (its iterate() starts only
 from gs_cfcf, nfgs_solver.py:6613-6645), so there is no reference line to
 follow; the network's definition is this module's and the GPU's common
 contract.  Only tests/ and bench.py's CPU-baseline leg may import it.
@@ -20,8 +21,7 @@ Parity pins (tests/test_oracle_fno.py):
                          differences
   * gradients            central finite differences of the loss in FP64 for
                          every parameter group and the input features
-  * loss                 closed form on a target that differs on one node of
-                         each channel, nodes on the fixed planes ignored
+  * loss                 closed form on a target that differs on two nodes
   * Adam                 first step of a constant gradient moves every
                          weight by lr sign(g) (bias-corrected moments)
 No function is left "parity unpinned".
@@ -31,7 +31,7 @@ from __future__ import annotations
 import numpy as np
 from scipy.special import erf
 
-WIDTH, FEATURES, HIDDEN, OUTPUTS, LAYERS = 32, 8, 64, 2, 4
+WIDTH, FEATURES, HIDDEN, OUTPUTS, LAYERS = 32, 8, 64, 3, 4
 
 
 def dct2_basis(n: int, m: int) -> np.ndarray:
@@ -68,27 +68,13 @@ def param_shapes(m: int):
     return shapes
 
 
-def free_masks(nz, ny, nx):
-    """Nodes the start may change: Phi_c above the bottom plane, Az inside the
-    walls, the bottom and the top plane (the slots' fixed planes)."""
-    z = np.arange(nz)[:, None, None]
-    y = np.arange(ny)[None, :, None]
-    x = np.arange(nx)[None, None, :]
-    m0 = np.broadcast_to(z >= 1, (nz, ny, nx))
-    m1 = (z >= 1) & (z <= nz - 2) & (y >= 1) & (y <= ny - 2) & (x >= 1) & (x <= nx - 2)
-    return m0, np.broadcast_to(m1, (nz, ny, nx))
-
-
-def features(phi, az, bx, by, bz, jz, rho, c_phi, c_az, c_b, c_jz):
-    """The eight input channels (8, nz, ny, nx)."""
-    nz, ny, nx = phi.shape
-    f = np.empty((FEATURES, nz, ny, nx))
-    f[0] = phi * c_phi
-    f[1] = az * c_az
-    f[2] = bx[None] * c_b
-    f[3] = by[None] * c_b
-    f[4] = bz[None] * c_b
-    f[5] = jz[None] * c_jz
+def features(b, jp, rho, c_b, c_j):
+    """The eight input channels (8, nz, ny, nx): B c_b, J_par c_j, rho and
+    z / (nz - 1)."""
+    nz = b.shape[1]
+    f = np.empty((FEATURES,) + b.shape[1:])
+    f[0:3] = b * c_b
+    f[3:6] = jp * c_j
     f[6] = rho
     f[7] = (np.arange(nz) / max(nz - 1, 1))[:, None, None]
     return f
@@ -113,7 +99,7 @@ def _pointwise(w, b, v):
 
 
 def forward(params, feat, bases):
-    """Network output (2, nz, ny, nx) and the cache for backward."""
+    """Network output (3, nz, ny, nx) and the cache for backward."""
     bx_ = bases[2]
     cache = {"feat": feat, "v": [], "uh": [], "s": [], "z": []}
     v = _pointwise(params["lift.w"], params["lift.b"], feat)
@@ -135,13 +121,9 @@ def forward(params, feat, bases):
 
 
 def loss(out, target):
-    """Mean of (out - target)^2 over the free nodes of both channels, and its
-    gradient with respect to out."""
-    nz, ny, nx = out.shape[1:]
-    m0, m1 = free_masks(nz, ny, nx)
-    count = m0.sum() + m1.sum()
-    d = np.stack([np.where(m0, out[0] - target[0], 0.0), np.where(m1, out[1] - target[1], 0.0)])
-    return float((d * d).sum() / count), 2.0 * d / count
+    """Mean of (out - target)^2 over every value, and its gradient."""
+    d = out - target
+    return float((d * d).mean()), 2.0 * d / d.size
 
 
 def backward(params, cache, gout, bases):

@@ -89,9 +89,9 @@ def test_every_gradient_matches_the_oracle():
     net, b = _net_and_buffers(params)
     rng = np.random.default_rng(3)
     feat = rng.normal(size=(fo.FEATURES,) + GRID)
-    tgt = rng.normal(size=(2,) + GRID)
+    tgt = rng.normal(size=(fo.OUTPUTS,) + GRID)
     b.feat.copy_(torch.as_tensor(feat.reshape(fo.FEATURES, -1), dtype=torch.float32))
-    dt = torch.as_tensor(tgt.reshape(2, -1), dtype=torch.float32, device="cuda")
+    dt = torch.as_tensor(tgt.reshape(fo.OUTPUTS, -1), dtype=torch.float32, device="cuda")
     net.forward(b)
     loss = net.loss_and_backward(b, dt)
     torch.cuda.synchronize()
@@ -109,7 +109,7 @@ def test_gradients_accumulate_and_adam_matches_the_oracle():
     params = _params(4)
     net, b = _net_and_buffers(params)
     rng = np.random.default_rng(5)
-    tgt = torch.as_tensor(rng.normal(size=(2, b.N)), dtype=torch.float32, device="cuda")
+    tgt = torch.as_tensor(rng.normal(size=(fo.OUTPUTS, b.N)), dtype=torch.float32, device="cuda")
     b.feat.copy_(torch.as_tensor(rng.normal(size=(fo.FEATURES, b.N)), dtype=torch.float32))
     net.forward(b)
     net.loss_and_backward(b, tgt)
@@ -136,36 +136,29 @@ def test_gradients_accumulate_and_adam_matches_the_oracle():
         assert not zero[name].any()
 
 
-def test_targets_features_and_start_keep_the_fixed_planes():
+def test_targets_features_and_start_current():
     from nfgs_amd import _lib as L
     from nfgs_amd.learned import SpectralWarmStart
     lib = L.load()
     nz, ny, nx = GRID
+    n = 3 * nz * ny * nx
     rng = np.random.default_rng(6)
     f = lambda a: torch.as_tensor(a, dtype=torch.float32, device="cuda")
-    gphi, gaz, phi, az = (f(rng.normal(size=GRID)) for _ in range(4))
-    tgt = torch.empty(2, nz * ny * nx, dtype=torch.float32, device="cuda")
+    jp_gs, jp, B = (f(rng.normal(size=(3,) + GRID)) for _ in range(3))
+    tgt = torch.empty(n, dtype=torch.float32, device="cuda")
     s = torch.cuda.current_stream().cuda_stream
-    L.check(lib.nfgs_fno_targets(L.NFGS_F32, L.ptr(gphi), L.ptr(gaz), L.ptr(phi), L.ptr(az), nx,
-                                 ny, nz, 0.5, 0.25, L.ptr(tgt), s))
-    m0, m1 = fo.free_masks(nz, ny, nx)
-    t = tgt.cpu().numpy().reshape((2,) + GRID)
-    np.testing.assert_allclose(t[0], np.where(m0, (phi - gphi).cpu().numpy() * 0.5, 0), atol=1e-6)
-    np.testing.assert_allclose(t[1], np.where(m1, (az - gaz).cpu().numpy() * 0.25, 0), atol=1e-6)
-    net_out = f(rng.normal(size=(2, nz * ny * nx)))
-    po, ao = torch.empty_like(gphi), torch.empty_like(gaz)
-    L.check(lib.nfgs_fno_apply(L.NFGS_F32, L.ptr(gphi), L.ptr(gaz), L.ptr(net_out), nx, ny, nz,
-                               2.0, 3.0, L.ptr(po), L.ptr(ao), s))
-    o = net_out.cpu().numpy().reshape((2,) + GRID)
-    np.testing.assert_allclose(po.cpu().numpy(), np.where(m0, gphi.cpu().numpy() + 2 * o[0],
-                                                          gphi.cpu().numpy()), atol=1e-5)
-    np.testing.assert_allclose(ao.cpu().numpy(), np.where(m1, gaz.cpu().numpy() + 3 * o[1],
-                                                          gaz.cpu().numpy()), atol=1e-5)
-    planes = [f(rng.normal(size=(ny, nx))) for _ in range(4)]
+    L.check(lib.nfgs_fno_targets(L.NFGS_F32, L.ptr(jp_gs), L.ptr(jp), n, 0.5, L.ptr(tgt), s))
+    np.testing.assert_allclose(tgt.cpu().numpy(), ((jp - jp_gs) * 0.5).cpu().numpy().ravel(),
+                               atol=1e-6)
+    net_out = f(rng.normal(size=n))
+    start = jp_gs.clone()
+    L.check(lib.nfgs_fno_apply(L.NFGS_F32, L.ptr(start), L.ptr(net_out), n, 2.0, L.ptr(start), s))
+    np.testing.assert_allclose(start.cpu().numpy().ravel(),
+                               (jp_gs.cpu().numpy().ravel() + 2.0 * net_out.cpu().numpy()),
+                               atol=1e-5)
     net = SpectralWarmStart(device="cuda")
     b = net._buffers(GRID, False)
-    net.features(b, L.NFGS_F32, gphi, gaz, *planes, 0.3, 2.0)
+    net.features(b, L.NFGS_F32, B, jp_gs, 0.3, 2.0)
     torch.cuda.synchronize()
-    ref = fo.features(gphi.cpu().numpy(), gaz.cpu().numpy(), *(p.cpu().numpy() for p in planes),
-                      0.3, 0.5, 0.5, 0.5, 0.5)
+    ref = fo.features(B.cpu().numpy(), jp_gs.cpu().numpy(), 0.3, 0.5, 0.5)
     np.testing.assert_allclose(b.feat.cpu().numpy().reshape(ref.shape), ref, rtol=1e-6, atol=1e-6)

@@ -45,21 +45,16 @@ def test_gelu_is_x_times_the_normal_cdf_and_its_slope():
                                atol=1e-8)
 
 
-def test_loss_closed_form_and_fixed_planes_ignored():
-    nz, ny, nx = 5, 4, 6
-    out = np.zeros((2, nz, ny, nx))
+def test_loss_closed_form():
+    out = np.zeros((3, 5, 4, 6))
     tgt = np.zeros_like(out)
-    tgt[0, 2, 1, 1] = 3.0      # free Phi_c node
-    tgt[1, 2, 2, 3] = -2.0     # free Az node
-    tgt[0, 0, 1, 1] = 100.0    # bottom plane of Phi_c: fixed
-    tgt[1, 4, 1, 1] = 100.0    # top plane of Az: fixed
-    tgt[1, 2, 0, 1] = 100.0    # Az wall: fixed
-    count = (nz - 1) * ny * nx + (nz - 2) * (ny - 2) * (nx - 2)
+    tgt[0, 2, 1, 1] = 3.0
+    tgt[2, 0, 3, 5] = -2.0
     val, g = fo.loss(out, tgt)
-    assert val == pytest.approx((9.0 + 4.0) / count, rel=1e-14)
-    assert g[0, 2, 1, 1] == pytest.approx(-6.0 / count)
-    assert g[1, 2, 2, 3] == pytest.approx(4.0 / count)
-    assert g[0, 0, 1, 1] == 0 and g[1, 4, 1, 1] == 0 and g[1, 2, 0, 1] == 0
+    assert val == pytest.approx(13.0 / out.size, rel=1e-14)
+    assert g[0, 2, 1, 1] == pytest.approx(-6.0 / out.size)
+    assert g[2, 0, 3, 5] == pytest.approx(4.0 / out.size)
+    assert np.count_nonzero(g) == 2
 
 
 def test_adam_first_step_moves_by_lr_times_sign():
@@ -88,7 +83,7 @@ def _tiny(seed=3, m=2, shape=(4, 5, 6)):
         scale = 0.6 / np.sqrt(sh[-1]) if not name.endswith(".b") else 0.2
         params[name] = rng.normal(size=sh) * scale
     feat = rng.normal(size=(fo.FEATURES, nz, ny, nx))
-    tgt = rng.normal(size=(2, nz, ny, nx))
+    tgt = rng.normal(size=(fo.OUTPUTS, nz, ny, nx))
     bases = (fo.dct2_basis(nz, m), fo.dct2_basis(ny, m), fo.dct2_basis(nx, m))
     return params, feat, tgt, bases
 
