@@ -65,6 +65,12 @@ def parse():
                    help="skip the precision leg (512x512x256: FP32 exact and split-bf16 "
                         "against FP64)")
     p.add_argument("--precision-n", type=int, default=512)
+    p.add_argument("--no-learned", action="store_true", help="skip the learned-start leg")
+    p.add_argument("--learned-pairs", type=int, default=128,
+                   help="solver-generated training pairs of the learned-start leg")
+    p.add_argument("--learned-epochs", type=int, default=20)
+    p.add_argument("--learned-holdout", type=int, default=16)
+    p.add_argument("--learned-n", type=int, default=128, help="observed size of the learned leg")
     p.add_argument("--transforms-only", action="store_true",
                    help="time one DCT-I pass along x and y at 258x258x257, 386x386x129 and "
                         "514x514x257 for FP32 exact, FP32 split-bf16 and FP64 plans, and exit")
@@ -317,6 +323,33 @@ def batched_leg(torch, a):
                 note="batched: boundary setup shared across Robin weights, 8-step HIP graphs, "
                      "4 streams, autotuned z-chunks and transform blocks; tuning changes "
                      "only the monitors' summation order")
+
+
+def learned_leg(torch, a):
+    """Learned start at 128^3 (calculation grid 130x130x129): train the
+    spectral network on solver-generated pairs of the multipole family, then
+    solve held-out boundaries plain and from the learned start."""
+    from nfgs_amd import learned as Lm
+    t0 = time.perf_counter()
+    pairs = Lm.generate_pairs(a.learned_pairs, a.learned_n, seed0=0)
+    t_gen = time.perf_counter() - t0
+    net = Lm.SpectralWarmStart(seed=0)
+    net.norm = Lm.fit_normalisation(pairs)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    hist = Lm.train(net, pairs, a.learned_epochs, batch=4, lr=1e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    samples = len(hist) * len(pairs)
+    del pairs
+    rows = Lm.compare_starts(net, a.learned_n,
+                             range(1_000_000, 1_000_000 + a.learned_holdout))
+    out = dict(grid_observed=a.learned_n, pairs=a.learned_pairs, epochs=len(hist),
+               final_train_loss=round(hist[-1], 4), generate_s=round(t_gen, 2),
+               train_s=round(t_train, 2), train_ms_per_sample=round(t_train / samples * 1e3, 2),
+               parameters=net.n_params, **Lm.summarize(rows))
+    del net
+    return out
 
 
 def field_errors(f, ref):
@@ -715,6 +748,13 @@ def main():
         err["common_scores"] = cs
         log(f"  common scores FP32 vs FP64: {cs}")
 
+    learned = None
+    if not a.no_learned:
+        log(f"learned start: {a.learned_pairs} pairs at {a.learned_n}^3, {a.learned_epochs} "
+            f"epochs, {a.learned_holdout} held-out boundaries")
+        learned = learned_leg(torch, a)
+        log(f"  {learned}")
+
     precision = None
     if not a.no_precision:
         log(f"precision leg: {a.precision_n}^2 x {a.precision_n // 2}, FP64 / FP32 exact / "
@@ -766,6 +806,7 @@ def main():
         "stencil_roofline": stencil_roof,
         "kernels": kernels,
         "warm_start": warm,
+        "learned_start": learned,
         "batched": batched,
         "precision": precision,
         "two_stage": two,

@@ -179,12 +179,14 @@ __global__ __launch_bounds__(256) void pw_backward(
 
 // ---------------------------------------------------------------------------
 // Weight gradients of a pointwise layer: gw[o][i] = sum_p a[o][p] b[i][p],
-// gb[o] = sum_p a[o][p].  Each block takes chunks of 64 nodes, staged in LDS;
-// a thread owns an RB x 4 block of (o, i) pairs over a share of the chunk's
-// nodes, and the shares are summed in LDS at the end.  One FP32 partial per
-// block and pair; the partials are summed in FP64 in a fixed order.
+// gb[o] = sum_p a[o][p].  Each block takes chunks of 64 nodes, staged in LDS
+// (the next chunk's loads are in flight in registers while the current one
+// is used); a thread owns an RB x 4 block of (o, i) pairs over a share of
+// the chunk's nodes, read four nodes at a time, and the shares are summed in
+// LDS at the end.  One FP32 partial per block and pair, stored pair-major;
+// the partials are summed in FP64 in a fixed order, one wavefront per pair.
 // ---------------------------------------------------------------------------
-constexpr int kOsBlocks = 512;
+constexpr int kOsBlocks = 2048;
 constexpr int kOsQ = 64;
 
 template <int CA, int CB>
@@ -194,7 +196,8 @@ struct OuterShape {
   static constexpr int G = 256 / NPB;                // node groups
   static constexpr int QG = kOsQ / G;                // nodes per group and chunk
   static constexpr int COUNT = CA * CB + CA;         // partials per block
-  static_assert(CA % RB == 0 && CB % 4 == 0 && 256 % NPB == 0 && kOsQ % G == 0, "shape");
+  static_assert(CA % RB == 0 && CB % 4 == 0 && 256 % NPB == 0 && kOsQ % G == 0 && QG % 4 == 0,
+                "shape");
 };
 
 template <int CA, int CB>
@@ -203,8 +206,10 @@ __global__ __launch_bounds__(256) void outer_sum_kernel(const float* __restrict_
                                                         float* __restrict__ part) {
   using S = OuterShape<CA, CB>;
   constexpr int RB = S::RB, G = S::G, QG = S::QG;
-  __shared__ float la[CA][kOsQ + 1];
-  __shared__ float lb[CB][kOsQ + 1];
+  constexpr int LD = kOsQ + 4;  // 16-byte rows
+  constexpr int NA = (CA * kOsQ + 255) / 256, NBL = (CB * kOsQ + 255) / 256;
+  __shared__ __align__(16) float la[CA][LD];
+  __shared__ __align__(16) float lb[CB][LD];
   __shared__ float red[256][RB * 4 + 1];
   const int t = threadIdx.x;
   const int pb = t % S::NPB, grp = t / S::NPB;
@@ -217,31 +222,57 @@ __global__ __launch_bounds__(256) void outer_sum_kernel(const float* __restrict_
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[r][c] = 0.0f;
   }
-  for (long base = (long)blockIdx.x * kOsQ; base < N; base += (long)gridDim.x * kOsQ) {
-    for (int k = t; k < CA * kOsQ; k += 256) {
-      const int c = k / kOsQ, q = k % kOsQ;
-      const long p = base + q;
-      la[c][q] = p < N ? a[(long)c * N + p] : 0.0f;
+  float ra[NA], rb[NBL];
+  auto fetch = [&](long base) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int k = t + 256 * u;
+      const long p = base + (k % kOsQ);
+      ra[u] = (k < CA * kOsQ && p < N) ? a[(long)(k / kOsQ) * N + p] : 0.0f;
     }
-    for (int k = t; k < CB * kOsQ; k += 256) {
-      const int c = k / kOsQ, q = k % kOsQ;
-      const long p = base + q;
-      lb[c][q] = p < N ? b[(long)c * N + p] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < NBL; ++u) {
+      const int k = t + 256 * u;
+      const long p = base + (k % kOsQ);
+      rb[u] = (k < CB * kOsQ && p < N) ? b[(long)(k / kOsQ) * N + p] : 0.0f;
+    }
+  };
+  const long stride = (long)gridDim.x * kOsQ;
+  long base = (long)blockIdx.x * kOsQ;
+  if (base < N) fetch(base);
+  for (; base < N; base += stride) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int k = t + 256 * u;
+      if (k < CA * kOsQ) la[k / kOsQ][k % kOsQ] = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < NBL; ++u) {
+      const int k = t + 256 * u;
+      if (k < CB * kOsQ) lb[k / kOsQ][k % kOsQ] = rb[u];
     }
     __syncthreads();
-#pragma unroll 4
-    for (int qq = 0; qq < QG; ++qq) {
+    if (base + stride < N) fetch(base + stride);
+#pragma unroll
+    for (int qq = 0; qq < QG; qq += 4) {
       const int q = grp * QG + qq;
-      float av[RB], bv[4];
+      float4 av[RB], bv[4];
 #pragma unroll
-      for (int r = 0; r < RB; ++r) av[r] = la[o0 + r][q];
+      for (int r = 0; r < RB; ++r) av[r] = *reinterpret_cast<const float4*>(&la[o0 + r][q]);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) bv[c] = lb[i0 + c][q];
+      for (int c = 0; c < 4; ++c) bv[c] = *reinterpret_cast<const float4*>(&lb[i0 + c][q]);
 #pragma unroll
       for (int r = 0; r < RB; ++r) {
-        accb[r] += av[r];
+        accb[r] += (av[r].x + av[r].y) + (av[r].z + av[r].w);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = fmaf(av[r], bv[c], acc[r][c]);
+        for (int c = 0; c < 4; ++c) {
+          float v = acc[r][c];
+          v = fmaf(av[r].x, bv[c].x, v);
+          v = fmaf(av[r].y, bv[c].y, v);
+          v = fmaf(av[r].z, bv[c].z, v);
+          v = fmaf(av[r].w, bv[c].w, v);
+          acc[r][c] = v;
+        }
       }
     }
     __syncthreads();
@@ -252,13 +283,13 @@ __global__ __launch_bounds__(256) void outer_sum_kernel(const float* __restrict_
 #pragma unroll
     for (int c = 0; c < 4; ++c) red[t][r * 4 + c] = acc[r][c];
   __syncthreads();
-  float* out = part + (long)blockIdx.x * S::COUNT;
+  const long nbk = gridDim.x, blk = blockIdx.x;
   for (int k = t; k < S::NPB * RB * 4; k += 256) {
     const int pbk = k / (RB * 4), e = k % (RB * 4);
-    float s = 0.0f;
-    for (int g = 0; g < G; ++g) s += red[g * S::NPB + pbk][e];
+    float sum = 0.0f;
+    for (int g = 0; g < G; ++g) sum += red[g * S::NPB + pbk][e];
     const int o = (pbk / (CB / 4)) * RB + e / 4, i = (pbk % (CB / 4)) * 4 + e % 4;
-    out[o * CB + i] = s;
+    part[(long)(o * CB + i) * nbk + blk] = sum;
   }
   __syncthreads();
   // bias sums: the pair blocks of column 0 hold every o
@@ -268,23 +299,31 @@ __global__ __launch_bounds__(256) void outer_sum_kernel(const float* __restrict_
   if (t < S::NPB && (t % (CB / 4)) == 0) {
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
-      float s = 0.0f;
-      for (int g = 0; g < G; ++g) s += red[g * S::NPB + t][r];
-      out[CA * CB + (t / (CB / 4)) * RB + r] = s;
+      float sum = 0.0f;
+      for (int g = 0; g < G; ++g) sum += red[g * S::NPB + t][r];
+      part[(long)(CA * CB + (t / (CB / 4)) * RB + r) * nbk + blk] = sum;
     }
   }
 }
 
+// One wavefront per output: lanes take every 64th block partial, then a
+// fixed butterfly over the wavefront.
 __global__ __launch_bounds__(256) void sum_partials_kernel(const float* __restrict__ part,
                                                            int nblocks, int count, int nw,
                                                            float* __restrict__ gw,
                                                            float* __restrict__ gb, int accumulate) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (j >= count) return;
-  double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += (double)part[(long)b * count + j];
-  float* dst = j < nw ? gw + j : gb + (j - nw);
-  *dst = accumulate ? *dst + (float)s : (float)s;
+  const float* src = part + (long)j * nblocks;
+  double sum = 0.0;
+  for (int k = lane; k < nblocks; k += 64) sum += (double)src[k];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+  if (lane == 0) {
+    float* dst = j < nw ? gw + j : gb + (j - nw);
+    *dst = accumulate ? *dst + (float)sum : (float)sum;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -292,8 +331,7 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(const float* __restri
 //   project: out[o][k][i] = sum_j B[j][k] in[o][j][i]   (k < 16)
 //   expand:  out[o][j][i] = sum_k B[j][k] in[o][k][i]   (j < n)
 // B is the n x 16 orthonormal basis.  inner > 1: one thread per (o, i), the
-// basis read as uniform (scalar) loads.  inner == 1 (the x axis): rows
-// staged in LDS, 16 to 64 per block by length.
+// basis read as uniform (scalar) loads.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void axis_project_inner(const float* __restrict__ in,
                                                           float* __restrict__ out,
@@ -336,36 +374,48 @@ __global__ __launch_bounds__(256) void axis_expand_inner(const float* __restrict
   }
 }
 
+// x axis: R rows per block copied linearly into LDS; each wavefront forms a
+// 16-row x 16-mode tile on v_mfma_f32_16x16x4f32 (A[i][k] from lane i + 16k,
+// B[k][j] from lane j + 16k, output row 4(l/16) + r of lane l in register r).
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4v mma(float a, float b, f32x4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
 template <int R>
-__global__ __launch_bounds__(256) void axis_project_rows(const float* __restrict__ in,
-                                                         float* __restrict__ out,
-                                                         const float* __restrict__ B, long rows,
-                                                         int n) {
-  extern __shared__ float lrow[];  // R rows of n, padded to n + 1
-  constexpr int TPR = 256 / R, KPT = kM / TPR;
-  static_assert(KPT == 4 || KPT == 2 || KPT == 1, "threads per row");
-  const int t = threadIdx.x;
+__global__ __launch_bounds__(R * 4) void axis_project_rows(const float* __restrict__ in,
+                                                           float* __restrict__ out,
+                                                           const float* __restrict__ B, long rows,
+                                                           int n) {
+  extern __shared__ __align__(16) float lrow[];  // R rows of n
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const long r0 = (long)blockIdx.x * R;
   const int nr = (int)((rows - r0) < R ? (rows - r0) : R);
+  const long cnt = (long)nr * n;
   const float* src = in + r0 * n;
-  for (int k = t; k < nr * n; k += 256) {
-    const int r = k / n, j = k - r * n;
-    lrow[r * (n + 1) + j] = src[k];
-  }
+  // r0 * n * 4 bytes is a multiple of 16 (R is a multiple of 16)
+  const long c4 = cnt >> 2;
+  for (long k = t; k < c4; k += R * 4)
+    reinterpret_cast<float4*>(lrow)[k] = reinterpret_cast<const float4*>(src)[k];
+  for (long k = (c4 << 2) + t; k < cnt; k += R * 4) lrow[k] = src[k];
   __syncthreads();
-  const int r = t / TPR, k0 = (t % TPR) * KPT;
-  if (r >= nr) return;
-  float acc[KPT];
-#pragma unroll
-  for (int q = 0; q < KPT; ++q) acc[q] = 0.0f;
-  const float* row = lrow + r * (n + 1);
-  for (int j = 0; j < n; ++j) {
-    const float x = row[j];
-#pragma unroll
-    for (int q = 0; q < KPT; ++q) acc[q] = fmaf(B[j * kM + k0 + q], x, acc[q]);
+  const int i = lane & 15, kk = lane >> 4;
+  const int row = w * 16 + i;
+  const bool rowok = row < nr;
+  const float* lr = lrow + row * n;
+  f32x4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int j0 = 0; j0 < n; j0 += 4) {
+    const int j = j0 + kk;
+    const float a = (rowok && j < n) ? lr[j] : 0.0f;
+    const float b = j < n ? B[j * kM + i] : 0.0f;
+    acc = mma(a, b, acc);
   }
 #pragma unroll
-  for (int q = 0; q < KPT; ++q) out[(r0 + r) * kM + k0 + q] = acc[q];
+  for (int r = 0; r < 4; ++r) {
+    const int orow = w * 16 + 4 * kk + r;
+    if (orow < nr) out[(r0 + orow) * kM + i] = acc[r];
+  }
 }
 
 __global__ __launch_bounds__(256) void axis_expand_rows(const float* __restrict__ in,
@@ -609,7 +659,7 @@ int nfgs_fno_outer_sum(const float* a, int ca, const float* b, int cb, long npoi
 #undef OS
   NFGS_LAUNCH_CHECK();
   const int count = ca * cb + ca;
-  hipLaunchKernelGGL(sum_partials_kernel, dim3((count + 255) / 256), dim3(256), 0, s, part,
+  hipLaunchKernelGGL(sum_partials_kernel, dim3((count + 3) / 4), dim3(256), 0, s, part,
                      (int)blocks, count, ca * cb, grad_w, grad_b, accumulate);
   NFGS_LAUNCH_CHECK();
   return NFGS_OK;
@@ -624,17 +674,17 @@ int nfgs_fno_axis(const float* in, float* out, const float* basis, long outer, i
                "fno axis: direction must be NFGS_FNO_PROJECT or NFGS_FNO_EXPAND");
   hipStream_t s = FNO_STREAM(stream);
   if (inner == 1 && direction == NFGS_FNO_PROJECT) {
-    // rows per block so that R (n + 1) floats fit in 64 KiB of LDS
-    NFGS_REQUIRE(n <= 1022, NFGS_ERR_ARG, "fno axis: rows longer than 1022 nodes");
-    const int R = n <= 254 ? 64 : (n <= 510 ? 32 : 16);
-    const size_t lds = (size_t)R * (n + 1) * sizeof(float);
+    // rows per block so that R n floats fit in 64 KiB of LDS
+    NFGS_REQUIRE(n <= 1024, NFGS_ERR_ARG, "fno axis: rows longer than 1024 nodes");
+    const int R = n <= 256 ? 64 : (n <= 512 ? 32 : 16);
+    const size_t lds = (size_t)R * n * sizeof(float);
     const dim3 g((unsigned)((outer + R - 1) / R));
     if (R == 64)
       hipLaunchKernelGGL(axis_project_rows<64>, g, dim3(256), lds, s, in, out, basis, outer, n);
     else if (R == 32)
-      hipLaunchKernelGGL(axis_project_rows<32>, g, dim3(256), lds, s, in, out, basis, outer, n);
+      hipLaunchKernelGGL(axis_project_rows<32>, g, dim3(128), lds, s, in, out, basis, outer, n);
     else
-      hipLaunchKernelGGL(axis_project_rows<16>, g, dim3(256), lds, s, in, out, basis, outer, n);
+      hipLaunchKernelGGL(axis_project_rows<16>, g, dim3(64), lds, s, in, out, basis, outer, n);
   } else if (inner == 1) {
     hipLaunchKernelGGL(axis_expand_rows, dim3((unsigned)((outer * n + 255) / 256)), dim3(256), 0,
                        s, in, out, basis, outer, n);

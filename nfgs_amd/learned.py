@@ -470,10 +470,12 @@ ITERATE_KW = dict(max_iter=500, tol=1e-5, mu=1, patience=40, sigma_improvement_t
 
 def case(n: int, seed: int, rho: Optional[float] = None):
     """Boundary planes (device FP64) and rho of one case of the family: the
-    seeded multipole boundary on n x n observed columns, rho ~ U[0, 1] from
-    the same seed unless given."""
+    seeded multipole boundary on n x n observed columns (pole widths 8 to 20
+    cells from n = 128 up, scaled by n / 128 below), rho ~ U[0, 1] from the
+    same seed unless given."""
     from .boundary import multipole_boundary
-    bx, by, bz, _ = multipole_boundary(n, seed=seed)
+    f = min(1.0, n / 128.0)
+    bx, by, bz, _ = multipole_boundary(n, seed=seed, sigma=(8.0 * f, 20.0 * f))
     if rho is None:
         rho = float(np.random.default_rng(10_000 + seed).uniform(0.0, 1.0))
     return tuple(torch.as_tensor(p, device="cuda") for p in (bx, by, bz)), float(rho)
