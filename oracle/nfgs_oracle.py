@@ -519,10 +519,26 @@ class OracleNFGS:
                  calculation_domain_mode="mudpack", weak_field_floor_b2=1e-20,
                  alpha_limit_factor=1.0, source_zero_mode_policy="error",
                  source_zero_mode_tolerance=1e-2, d0_zero_mode_policy="error",
-                 mean_mode_tolerance=1e-6, poloidal_update_mode="direct"):
+                 mean_mode_tolerance=1e-6, poloidal_update_mode="direct",
+                 horizontal_zero_mode_policy="frozen_linear_harmonic", k0_ode_smoothness=1.0e-3,
+                 coupled_k0_discretization="staggered_face", k0_update_mu=None,
+                 k0_spline_basis_size=8):
         if poloidal_update_mode not in ("direct", "two_stage"):
             raise ValueError("poloidal_update_mode must be 'direct' or 'two_stage'")
+        if horizontal_zero_mode_policy not in ("frozen_linear_harmonic", "coupled_k0",
+                                               "constrained_spline_harmonic"):
+            raise ValueError("unsupported horizontal_zero_mode_policy")
         self.poloidal_update_mode = poloidal_update_mode
+        # coupled horizontal-harmonic closures (nfgs_solver.py:3963-3972,
+        # :4120-4170, :4283-4287)
+        self.k0_policy = horizontal_zero_mode_policy
+        self.k0_smooth = float(k0_ode_smoothness)
+        self.k0_disc = "legacy_plane_centred" if coupled_k0_discretization == \
+            "legacy_plane_centred" else "staggered_face"
+        if k0_update_mu is None and horizontal_zero_mode_policy == "constrained_spline_harmonic":
+            k0_update_mu = 0.2
+        self.k0_mu = None if k0_update_mu is None else float(k0_update_mu)
+        self.k0_nb = int(k0_spline_basis_size)
         self.input_shape_zyx = (int(Nz), int(Ny), int(Nx))
         cx, cy, cz, pad_xy = calculation_grid(int(Nx), int(Ny), int(Nz), pad_calculation_domain,
                                               calculation_domain_mode)
@@ -741,8 +757,13 @@ class OracleNFGS:
             "background_scalar_plus_explicit_rms": float(np.sqrt(np.mean(inv ** 2))),
         }
         self.gaussian_diagnostics.update(self._harmonic_diagnostics())
-        # frozen linear harmonic (nfgs_solver.py:5069-5096)
+        # every closure starts from the bottom target repeated with height
+        # (nfgs_solver.py:5058-5096, :5150-5190)
         self.H = np.repeat(self.H_target[:, None], self.Nz, axis=1).astype(dt)
+        self.k0_residual_history = []
+        self.k0_diag = {}
+        if self.k0_policy != "frozen_linear_harmonic":
+            self._k0_setup()
 
     def _harmonic_diagnostics(self):
         """nfgs_solver.py:4990-5056."""
@@ -943,6 +964,174 @@ class OracleNFGS:
     # Reconstruction, init and one step (nfgs_solver.py:3311-3341,
     # :5901-5942, :6069-6078, :6613-6935)
     # ------------------------------------------------------------------
+    # ------------------------------------------------------------------
+    # Coupled horizontal-harmonic closures (coupled_k0 and
+    # constrained_spline_harmonic): the uniform field H(z) follows the
+    # horizontal mean of the parallel current not carried by the PT scalars
+    # (nfgs_solver.py:5058-5398, :5952-6067, :6760-6791), in FP64.
+    # ------------------------------------------------------------------
+    @staticmethod
+    def bspline_basis(n_points, n_basis):
+        """Clamped open-uniform cubic B-spline collocation (nfgs_solver.py:5098-5147)."""
+        deg = 3
+        inner = n_basis - deg - 1
+        knot = [0.0] * (deg + 1) + [i / (inner + 1) for i in range(1, inner + 1)] + [1.0] * (deg + 1)
+        knot = np.array(knot)
+        t = np.linspace(0.0, 1.0, n_points)
+        width = knot.size - 1
+        basis = np.zeros((n_points, width))
+        for i in range(width):
+            basis[:, i] = ((t >= knot[i]) & (t < knot[i + 1])).astype(float)
+        basis[-1, -1] = 1.0
+        for order in range(1, deg + 1):
+            nxt = np.zeros((n_points, width - 1))
+            for i in range(width - 1):
+                ld = knot[i + order] - knot[i]
+                rd = knot[i + order + 1] - knot[i + 1]
+                if ld > 0.0:
+                    nxt[:, i] += (t - knot[i]) / ld * basis[:, i]
+                if rd > 0.0:
+                    nxt[:, i] += (knot[i + order + 1] - t) / rd * basis[:, i + 1]
+            basis = nxt
+            width -= 1
+            basis[-1] = 0.0
+            basis[-1, -1] = 1.0
+        return basis
+
+    def _k0_setup(self):
+        nz, dz = self.Nz, self.dz
+        s = self.k0_smooth
+        D = np.zeros((nz - 1, nz))
+        r = np.arange(nz - 1)
+        D[r, r] = -1.0 / dz
+        D[r, r + 1] = 1.0 / dz
+        C = np.zeros((max(nz - 2, 0), nz))
+        if nz >= 3:
+            r = np.arange(nz - 2)
+            C[r, r] = 1.0 / dz
+            C[r, r + 1] = -2.0 / dz
+            C[r, r + 2] = 1.0 / dz
+        if self.k0_policy == "constrained_spline_harmonic":
+            basis = self.bspline_basis(nz, self.k0_nb)
+            pieces = [D @ basis]
+            if s > 0.0 and C.shape[0] > 0:
+                pieces.append(np.sqrt(s) * (C @ basis))
+            design = np.concatenate(pieces, axis=0)
+            normal = design.T @ design
+            ridge = max(np.abs(np.diagonal(normal)).max(), 1.0) * 1.0e-12
+            normal = normal + ridge * np.eye(self.k0_nb)
+            top = np.zeros(nz)
+            top[-1], top[-2], top[-3] = 3.0 / (2 * dz), -4.0 / (2 * dz), 1.0 / (2 * dz)
+            cons = np.stack([basis[0], top @ basis])
+            kkt = np.block([[normal, cons.T], [cons, np.zeros((2, 2))]])
+            self.k0_mats = dict(basis=basis, design=design, kkt=kkt, C=C)
+            self.k0_diag = {"enabled": True, "closure": "constrained_spline_harmonic",
+                            "basis_size": self.k0_nb, "smoothness": s,
+                            "update_mu": self.k0_mu,
+                            "kkt_condition_number": float(np.linalg.cond(kkt))}
+            return
+        if self.k0_disc == "legacy_plane_centred":
+            Dl = np.zeros((nz, nz))
+            r = np.arange(1, nz - 1)
+            Dl[r, r - 1] = -0.5 / dz
+            Dl[r, r + 1] = 0.5 / dz
+            Dl[0, :3] = [-1.5 / dz, 2.0 / dz, -0.5 / dz]
+            Dl[nz - 1, nz - 3:] = [0.5 / dz, -2.0 / dz, 1.5 / dz]
+            Cl = Dl @ Dl
+            pieces = [Dl[1:, 1:]]
+            if s > 0.0:
+                pieces.append(np.sqrt(s) * Cl[1:, 1:])
+            D, C = Dl, Cl
+        else:
+            pieces = [D[:, 1:]]
+            if C.shape[0] > 0 and s > 0.0:
+                pieces.append(np.sqrt(s) * C[:, 1:])
+        A = np.concatenate(pieces, axis=0)
+        self.k0_mats = dict(A=A, L=np.linalg.cholesky(A.T @ A), D=D, C=C)
+        self.k0_diag = {"enabled": True, "smoothness": s, "discretization": self.k0_disc}
+
+    def _k0_profile(self, q, h0):
+        """One component of the closure's profile (nfgs_solver.py:5229-5261, :5355-5398)."""
+        m = self.k0_mats
+        s = np.sqrt(self.k0_smooth)
+        if self.k0_policy == "constrained_spline_harmonic":
+            rhs = [0.5 * (q[:-1] + q[1:])]
+            if self.k0_smooth > 0.0 and m["C"].shape[0] > 0:
+                rhs.append(np.zeros(m["C"].shape[0]))
+            nrhs = m["design"].T @ np.concatenate(rhs)
+            sol = np.linalg.solve(m["kkt"], np.concatenate([nrhs, [h0, 0.0]]))
+            return m["basis"] @ sol[:self.k0_nb]
+        D, C = m["D"], m["C"]
+        if self.k0_disc == "legacy_plane_centred":
+            rhs = [q[1:] - D[1:, 0] * h0]
+            if self.k0_smooth > 0.0:
+                rhs.append(-s * C[1:, 0] * h0)
+        else:
+            rhs = [0.5 * (q[:-1] + q[1:]) - D[:, 0] * h0]
+            if C.shape[0] > 0 and self.k0_smooth > 0.0:
+                rhs.append(-s * C[:, 0] * h0)
+        nrhs = m["A"].T @ np.concatenate(rhs)
+        L = m["L"]
+        y = np.linalg.solve(L, nrhs)
+        return np.concatenate([[h0], np.linalg.solve(L.T, y)])
+
+    def _k0_centre_derivative(self, p):
+        """nfgs_solver.py:5400-5431 (either discretisation)."""
+        if self.k0_policy == "coupled_k0" and self.k0_disc == "legacy_plane_centred":
+            return (self.k0_mats["D"] @ p.T).T
+        d = np.zeros_like(p)
+        face = (p[:, 1:] - p[:, :-1]) / self.dz
+        d[:, 1:-1] = 0.5 * (face[:, :-1] + face[:, 1:])
+        d[:, 0] = (-3.0 * p[:, 0] + 4.0 * p[:, 1] - p[:, 2]) / (2.0 * self.dz)
+        d[:, -1] = (3.0 * p[:, -1] - 4.0 * p[:, -2] + p[:, -3]) / (2.0 * self.dz)
+        return d
+
+    def k0_centre_derivative_of(self, p):
+        return self._k0_centre_derivative(np.asarray(p, dtype=np.float64)).astype(self.dtype)
+
+    def _trap_mean(self, f):
+        """The constant DCT-I mode of every plane (nfgs_solver.py:4466-4468)."""
+        return dct2(f)[:, 0, 0] / (4.0 * (self.Nx - 1) * (self.Ny - 1))
+
+    def k0_update(self, jp, phi, az):
+        """The closure's profile from the step's parallel current and the new
+        scalars (nfgs_solver.py:5952-6067, :6760-6791); sets self.H."""
+        bcz = lap_xy(phi, self.dx, self.dy)
+        _, jpt, _ = reconstruct_pt(phi, az, bcz, self.B_background, self.Bz_obs, self.Jz_T,
+                                   self.dx, self.dy, self.dz)
+        want = np.stack([self._trap_mean(jp[0]) - self._trap_mean(jpt[0]),
+                         self._trap_mean(jp[1]) - self._trap_mean(jpt[1])]).astype(np.float64)
+        hb = self.H_target.astype(np.float64)
+        prof = np.stack([self._k0_profile(want[1], hb[0]), self._k0_profile(-want[0], hb[1])])
+        d = self._k0_centre_derivative(prof)
+        actual = np.stack([-d[1], d[0]])
+        res = actual - want
+        rel = np.sqrt(np.mean(res ** 2)) / max(np.sqrt(np.mean(want ** 2)), 1e-30)
+        self.k0_residual_history.append(rel)
+        face = (prof[:, 1:] - prof[:, :-1]) / self.dz
+        want_face = np.stack([0.5 * (want[1, :-1] + want[1, 1:]),
+                              -0.5 * (want[0, :-1] + want[0, 1:])])
+        fres = face - want_face
+        grid = prof[:, 1:-1] - 0.25 * (prof[:, :-2] + 2.0 * prof[:, 1:-1] + prof[:, 2:])
+        prms = np.sqrt(np.mean(prof ** 2))
+        self.k0_diag.update(
+            ode_residual_rms=float(np.sqrt(np.mean(res ** 2))), ode_residual_relative=float(rel),
+            ode_residual_relative_max=float(max(self.k0_residual_history)),
+            face_ode_residual_rms=float(np.sqrt(np.mean(fres ** 2))),
+            face_ode_residual_relative=float(np.sqrt(np.mean(fres ** 2)) /
+                                             max(np.sqrt(np.mean(want_face ** 2)), 1e-30)),
+            gridscale_profile_rms=float(np.sqrt(np.mean(grid ** 2))),
+            gridscale_profile_relative=float(np.sqrt(np.mean(grid ** 2)) / max(prms, 1e-30)),
+            top_derivative=[float(v) for v in d[:, -1]],
+            profile_max_abs=float(np.abs(prof).max()),
+            profile_top=[float(v) for v in prof[:, -1]],
+            desired_mean_current_rms=float(np.sqrt(np.mean(want ** 2))))
+        mu = 1.0 if self.k0_mu is None else self.k0_mu
+        new = prof if mu >= 1.0 else mu * prof + (1.0 - mu) * self.H.astype(np.float64)
+        self.H = new.astype(self.dtype)
+        self.H_unrelaxed = prof
+        return prof
+
     def _apply_bc(self, B, J):
         J[:, :, :, 0] = 0.0
         J[:, :, :, -1] = 0.0
@@ -982,12 +1171,28 @@ class OracleNFGS:
         del d  # contributes only to Jx/Jy, which are overwritten
         Bf, Jf = self.face_state(B, J[2])
         Js = self.faces_to_centres(Jf)
-        J[:2] = Js[:2]
-        J[:, :, :, 0] = 0.0
-        J[:, :, :, -1] = 0.0
-        J[:, :, 0, :] = 0.0
-        J[:, :, -1, :] = 0.0
-        J[:, -1] = 0.0
+        if self.k0_policy == "constrained_spline_harmonic":
+            # walls close the PT current only; the uniform harmonic current
+            # stays (nfgs_solver.py:5919-5932)
+            dk = self.k0_centre_derivative_of(self.H)
+            jh = np.zeros_like(Js)
+            jh[0] = -dk[1][:, None, None]
+            jh[1] = dk[0][:, None, None]
+            jpt = Js - jh
+            jpt[:, :, :, 0] = 0.0
+            jpt[:, :, :, -1] = 0.0
+            jpt[:, :, 0, :] = 0.0
+            jpt[:, :, -1, :] = 0.0
+            jpt[:, -1] = 0.0
+            J = jpt + jh
+            J[:, -1] = 0.0
+        else:
+            J[:2] = Js[:2]
+            J[:, :, :, 0] = 0.0
+            J[:, :, :, -1] = 0.0
+            J[:, :, 0, :] = 0.0
+            J[:, :, -1, :] = 0.0
+            J[:, -1] = 0.0
         J[2, 0] = self.Jz_T
         self.last_face_diag = {
             "face_current_rms": float(np.sqrt(np.mean(Jf.astype(np.float64) ** 2))),
@@ -1031,13 +1236,15 @@ class OracleNFGS:
         return jp, s_bcz, s_jz
 
     def step(self, B, J, Phi_c, Az):
-        _, s_bcz, s_jz = self.sources(B, J)
+        jp, s_bcz, s_jz = self.sources(B, J)
         phi = self.solve_phi_c(s_bcz)
         az = self.solve_az(s_jz)
         phi[0] = self.Phi_above_bottom
         az[0] = self.Az_bottom
         az[-1] = 0.0
         self._az_lateral_zero(az)
+        if self.k0_policy != "frozen_linear_harmonic":
+            self.k0_update(jp, phi, az)
         B2, J2 = self.reconstruct(phi, az, lap_xy(phi, self.dx, self.dy))
         return B2, J2, phi, az
 
@@ -1157,6 +1364,7 @@ class OracleNFGS:
                              "energy_relative_change", "fixed_point_criteria_met")}
         best_sigma, best_iter = s0, 0
         best = (B.copy(), J.copy(), phi.copy(), az.copy())
+        best_h = self.H.copy()
         stall = flat = 0
         need = max(int(patience), 1)
         prev = s0
@@ -1204,6 +1412,7 @@ class OracleNFGS:
             if s < best_sigma:
                 best_sigma, best_iter = s, n
                 best = (B.copy(), J.copy(), phi.copy(), az.copy())
+                best_h = self.H.copy()
                 stall = 0
             else:
                 stall += 1
@@ -1231,6 +1440,9 @@ class OracleNFGS:
                 h[k] = [float(dg[k]) for dg in diags]
         if stop_mode == "fixed_point":
             best = tuple(a.copy() for a in final_state)
+            best_h = self.H.copy()
+        # the selected state's harmonic profile (nfgs_solver.py:6990-7004)
+        self.H = best_h
         h.update(fixed_point_converged=fp_converged, fixed_point_streak=streak)
         h.update(best_sigma=best_sigma, best_iter=best_iter, rollbacks=0, stop_reason=reason,
                  stop_mode=stop_mode,
