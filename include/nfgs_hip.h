@@ -525,6 +525,64 @@ int nfgs_outward_errors(const double* b, const double* c, int ny, int nx, int y0
                         void* stream);
 
 /* ------------------------------------------------------------------------
+ * Coupled horizontal-harmonic closures (horizontal_zero_mode_policy
+ * "coupled_k0" and "constrained_spline_harmonic"; nfgs_solver.py:5058-5398,
+ * :5952-6067, :6760-6791).  The state's uniform horizontal field H(z)
+ * (2 x nz, the plan dtype; nfgs_boundary.h_profile) follows the horizontal
+ * mean of the parallel current the PT scalars do not carry.
+ * ------------------------------------------------------------------------ */
+enum { NFGS_K0_COUPLED = 1, NFGS_K0_SPLINE = 2 };
+enum { NFGS_K0_STAGGERED = 0, NFGS_K0_LEGACY = 1 };
+
+/* Doubles of the closure table of nz planes (nz*nz + nz). */
+long nfgs_k0_table_count(int nz);
+/* Build the table (device, FP64): the profile of one component is
+ * H = M q + v h0 for a current target q (nz) and bottom value h0.  kind
+ * NFGS_K0_COUPLED takes the least-squares solve of nfgs_solver.py:5262-5398
+ * (discretization NFGS_K0_STAGGERED: face D1 and three-point curvature;
+ * NFGS_K0_LEGACY: plane-centred, :5308-5353); NFGS_K0_SPLINE the
+ * constrained cubic B-spline of :5149-5261 (basis_size functions).  Replaces
+ * _initialize_coupled_k0 / _initialize_constrained_spline_harmonic.  With
+ * condition non-NULL the spline's KKT 2-norm condition number is written
+ * there (host double).  Synchronous; builds on the host, copies to tables. */
+int nfgs_k0_tables(int nz, double dz, int kind, int discretization, double smoothness,
+                   int basis_size, double* tables, double* condition);
+
+/* One closure update (nfgs_solver.py:5952-6067, :6760-6791). */
+typedef struct {
+  int kind, discretization; /* as for nfgs_k0_tables                              */
+  double mu;                /* k0_update_mu in (0, 1]                             */
+  const double* tables;     /* nfgs_k0_table_count(nz) doubles                    */
+  const void* h_bottom;     /* 2: the bottom harmonic target (plan dtype)        */
+  const void* h_old;        /* 2*nz profile of the step's starting state          */
+  void* h_new;              /* 2*nz profile of the new state (may equal h_old)    */
+  double* work;             /* 4*nz doubles: trapezoid means by height            */
+  double* diag;             /* NULL or 4*nz doubles: want_x, want_y, H*_x, H*_y    */
+  double* residual;         /* NULL or 1 double: relative ODE residual of H*      */
+} nfgs_k0_step;
+/* h_new = mu H* + (1 - mu) h_old with H* the profile of want = mean(J_par) -
+ * mean(J_pt) per height (the constant DCT-I mode of each plane), J_par the
+ * step's parallel current jp (3*N) and J_pt the PT current of (phi, az). */
+int nfgs_k0_update(const nfgs_plan* plan, const nfgs_k0_step* k0, const void* jp,
+                   const void* phi, const void* az, void* stream);
+/* nfgs_advance_ex with the closure: the update runs between the Poisson
+ * solves and the closure's state pass (nfgs_k0_state_pass, built in the
+ * workspace's s_bcz and hat0), which takes bnd->h_profile == k0->h_new.
+ * With zero_source (the gs_cfcf start) h_new is a copy of h_old. */
+int nfgs_advance_k0(const nfgs_plan* plan, const nfgs_boundary* bnd, const nfgs_workspace* ws,
+                    void* phi_out, void* az_out, void* b_out, int zero_source, int poloidal_mode,
+                    const nfgs_k0_step* k0, void* stream);
+/* nfgs_state_pass of a closure's state (bnd->h_profile: the state's
+ * profile).  The coupled closure's current is the frozen rule's; the spline
+ * closure's (nfgs_solver.py:5919-5941) has J_z the cell projection of the
+ * face average of the PT J_z and the harmonic current (-dH_y/dz, dH_x/dz) on
+ * the side walls, which enter J_par, the monitors and j_out.  work: N + 2*nz
+ * values of the plan dtype for the spline closure, else unused (may be NULL). */
+int nfgs_k0_state_pass(const nfgs_plan* plan, const nfgs_boundary* bnd, int kind, const void* phi,
+                       const void* az, void* b_out, void* jp_out, void* j_out, double* partials,
+                       void* work, void* stream);
+
+/* ------------------------------------------------------------------------
  * Learned warm start.  No counterpart in the reference: its iterate() starts
  * only from gs_cfcf (nfgs_solver.py:6613-6645).  These are the kernels of a
  * small cosine-basis spectral neural operator that maps the gs_cfcf state

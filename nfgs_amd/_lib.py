@@ -37,6 +37,9 @@ POLOIDAL_DIRECT, POLOIDAL_TWO_STAGE = 0, 1
 DF_WEAK, DF_CLIP, DF_IMPRINT, DF_OBS, DF_COUNT = range(5)
 # lateral transform arithmetic of FP32 plans (NFGS_LATERAL_*)
 LATERAL_EXACT, LATERAL_BF16X3 = 0, 1
+# coupled horizontal-harmonic closures (NFGS_K0_*)
+K0_COUPLED, K0_SPLINE = 1, 2
+K0_STAGGERED, K0_LEGACY = 0, 1
 # learned start (NFGS_FNO_*)
 NFGS_FNO_IDENTITY, NFGS_FNO_GELU = 0, 1
 NFGS_FNO_PROJECT, NFGS_FNO_EXPAND = 0, 1
@@ -58,6 +61,12 @@ class GridDesc(C.Structure):
 class Boundary(C.Structure):
     _fields_ = [(n, vp) for n in ("b_bg", "bz_obs", "jz_t", "phi_bottom", "az_bottom",
                                   "phi_bottom_hat", "q_hat", "az_bottom_hat", "h_profile")]
+
+
+class K0Step(C.Structure):
+    _fields_ = [("kind", C.c_int), ("discretization", C.c_int), ("mu", C.c_double),
+                ("tables", vp), ("h_bottom", vp), ("h_old", vp), ("h_new", vp), ("work", vp),
+                ("diag", vp), ("residual", vp)]
 
 
 class Workspace(C.Structure):
@@ -135,6 +144,15 @@ def _declare(lib):
                                           C.c_int, P(C.c_double), C.c_int, C.c_double, vp, vp]),
         "nfgs_source_diagnostics_workspace_bytes": (C.c_long, [vp]),
         "nfgs_source_diagnostics": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp]),
+        # coupled horizontal-harmonic closures (nfgs_k0_*)
+        "nfgs_k0_table_count": (C.c_long, [C.c_int]),
+        "nfgs_k0_tables": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
+                                     vp, P(C.c_double)]),
+        "nfgs_k0_update": (C.c_int, [vp, P(K0Step), vp, vp, vp, vp]),
+        "nfgs_advance_k0": (C.c_int, [vp, P(Boundary), P(Workspace), vp, vp, vp, C.c_int, C.c_int,
+                                      P(K0Step), vp]),
+        "nfgs_k0_state_pass": (C.c_int, [vp, P(Boundary), C.c_int, vp, vp, vp, vp, vp, vp, vp,
+                                         vp]),
         # learned start (nfgs_fno_*)
         "nfgs_fno_pointwise": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
                                          C.c_int, C.c_int, vp, vp]),

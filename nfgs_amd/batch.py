@@ -35,7 +35,7 @@ import torch
 
 from . import _lib as L
 from . import autotune
-from .solver import NFGSSolverGPU, _Driver
+from .solver import COUPLED_CLOSURES, NFGSSolverGPU, _Driver
 
 __all__ = ["Case", "solve_batch"]
 
@@ -207,6 +207,9 @@ def solve_batch(cases: Sequence[Case], *, max_iter=500, tol=1e-5, patience=40,
         if s.record_iteration_diagnostics:
             raise NotImplementedError("solve_batch does not record per-iteration diagnostics; "
                                       "run such a case with NFGSSolverGPU.iterate()")
+        if s.opts.horizontal_zero_mode_policy in COUPLED_CLOSURES:
+            raise NotImplementedError("solve_batch runs the frozen horizontal harmonic only; "
+                                      "run a coupled closure with NFGSSolverGPU.iterate()")
         key = (id(c.boundary) if c.boundary is not None else id(c.bz),
                (s.Nz, s.Ny, s.Nx), c.spacing, c.dtype)
         if key in roots:

@@ -77,4 +77,16 @@ int robin_modal(const nfgs_plan* p, const void* bh, const void* qh, const void* 
 // on nplanes planes, the constant mode set to 0.
 int two_stage_tables(const nfgs_plan* p, const nfgs_boundary* bnd, void* out, hipStream_t s);
 int divide_by_lambda(const nfgs_plan* p, void* hat, int nplanes, hipStream_t s);
+// The state pass with a closure's cell J_z (jzc: a volume, boundary values
+// included) and side-wall current (jh: 2 x nz, J_x then J_y by height) in
+// place of the PT J_z and zero walls; both null is nfgs_state_pass.
+int state_pass_closure(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi,
+                       const void* az, void* b, void* jp, void* jout, double* partials,
+                       const void* jzc, const void* jh, hipStream_t s);
+// The state pass of a closure (kind NFGS_K0_COUPLED or NFGS_K0_SPLINE);
+// the spline closure's inputs to state_pass_closure are built in jz_work
+// (N values) and jh_work (2*nz values), both of the plan dtype.
+int closure_state_pass(const nfgs_plan* p, const nfgs_boundary* bnd, int kind, const void* phi,
+                       const void* az, void* b, void* jp, void* jout, double* partials,
+                       void* jz_work, void* jh_work, hipStream_t s);
 }  // namespace nfgs

@@ -19,9 +19,10 @@ inline const void* at(const nfgs_plan* p, const void* base, long elems) {
 
 }  // namespace
 
-extern "C" int nfgs_advance_ex(const nfgs_plan* p, const nfgs_boundary* bnd,
-                               const nfgs_workspace* ws, void* phi_out, void* az_out, void* b_out,
-                               int zero_source, int poloidal_mode, void* stream) {
+// Sources and both Poisson solves into (phi_out, az_out); the state pass is
+// left to the caller.
+static int advance_solves(const nfgs_plan* p, const nfgs_boundary* bnd, const nfgs_workspace* ws,
+                   void* phi_out, void* az_out, int zero_source, int poloidal_mode, void* stream) {
   using namespace nfgs;
   NFGS_REQUIRE(p && bnd && ws && phi_out && az_out, NFGS_ERR_ARG, "advance: null argument");
   NFGS_REQUIRE(poloidal_mode == NFGS_POLOIDAL_DIRECT || poloidal_mode == NFGS_POLOIDAL_TWO_STAGE,
@@ -86,11 +87,46 @@ extern "C" int nfgs_advance_ex(const nfgs_plan* p, const nfgs_boundary* bnd,
   NFGS_TRY(launch_transform(p, &p->dst_y, NFGS_INVERSE, at(p, ws->hat1, ip), nxi, 1, ip,
                             at(p, az_out, pl + nx + 1), nx, 1, pl, nxi, nz - 2, s));
 
-  // New state: B and its face pass in one z-march (next J_par and the
-  // monitors; B itself only when b_out is given), then the reduction.
-  NFGS_TRY(nfgs_state_pass(p, bnd, phi_out, az_out, b_out, ws->jp, nullptr, ws->partials, stream));
-  NFGS_TRY(nfgs_reduce(p, ws->partials, ws->metrics, stream));
   return NFGS_OK;
+}
+
+// New state: B and its face pass in one z-march (next J_par and the
+// monitors; B itself only when b_out is given), then the reduction.
+static int finish_state(const nfgs_plan* p, const nfgs_boundary* bnd, const nfgs_workspace* ws,
+                 void* phi_out, void* az_out, void* b_out, void* stream) {
+  NFGS_TRY(nfgs_state_pass(p, bnd, phi_out, az_out, b_out, ws->jp, nullptr, ws->partials, stream));
+  return nfgs_reduce(p, ws->partials, ws->metrics, stream);
+}
+
+extern "C" int nfgs_advance_ex(const nfgs_plan* p, const nfgs_boundary* bnd,
+                               const nfgs_workspace* ws, void* phi_out, void* az_out, void* b_out,
+                               int zero_source, int poloidal_mode, void* stream) {
+  NFGS_TRY(advance_solves(p, bnd, ws, phi_out, az_out, zero_source, poloidal_mode, stream));
+  return finish_state(p, bnd, ws, phi_out, az_out, b_out, stream);
+}
+
+extern "C" int nfgs_advance_k0(const nfgs_plan* p, const nfgs_boundary* bnd,
+                               const nfgs_workspace* ws, void* phi_out, void* az_out, void* b_out,
+                               int zero_source, int poloidal_mode, const nfgs_k0_step* k0,
+                               void* stream) {
+  using namespace nfgs;
+  NFGS_REQUIRE(k0 && bnd && k0->h_new == bnd->h_profile, NFGS_ERR_ARG,
+               "advance_k0: the boundary's h_profile must be the step's new profile");
+  NFGS_TRY(advance_solves(p, bnd, ws, phi_out, az_out, zero_source, poloidal_mode, stream));
+  // the closure update reads the step's J_par (still in ws->jp) and the new
+  // scalars; the gs_cfcf start keeps the old profile
+  if (zero_source) {
+    if (k0->h_new != k0->h_old)
+      NFGS_HIP_CHECK(hipMemcpyAsync(k0->h_new, k0->h_old, (size_t)2 * p->d.nz * p->esize,
+                                    hipMemcpyDeviceToDevice, as_stream(stream)));
+  } else {
+    NFGS_TRY(nfgs_k0_update(p, k0, ws->jp, phi_out, az_out, stream));
+  }
+  // the sources and the solves are done with s_bcz and hat0: the spline
+  // closure's J_z and wall current are built there
+  NFGS_TRY(closure_state_pass(p, bnd, k0->kind, phi_out, az_out, b_out, ws->jp, nullptr,
+                              ws->partials, ws->s_bcz, ws->hat0, as_stream(stream)));
+  return nfgs_reduce(p, ws->partials, ws->metrics, stream);
 }
 
 extern "C" int nfgs_advance(const nfgs_plan* p, const nfgs_boundary* bnd,

@@ -616,9 +616,9 @@ template <typename T, bool IX, bool IY, bool ITER>
 __device__ __forceinline__ void state_pass_body(
     StateLds<T>& L, const Tile& t, const Grid<T>& g, const T* __restrict__ phi,
     const T* __restrict__ az, const T* __restrict__ bg, const T* __restrict__ bz_obs,
-    const T* __restrict__ hprof, const T* __restrict__ jzt, T* __restrict__ bout,
-    T* __restrict__ jp, T* __restrict__ jout, double* partials, long pstride, T floor_b2,
-    T amax, const Window& w) {
+    const T* __restrict__ hprof, const T* __restrict__ jzt, const T* __restrict__ jzc,
+    const T* __restrict__ jh, T* __restrict__ bout, T* __restrict__ jp, T* __restrict__ jout,
+    double* partials, long pstride, T floor_b2, T amax, const Window& w) {
   const Halo h = make_halo(t, g);
   const int k0 = t.k0, k1 = min(k0 + g.kc, g.nz);
   const int ps = max(k0 - 1, 0), pe = min(k1, g.nz - 1);
@@ -735,7 +735,9 @@ __device__ __forceinline__ void state_pass_body(
       cb[2] = p == 0 ? ld(bz_obs, h.own) : L.sgz[sl][threadIdx.x] + L.scz[sl][t.c];
       cg[0] = L.sgx[sl][t.c];
       cg[1] = L.sgy[sl][t.c];
-      if (p == 0)
+      if (jzc)
+        cjz = ld(pl(jzc, p), h.own);  // a closure's cell J_z, boundary values included
+      else if (p == 0)
         cjz = jz0;
       else if (p == g.nz - 1 || edge)
         cjz = T(0);
@@ -778,8 +780,9 @@ __device__ __forceinline__ void state_pass_body(
       if (k >= k0 && k < k1) {
         if (has_jp) {
           if (k == 0) {
-            st(pl(jp, k), h.own, edge ? T(0) : F.j[0]);
-            st(pl(jp + g.N, k), h.own, edge ? T(0) : F.j[1]);
+            // the side walls carry the closure's wall current (none: zero)
+            st(pl(jp, k), h.own, edge ? (jh ? jh[0] : T(0)) : F.j[0]);
+            st(pl(jp + g.N, k), h.own, edge ? (jh ? jh[g.nz] : T(0)) : F.j[1]);
             st(pl(jp + 2 * g.N, k), h.own, jz0);
           } else {
             st(pl(jp, k), h.own, T(0.5) * (fprev.p[0] + F.p[0]));
@@ -798,8 +801,8 @@ __device__ __forceinline__ void state_pass_body(
           }
           acc[6] += c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
           acc[5] += F.j[0] * F.j[0] + F.j[1] * F.j[1] + F.j[2] * F.j[2];
-          st(pl(jout, k), h.own, edge ? T(0) : c[0]);
-          st(pl(jout + g.N, k), h.own, edge ? T(0) : c[1]);
+          st(pl(jout, k), h.own, edge ? (jh ? jh[k] : T(0)) : c[0]);
+          st(pl(jout + g.N, k), h.own, edge ? (jh ? jh[g.nz + k] : T(0)) : c[1]);
           st(pl(jout + 2 * g.N, k), h.own, pjz);
         }
         if (has_pa) {
@@ -889,8 +892,9 @@ template <typename T, bool ITER>
 __global__ __launch_bounds__(256) void state_pass_kernel(
     Grid<T> g, const T* __restrict__ phi, const T* __restrict__ az, const T* __restrict__ bg,
     const T* __restrict__ bz_obs, const T* __restrict__ hprof, const T* __restrict__ jzt,
-    T* __restrict__ bout, T* __restrict__ jp, T* __restrict__ jout, double* partials,
-    long pstride, T floor_b2, T amax, Window w) {
+    const T* __restrict__ jzc, const T* __restrict__ jh, T* __restrict__ bout,
+    T* __restrict__ jp, T* __restrict__ jout, double* partials, long pstride, T floor_b2,
+    T amax, Window w) {
   __shared__ StateLds<T> L;
   const Tile t = make_tile(g);
   // centred stencils along an axis when no point the pass touches lies on
@@ -898,8 +902,8 @@ __global__ __launch_bounds__(256) void state_pass_kernel(
   const bool ix = t.x0 >= 2 && t.x0 + TX <= g.nx - 2;
   const bool iy = t.y0 >= 2 && t.y0 + TY <= g.ny - 2;
 #define NFGS_STATE_BODY(IX_, IY_)                                                            \
-  state_pass_body<T, IX_, IY_, ITER>(L, t, g, phi, az, bg, bz_obs, hprof, jzt, bout, jp, jout, \
-                                     partials, pstride, floor_b2, amax, w)
+  state_pass_body<T, IX_, IY_, ITER>(L, t, g, phi, az, bg, bz_obs, hprof, jzt, jzc, jh, bout,  \
+                                     jp, jout, partials, pstride, floor_b2, amax, w)
   if (ix && iy)
     NFGS_STATE_BODY(true, true);
   else if (ix)
@@ -1090,7 +1094,8 @@ int face_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* b, const vo
 
 template <typename T>
 int state_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi, const void* az,
-            void* b, void* jp, void* jout, double* partials, hipStream_t s) {
+            void* b, void* jp, void* jout, double* partials, const void* jzc, const void* jh,
+            hipStream_t s) {
   if (const int rc = check_partials(p, partials)) return rc;
   const Grid<T> g = make_grid<T>(p);
   const Window w{p->d.win_z0, p->d.win_z1, p->d.win_y0, p->d.win_y1, p->d.win_x0, p->d.win_x1};
@@ -1099,6 +1104,7 @@ int state_t(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi, const
                      static_cast<const T*>(phi), static_cast<const T*>(az),
                      static_cast<const T*>(bnd->b_bg), static_cast<const T*>(bnd->bz_obs),
                      static_cast<const T*>(bnd->h_profile), static_cast<const T*>(bnd->jz_t),
+                     static_cast<const T*>(jzc), static_cast<const T*>(jh),
                      static_cast<T*>(b), static_cast<T*>(jp), static_cast<T*>(jout), partials,
                      (long)p->pblocks, T(p->d.weak_field_floor_b2), T(p->d.alpha_max), w);
   NFGS_LAUNCH_CHECK();
@@ -1160,6 +1166,13 @@ int fill_zero_f64(double* ptr, long n, hipStream_t s) {
   return NFGS_OK;
 }
 
+int state_pass_closure(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi,
+                       const void* az, void* b, void* jp, void* jout, double* partials,
+                       const void* jzc, const void* jh, hipStream_t s) {
+  if (p->d.dtype == NFGS_F32) return state_t<float>(p, bnd, phi, az, b, jp, jout, partials, jzc, jh, s);
+  return state_t<double>(p, bnd, phi, az, b, jp, jout, partials, jzc, jh, s);
+}
+
 }  // namespace nfgs
 
 using namespace nfgs;
@@ -1200,9 +1213,8 @@ extern "C" int nfgs_state_pass(const nfgs_plan* p, const nfgs_boundary* bnd, con
   NFGS_REQUIRE(p->d.nz >= 3, NFGS_ERR_ARG, "state_pass: nz must be >= 3");
   NFGS_REQUIRE(bnd->b_bg && bnd->bz_obs && bnd->h_profile && bnd->jz_t, NFGS_ERR_STATE,
                "state_pass: boundary is not set");
-  if (p->d.dtype == NFGS_F32)
-    return state_t<float>(p, bnd, phi, az, b_out, jp_out, j_out, partials, as_stream(stream));
-  return state_t<double>(p, bnd, phi, az, b_out, jp_out, j_out, partials, as_stream(stream));
+  return state_pass_closure(p, bnd, phi, az, b_out, jp_out, j_out, partials, nullptr, nullptr,
+                            as_stream(stream));
 }
 
 extern "C" int nfgs_sources(const nfgs_plan* p, const void* jp, void* s_bcz, void* s_jz,

@@ -38,6 +38,10 @@ def _norm(v) -> str:
     return str(v).lower().replace("-", "_")
 
 
+# horizontal_zero_mode_policy values whose profile H(z) follows the iteration
+COUPLED_CLOSURES = ("coupled_k0", "constrained_spline_harmonic")
+
+
 def _unsupported(what: str):
     raise NotImplementedError(f"{what} is outside the plain NFGS path implemented by nfgs_amd")
 
@@ -202,8 +206,22 @@ class _Options:
             _unsupported("source_partition_mode='legacy_single_field'")
         if self.mean_bz_policy not in {"uniform_open", "require_balanced"}:
             _unsupported(f"mean_bz_policy={self.mean_bz_policy!r}")
-        if self.horizontal_zero_mode_policy not in {"frozen_linear_harmonic", "omit"}:
+        if self.horizontal_zero_mode_policy not in {"frozen_linear_harmonic", "omit",
+                                                    "coupled_k0", "constrained_spline_harmonic"}:
             _unsupported(f"horizontal_zero_mode_policy={self.horizontal_zero_mode_policy!r}")
+        if self.horizontal_zero_mode_policy in COUPLED_CLOSURES:
+            if not self.relax_scalars:
+                raise ValueError("coupled horizontal harmonic modes require relax_scalars=True")
+            if self.k0_current_target_placement != "cell":
+                _unsupported("k0_current_target_placement='face'")
+            if self.record_iteration_diagnostics:
+                _unsupported("record_iteration_diagnostics=True with a coupled horizontal-harmonic "
+                             "closure")
+            if self.k0_update_mu is None and \
+                    self.horizontal_zero_mode_policy == "constrained_spline_harmonic":
+                # the reference's conservative default for the spline closure
+                # (nfgs_solver.py:4143-4147)
+                self.k0_update_mu = 0.2
         if self.external_potential_field is not None:
             raise ValueError("external_potential_field was supplied but neither mean-mode policy "
                              "selects 'external_potential'")
@@ -547,6 +565,14 @@ class NFGSSolverGPU:
         if self.Nx < 3 or self.Ny < 3:
             raise ValueError("the calculation grid needs at least 3 nodes per lateral axis")
         self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+        if opts.horizontal_zero_mode_policy in COUPLED_CLOSURES:
+            if self.Nz < 4:
+                raise ValueError("coupled horizontal harmonic modes need Nz >= 4")
+            if self.Nz > 1024:
+                _unsupported("Nz > 1024 with a coupled horizontal-harmonic closure")
+            if opts.horizontal_zero_mode_policy == "constrained_spline_harmonic" and \
+                    opts.k0_spline_basis_size > self.Nz:
+                raise ValueError("k0_spline_basis_size must not exceed Nz")
         self.dtype = dtype
         if not torch.cuda.is_available():
             raise RuntimeError("NFGSSolverGPU (nfgs_amd) needs a ROCm GPU: the HIP library has no "
@@ -571,6 +597,10 @@ class NFGSSolverGPU:
         self.fov_slice_yx = None
         self._observed_shape_zyx = None
         self._source_zero_mode_history = []
+        self._k0 = None  # coupled closure tables and per-step records
+        self._k0_work = None
+        self._slot_h = None
+        self._H_sel = None
 
     def __del__(self):
         plan = getattr(self, "_plan", None)
@@ -819,10 +849,106 @@ class NFGSSolverGPU:
                 "current_free": True, "open_side_current": False, "top_derivative": [0.0, 0.0],
                 "profile_max_abs": max(abs(hc[0]), abs(hc[1])),
             }
+        elif o.horizontal_zero_mode_policy in COUPLED_CLOSURES:
+            self._H_k0 = h_target.view(2, 1).expand(2, self.Nz).contiguous()
+            self._k0_setup(hc)
         else:
             self._H_k0 = None
             self._k0_diagnostics = {}
         self._H_k0_bottom_target = h_target
+
+    def _k0_setup(self, hc):
+        """Tables and diagnostics of a coupled closure (nfgs_solver.py:5058-5068,
+        :5149-5227, :5262-5353): the profile starts as the bottom target
+        repeated with height and follows the iteration from there."""
+        o = self.opts
+        spline = o.horizontal_zero_mode_policy == "constrained_spline_harmonic"
+        legacy = o.coupled_k0_discretization == "legacy_plane_centred"
+        nz = self.Nz
+        tables = torch.empty(int(self._lib.nfgs_k0_table_count(nz)), device=self.device,
+                             dtype=torch.float64)
+        cond = L.C.c_double(0.0)
+        L.check(self._lib.nfgs_k0_tables(nz, self.dz, L.K0_SPLINE if spline else L.K0_COUPLED,
+                                         L.K0_LEGACY if legacy else L.K0_STAGGERED,
+                                         o.k0_ode_smoothness, o.k0_spline_basis_size,
+                                         L.ptr(tables), L.C.byref(cond)), "k0 tables")
+        mu = 1.0 if o.k0_update_mu is None else float(o.k0_update_mu)
+        self._k0 = dict(kind=L.K0_SPLINE if spline else L.K0_COUPLED,
+                        disc=L.K0_LEGACY if legacy else L.K0_STAGGERED, mu=mu, tables=tables,
+                        work=torch.empty(4 * nz, device=self.device, dtype=torch.float64),
+                        diag=[torch.zeros(4 * nz, device=self.device, dtype=torch.float64)
+                              for _ in range(2)],
+                        res=torch.zeros(4096, device=self.device, dtype=torch.float64))
+        self._k0_residual_history = []
+        if spline:
+            self._k0_diagnostics = {
+                "enabled": True, "closure": "constrained_spline_harmonic",
+                "representation": "cubic_bspline_explicit_linear_harmonic_lift",
+                "basis_size": o.k0_spline_basis_size, "smoothness": o.k0_ode_smoothness,
+                "update_mu": o.k0_update_mu, "bottom_target": list(hc),
+                "bottom_constraint": "exact", "top_constraint": "discrete_second_order_Hprime_zero",
+                "lateral_current_contract": "open_harmonic_plus_closed_PT",
+                "open_side_current": True, "top_derivative": [0.0, 0.0],
+                "kkt_condition_number": float(cond.value)}
+        elif legacy:
+            self._k0_diagnostics = {
+                "enabled": True, "smoothness": o.k0_ode_smoothness,
+                "discretization": "legacy_plane_centred", "bottom_target": list(hc),
+                "open_side_current": True, "diagnosis_only": True}
+        else:
+            self._k0_diagnostics = {
+                "enabled": True, "smoothness": o.k0_ode_smoothness,
+                "discretization": "staggered_face_D1_true_three_point_curvature",
+                "bottom_target": list(hc), "open_side_current": True}
+
+    def _k0_record(self, steps):
+        """The closure's diagnostics after the last of `steps` (iteration
+        numbers) from the device's per-step record (nfgs_solver.py:5966-6010,
+        :6025-6067)."""
+        if self._k0 is None or not steps:
+            return
+        nz, dz = self.Nz, self.dz
+        res = self._k0["res"].cpu().numpy()
+        self._k0_residual_history.extend(float(res[n]) for n in steps)
+        d = self._k0["diag"][steps[-1] & 1].cpu().numpy()
+        want = np.stack([d[:nz], d[nz:2 * nz]])
+        prof = np.stack([d[2 * nz:3 * nz], d[3 * nz:]])
+        legacy = self._k0["disc"] == L.K0_LEGACY
+
+        def centre(p):
+            out = np.zeros_like(p)
+            if legacy:
+                out[:, 1:-1] = -0.5 / dz * p[:, :-2] + 0.5 / dz * p[:, 2:]
+                out[:, 0] = -1.5 / dz * p[:, 0] + 2.0 / dz * p[:, 1] - 0.5 / dz * p[:, 2]
+                out[:, -1] = 0.5 / dz * p[:, -3] - 2.0 / dz * p[:, -2] + 1.5 / dz * p[:, -1]
+                return out
+            face = (p[:, 1:] - p[:, :-1]) / dz
+            out[:, 1:-1] = 0.5 * (face[:, :-1] + face[:, 1:])
+            out[:, 0] = (-3.0 * p[:, 0] + 4.0 * p[:, 1] - p[:, 2]) / (2.0 * dz)
+            out[:, -1] = (3.0 * p[:, -1] - 4.0 * p[:, -2] + p[:, -3]) / (2.0 * dz)
+            return out
+
+        der = centre(prof)
+        actual = np.stack([-der[1], der[0]])
+        rms = lambda a: float(np.sqrt(np.mean(a ** 2)))
+        r_rms, w_rms = rms(actual - want), rms(want)
+        face = (prof[:, 1:] - prof[:, :-1]) / dz
+        want_face = np.stack([0.5 * (want[1, :-1] + want[1, 1:]),
+                              -0.5 * (want[0, :-1] + want[0, 1:])])
+        f_rms = rms(face - want_face)
+        grid = prof[:, 1:-1] - 0.25 * (prof[:, :-2] + 2.0 * prof[:, 1:-1] + prof[:, 2:])
+        p_rms = rms(prof)
+        self._k0_diagnostics.update(
+            ode_residual_rms=r_rms, ode_residual_relative=r_rms / max(w_rms, 1e-30),
+            ode_residual_relative_max=max(self._k0_residual_history),
+            face_ode_residual_rms=f_rms,
+            face_ode_residual_relative=f_rms / max(rms(want_face), 1e-30),
+            gridscale_profile_rms=rms(grid),
+            gridscale_profile_relative=rms(grid) / max(p_rms, 1e-30),
+            top_derivative=[float(v) for v in der[:, -1]],
+            profile_max_abs=float(np.abs(prof).max()),
+            profile_top=[float(v) for v in prof[:, -1]],
+            desired_mean_current_rms=w_rms)
 
     def _handle_projected_zero_mode(self, ratio, tolerance, policy, message):
         """nfgs_solver.py:4477-4487."""
@@ -873,6 +999,9 @@ class NFGSSolverGPU:
                                  ("jp", "s_bcz", "s_jz", "hat0", "hat1", "cscr")),
                                L.ptr(self._partials), L.ptr(self._metrics))
         self._slots = [self._new_slot() for _ in range(3)]
+        # coupled closures: every slot carries its own profile H(z)
+        self._slot_h = [self._H_k0.clone() for _ in range(3)] if self._k0 is not None else None
+        self._H_sel = None
         self._Bwork = self._empty(3, Nz, Ny, Nx)
 
     def _new_slot(self):
@@ -917,6 +1046,12 @@ class NFGSSolverGPU:
         self._reset_state()
         for key in self._BOUNDARY_ATTRS:
             setattr(self, key, getattr(other, key))
+        if self.opts.horizontal_zero_mode_policy in COUPLED_CLOSURES:
+            # this solver's closure starts from the setup profile with its own
+            # tables and records
+            self._H_k0 = self._H_k0_bottom_target.view(2, 1).expand(2, self.Nz).contiguous()
+            self._k0_setup(list(self.gaussian_separation_diagnostics[
+                "harmonic_residual_constant"]))
         self._make_plan()
         self._bind_boundary()
 
@@ -935,16 +1070,46 @@ class NFGSSolverGPU:
         if self.B_background is None:
             raise RuntimeError("NFGS boundary is not set. Call set_bottom_vector_boundary() first.")
 
-    def _advance(self, slot: int, zero_source: bool, want_b: bool = False):
+    def _advance(self, slot: int, zero_source: bool, want_b: bool = False,
+                 src: Optional[int] = None, step: int = 0):
         """One iteration into `slot`; B of the new state lands in _Bwork only
-        when want_b (the iteration itself never needs B in memory)."""
+        when want_b (the iteration itself never needs B in memory).  With a
+        coupled closure the step starts from slot `src`'s profile (None: the
+        solver's current profile H_k0) and records its closure diagnostics
+        under iteration number `step`."""
         phi, az = self._slots[slot]
         mode = L.POLOIDAL_TWO_STAGE if self.opts.poloidal_update_mode == "two_stage" \
             else L.POLOIDAL_DIRECT
-        L.check(self._lib.nfgs_advance_ex(self._plan, L.C.byref(self._bnd), L.C.byref(self._ws),
-                                          L.ptr(phi), L.ptr(az),
-                                          L.ptr(self._Bwork) if want_b else None,
-                                          int(zero_source), mode, self._stream), "iteration")
+        b_out = L.ptr(self._Bwork) if want_b else None
+        if self._k0 is None:
+            L.check(self._lib.nfgs_advance_ex(self._plan, L.C.byref(self._bnd),
+                                              L.C.byref(self._ws), L.ptr(phi), L.ptr(az), b_out,
+                                              int(zero_source), mode, self._stream), "iteration")
+            return
+        k = self._k0
+        h_old = self._H_k0 if zero_source or src is None else self._slot_h[src]
+        h_new = self._slot_h[slot]
+        if step < 0 or step >= k["res"].numel():
+            raise IndexError("closure record index out of range")
+        rec = L.K0Step(k["kind"], k["disc"], k["mu"], L.ptr(k["tables"]),
+                       L.ptr(self._H_k0_bottom_target), L.ptr(h_old), L.ptr(h_new),
+                       L.ptr(k["work"]), L.ptr(k["diag"][step & 1]), L.ptr(k["res"]) + 8 * step)
+        L.check(self._lib.nfgs_advance_k0(self._plan, L.C.byref(self._bnd_for(h_new)),
+                                          L.C.byref(self._ws), L.ptr(phi), L.ptr(az), b_out,
+                                          int(zero_source), mode, L.C.byref(rec), self._stream),
+                "iteration")
+
+    def _h_of(self, slot):
+        """Profile of a slot index under a coupled closure (else None)."""
+        return None if self._slot_h is None or not isinstance(slot, int) else self._slot_h[slot]
+
+    def _bnd_for(self, h):
+        """The boundary record with another h_profile (a slot's profile)."""
+        if h is None:
+            return self._bnd
+        b = L.Boundary(*[getattr(self._bnd, f) for f, _ in L.Boundary._fields_])
+        b.h_profile = L.ptr(h)
+        return b
 
     def _read_metrics(self):
         self._metrics_host.copy_(self._metrics, non_blocking=True)
@@ -988,16 +1153,36 @@ class NFGSSolverGPU:
                                          self.source_zero_mode_policy,
                                          "The Phi_c nonlinear source has an incompatible zero mode")
 
-    def _state_fields(self, slot, B_out=None, J_out=None, face_diag: bool = False):
+    def _pass(self, phi, az, b=None, jp=None, j=None, partials=None, h=None):
+        """The state pass of (phi, az) into the given outputs (tensors or
+        None).  Under a coupled closure it takes the state's profile h and
+        forms the closure's current (nfgs_k0_state_pass)."""
+        args = (L.ptr(phi), L.ptr(az), L.ptr(b) if b is not None else None,
+                L.ptr(jp) if jp is not None else None, L.ptr(j) if j is not None else None,
+                L.ptr(partials) if partials is not None else None)
+        if self._k0 is None:
+            L.check(self._lib.nfgs_state_pass(self._plan, L.C.byref(self._bnd), *args,
+                                              self._stream), "state pass")
+            return
+        work = None
+        if self._k0["kind"] == L.K0_SPLINE:
+            if self._k0_work is None:
+                self._k0_work = self._empty(self.Nz * self.Ny * self.Nx + 2 * self.Nz)
+            work = L.ptr(self._k0_work)
+        L.check(self._lib.nfgs_k0_state_pass(self._plan, L.C.byref(self._bnd_for(h)),
+                                             self._k0["kind"], *args, work, self._stream),
+                "state pass")
+
+    def _state_fields(self, slot, B_out=None, J_out=None, face_diag: bool = False, h=None):
         """Rebuild B and J of a slot index or a (Phi_c, Az) pair (identical to
-        the iteration's own)."""
+        the iteration's own); h is the pair's profile under a coupled
+        closure (a slot index brings its own)."""
         phi, az = self._slots[slot] if isinstance(slot, int) else slot
+        if h is None:
+            h = self._h_of(slot)
         B = self._empty(3, self.Nz, self.Ny, self.Nx) if B_out is None else B_out
         J = self._empty(3, self.Nz, self.Ny, self.Nx) if J_out is None else J_out
-        L.check(self._lib.nfgs_state_pass(self._plan, L.C.byref(self._bnd), L.ptr(phi), L.ptr(az),
-                                          L.ptr(B), None, L.ptr(J),
-                                          L.ptr(self._partials) if face_diag else None,
-                                          self._stream), "state pass")
+        self._pass(phi, az, B, None, J, self._partials if face_diag else None, h)
         if face_diag:
             L.check(self._lib.nfgs_reduce(self._plan, L.ptr(self._partials), L.ptr(self._metrics),
                                           self._stream), "reduce")
@@ -1081,9 +1266,9 @@ class NFGSSolverGPU:
         del ws
         B = self._empty(3, Nz, Ny, Nx)
         J = self._empty(3, Nz, Ny, Nx)
-        L.check(lib.nfgs_state_pass(plan, L.C.byref(self._bnd), L.ptr(phi), L.ptr(az), L.ptr(B),
-                                    L.ptr(self._ws_t["jp"]), L.ptr(J), L.ptr(self._partials), s),
-                "state pass")
+        if self._slot_h is not None:
+            self._slot_h[0].copy_(self._H_k0)
+        self._pass(phi, az, B, self._ws_t["jp"], J, self._partials, self._h_of(0))
         L.check(lib.nfgs_reduce(plan, L.ptr(self._partials), L.ptr(self._metrics), s), "reduce")
         d = torch.zeros(4, device=self.device, dtype=torch.float64)
         L.check(lib.nfgs_sq_diff(plan, L.ptr(B), L.ptr(B_in), 3 * N, L.ptr(d), s), "field error")
@@ -1141,11 +1326,11 @@ class NFGSSolverGPU:
         use = math.isfinite(sigma_l) and sigma_l < sigma_gs
         if use:
             self._slots[0], self._slots[1] = self._slots[1], self._slots[0]
+            if self._slot_h is not None:
+                self._slot_h[0], self._slot_h[1] = self._slot_h[1], self._slot_h[0]
         else:
             phi0, az0 = self._slots[0]
-            L.check(lib.nfgs_state_pass(plan, L.C.byref(self._bnd), L.ptr(phi0), L.ptr(az0), None,
-                                        L.ptr(self._ws_t["jp"]), None, L.ptr(self._partials), s),
-                    "state pass")
+            self._pass(phi0, az0, None, self._ws_t["jp"], None, self._partials, self._h_of(0))
             L.check(lib.nfgs_reduce(plan, L.ptr(self._partials), L.ptr(self._metrics), s), "reduce")
         self._init_diagnostics = {
             "init_mode": "learned", "learned_start_used": bool(use),
@@ -1168,9 +1353,7 @@ class NFGSSolverGPU:
         if self._Phi_c is None:
             raise RuntimeError("No selected state yet. Call iterate() first.")
         jp = self._empty(3, self.Nz, self.Ny, self.Nx)
-        L.check(self._lib.nfgs_state_pass(self._plan, L.C.byref(self._bnd), L.ptr(self._Phi_c),
-                                          L.ptr(self._Az), None, L.ptr(jp), None, None,
-                                          self._stream), "state pass")
+        self._pass(self._Phi_c, self._Az, None, jp, None, None, self._H_sel)
         return jp
 
     def warm_start_state(self):
@@ -1229,6 +1412,10 @@ class NFGSSolverGPU:
         self._advance(1, zero_source=False)
         _, _, _, ratio = self._read_metrics()
         self._check_zero_mode(ratio)
+        if self._k0 is not None:
+            # the step's profile becomes the solver's (nfgs_solver.py:6760-6791)
+            self._H_k0 = self._slot_h[1].clone()
+            self._k0_record([0])
         Bn, Jn = self._state_fields(1)
         phi, az = self._slots[1]
         if diag is not None:
@@ -1604,6 +1791,10 @@ class NFGSSolverGPU:
         diag = self._diagnostics() if self.record_iteration_diagnostics else None
         self._iteration_diagnostics_history = []
         self._last_iteration_diagnostics = {}
+        if self._k0 is not None and self._k0["res"].numel() < int(max_iter) + 2:
+            # one closure residual per step, read once the loop ends
+            self._k0["res"] = torch.zeros(int(max_iter) + 2, device=self.device,
+                                          dtype=torch.float64)
         # initial state in slot 0 (gs_cfcf, or a caller's field) and its monitors
         if mode == "field":
             self._init_from_field(init_field, init_az, init_current)
@@ -1646,7 +1837,8 @@ class NFGSSolverGPU:
             s_new = next(k for k in range(3) if k not in (best, old))
             if diag is not None:
                 diag.edges()
-            self._advance(s_new, zero_source=False, want_b=not ahead or diag is not None)
+            self._advance(s_new, zero_source=False, want_b=not ahead or diag is not None,
+                          src=old, step=n_step)
             if diag is not None:
                 diag.after_step(self._slots[old], self._slots[s_new], self._Bwork)
                 diag.post(n_step & 1)
@@ -1673,11 +1865,14 @@ class NFGSSolverGPU:
             if per_step_callback is not None:
                 # the callback owns fresh B and J of this step, as the
                 # reference hands it the new state tensors (:3736-3737)
-                B_cb = self._Bwork.clone()
-                J_cb = self._empty(3, self.Nz, self.Ny, self.Nx)
-                L.check(self._lib.nfgs_face_pass(self._plan, L.C.byref(self._bnd),
-                                                 L.ptr(B_cb), L.ptr(self._slots[slot][1]),
-                                                 None, L.ptr(J_cb), None, self._stream), "J")
+                if self._k0 is not None:
+                    B_cb, J_cb = self._state_fields(slot)
+                else:
+                    B_cb = self._Bwork.clone()
+                    J_cb = self._empty(3, self.Nz, self.Ny, self.Nx)
+                    L.check(self._lib.nfgs_face_pass(self._plan, L.C.byref(self._bnd),
+                                                     L.ptr(B_cb), L.ptr(self._slots[slot][1]),
+                                                     None, L.ptr(J_cb), None, self._stream), "J")
                 per_step_callback(n, sigma, B_cb, J_cb)
             if drv.accept(n, sigma):
                 best_slot = slot
@@ -1686,15 +1881,20 @@ class NFGSSolverGPU:
                       f"mu={1.0:.3f}")
             if drv.check_stop(n, verbose):
                 break
+        if self._k0 is not None:
+            self._k0_record(list(range(1, len(drv.history["sigma"]) + 1)))
         return self._finish(drv, self._slots[best_slot], self._slots[final_slot],
-                            best_slot == final_slot)
+                            best_slot == final_slot, self._h_of(best_slot),
+                            self._h_of(final_slot))
 
-    def _finish(self, drv, best_state, final_state, same: bool):
+    def _finish(self, drv, best_state, final_state, same: bool, best_h=None, final_h=None):
         """Selected state, final-vs-best, the staggered diagnostic and the
         history of the reference driver (nfgs_solver.py:3816-3850,
-        :6956-7008).  best_state and final_state are (Phi_c, Az) pairs."""
+        :6956-7008).  best_state and final_state are (Phi_c, Az) pairs,
+        best_h and final_h their profiles under a coupled closure."""
         selected = final_state if drv.stop_mode == "fixed_point" else best_state
-        B_sel, J_sel = self._state_fields(selected, face_diag=True)
+        h_sel = final_h if drv.stop_mode == "fixed_point" else best_h
+        B_sel, J_sel = self._state_fields(selected, face_diag=True, h=h_sel)
         m = self._metrics.cpu().numpy()
         N = float(self.Nz * self.Ny * self.Nx)
         self._last_staggered_face_diagnostics = {
@@ -1704,8 +1904,9 @@ class NFGSSolverGPU:
             "face_planes": self.Nz - 1,
         }
         if not same:
-            B_fin, _ = self._state_fields(final_state)
-            B_best = B_sel if selected is best_state else self._state_fields(best_state)[0]
+            B_fin, _ = self._state_fields(final_state, h=final_h)
+            B_best = B_sel if selected is best_state else \
+                self._state_fields(best_state, h=best_h)[0]
             d = torch.zeros(2, device=self.device, dtype=torch.float64)
             L.check(self._lib.nfgs_sq_diff(self._plan, L.ptr(B_fin), L.ptr(B_best), 3 * int(N),
                                            L.ptr(d), self._stream), "final vs best")
@@ -1716,6 +1917,11 @@ class NFGSSolverGPU:
             fvb = 0.0
         self._B, self._J = B_sel, J_sel
         self._Phi_c, self._Az = (t.clone() for t in selected)
+        if h_sel is not None:
+            # the selected state's profile becomes the solver's
+            # (nfgs_solver.py:6990-7004)
+            self._H_sel = h_sel.clone()
+            self._H_k0 = self._H_sel
         history = drv.history
         final_sigma = history["sigma"][-1] if history["sigma"] else drv.sigma0
         history.update(drv.summary())
@@ -1794,10 +2000,12 @@ class NFGSSolverGPU:
                 "bottom_horizontal_reconstruction_ratio", 0.0)),
         })
         diag.update(self.gaussian_separation_diagnostics)
-        if o_.horizontal_zero_mode_policy == "frozen_linear_harmonic":
+        policy = o_.horizontal_zero_mode_policy
+        if policy in ("frozen_linear_harmonic",) + COUPLED_CLOSURES:
+            # (nfgs_solver.py:7171-7195)
             prof = self._H_k0.double().cpu().numpy()
-            diag["frozen_linear_harmonic"] = dict(self._k0_diagnostics)
-            diag["frozen_linear_harmonic"]["profile_by_height"] = [list(map(float, r)) for r in prof]
+            diag[policy] = dict(self._k0_diagnostics)
+            diag[policy]["profile_by_height"] = [list(map(float, r)) for r in prof]
         return diag
 
     def fields_to_numpy(self, *, crop_to_observed_fov: bool = True):

@@ -56,11 +56,35 @@ def test_option_validation_mirrors_reference():
     with pytest.raises(ValueError, match="alpha_clip_scale"):
         _Options(dict(alpha_clip_scale=2.0, alpha_limit_factor=1.0))
     with pytest.raises(NotImplementedError):
-        _Options(dict(horizontal_zero_mode_policy="coupled_k0"))
-    with pytest.raises(NotImplementedError):
         _Options(dict(vertical_current_operator="collocated_centered"))
     o = _Options(dict(bottom_divergence_robin_weight=0.5, alpha_clip_scale=1.0))
     assert o.rho == 0.5 and o.alpha_limit_factor == 1.0
+
+
+def test_coupled_closure_options():
+    """Both per-iteration closures are accepted; the spline's relaxation
+    defaults to 0.2 (nfgs_solver.py:4143-4147); the combinations the
+    reference rejects, or this build does not run, raise before any device
+    work."""
+    for policy in ("coupled_k0", "constrained_spline_harmonic"):
+        o = _Options(dict(horizontal_zero_mode_policy=policy))
+        assert o.horizontal_zero_mode_policy == policy
+        with pytest.raises(ValueError, match="relax_scalars"):
+            _Options(dict(horizontal_zero_mode_policy=policy, relax_scalars=False))
+        with pytest.raises(NotImplementedError):
+            _Options(dict(horizontal_zero_mode_policy=policy, record_iteration_diagnostics=True))
+        with pytest.raises(NotImplementedError):
+            _Options(dict(horizontal_zero_mode_policy=policy,
+                          k0_current_target_placement="face"))
+    assert _Options(dict(horizontal_zero_mode_policy="constrained_spline_harmonic")
+                    ).k0_update_mu == 0.2
+    assert _Options(dict(horizontal_zero_mode_policy="coupled_k0")).k0_update_mu is None
+    with pytest.raises(ValueError, match="k0_spline_basis_size"):
+        NFGSSolverGPU(12, 10, 5, 1, 1, 1, horizontal_zero_mode_policy="constrained_spline_harmonic",
+                      k0_spline_basis_size=12, pad_calculation_domain=False)
+    with pytest.raises(ValueError, match="Nz >= 4"):
+        NFGSSolverGPU(12, 10, 3, 1, 1, 1, horizontal_zero_mode_policy="coupled_k0",
+                      pad_calculation_domain=False)
 
 
 @pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU failure mode")
