@@ -28,6 +28,11 @@ Parity pins (what checks each part; see tests/test_oracle_*.py):
                          bottom plane (test_oracle_above_field.py)
   * td_metrics           the divergence term on fields of known divergence
                          (test_controlled_contract.py)
+  * coupled closures     reference vectors (tests/golden/make_k0.py), the
+                         profile solves' normal equations (test_oracle_k0.py)
+  * soft_cfcf start      reference vectors (tests/golden/make_soft.py); the
+                         exact rows, minimality of the modal form and the
+                         eps = 0 closed form (test_oracle_soft.py)
 No function is left "parity unpinned".
 """
 from __future__ import annotations
@@ -522,7 +527,7 @@ class OracleNFGS:
                  mean_mode_tolerance=1e-6, poloidal_update_mode="direct",
                  horizontal_zero_mode_policy="frozen_linear_harmonic", k0_ode_smoothness=1.0e-3,
                  coupled_k0_discretization="staggered_face", k0_update_mu=None,
-                 k0_spline_basis_size=8):
+                 k0_spline_basis_size=8, soft_cfcf_epsilon=None, soft_cfcf_normal_ridge=0.0):
         if poloidal_update_mode not in ("direct", "two_stage"):
             raise ValueError("poloidal_update_mode must be 'direct' or 'two_stage'")
         if horizontal_zero_mode_policy not in ("frozen_linear_harmonic", "coupled_k0",
@@ -556,6 +561,12 @@ class OracleNFGS:
         self.d0_policy = d0_zero_mode_policy
         self.mean_tol = float(mean_mode_tolerance)
         self.zero_mode_history = []
+        # soft_cfcf start (nfgs_solver.py:3991-3996, :4218-4223, :4298-4305)
+        h = min(self.dx, self.dy, self.dz)
+        self.soft_eps = 1.0e-4 / (h * h) if soft_cfcf_epsilon is None else float(soft_cfcf_epsilon)
+        self.soft_ridge = float(soft_cfcf_normal_ridge)
+        if self.soft_eps < 0.0 or self.soft_ridge < 0.0:
+            raise ValueError("soft_cfcf epsilon and ridge must be non-negative")
 
     # -- tables (nfgs_solver.py:1083-1118, :4357-4383) --
     def lam_neumann(self):
@@ -1207,6 +1218,182 @@ class OracleNFGS:
         B, J = self.reconstruct(phi, az, lap_xy(phi, self.dx, self.dy))
         return B, J, phi, az
 
+    # ------------------------------------------------------------------
+    # soft_cfcf start (nfgs_solver.py:6152-6611)
+    # ------------------------------------------------------------------
+    def _soft_profile_map(self, k, kind):
+        """The z profile of every mode as p = E u + e b: u the free values,
+        b the bottom value.  The bottom value is exact; for Phi_c the second
+        plane makes the bottom derivative equal k b (:6244-6256), and the top
+        plane closes the transparent Robin row (3 p_N - 4 p_{N-1} + p_{N-2}) /
+        (2 dz) + k p_N = 0 (:6266-6271).  k: (M,).  Returns E (M, N, n) and
+        e (M, N)."""
+        N, dz, M = self.Nz, self.dz, k.shape[0]
+        first = 2 if kind == "phi" else 1  # plane of the first free value
+        n = N - 1 - first
+        E = np.zeros((M, N, n))
+        e = np.zeros((M, N))
+        e[:, 0] = 1.0
+        for z in range(first, N - 1):
+            E[:, z, z - first] = 1.0
+        if kind == "phi":
+            E[:, 1, 0] = 0.25
+            e[:, 1] = 0.75 + 0.5 * dz * k
+        den = 3.0 + 2.0 * dz * k
+        E[:, N - 1] = (4.0 / den)[:, None] * E[:, N - 2] - (1.0 / den)[:, None] * E[:, N - 3]
+        e[:, N - 1] = (4.0 / den) * e[:, N - 2] - (1.0 / den) * e[:, N - 3]
+        return E, e
+
+    def _soft_operators(self):
+        """Second derivative on the interior planes and the first derivative
+        of an interior sequence (one-sided second order at its ends), as
+        matrices in z (:6178-6238)."""
+        N, dz = self.Nz, self.dz
+        D2 = np.zeros((N - 2, N))
+        for r in range(N - 2):
+            D2[r, r:r + 3] = np.array([1.0, -2.0, 1.0]) / (dz * dz)
+        m = N - 2
+        D1 = np.zeros((m, m))
+        D1[0, :3] = np.array([-3.0, 4.0, -1.0]) / (2.0 * dz)
+        for r in range(1, m - 1):
+            D1[r, r - 1], D1[r, r + 1] = -1.0 / (2.0 * dz), 1.0 / (2.0 * dz)
+        D1[-1, -3:] = np.array([1.0, -4.0, 3.0]) / (2.0 * dz)
+        Dc = np.zeros((N - 2, N))  # centred first derivative at the interior planes
+        for r in range(N - 2):
+            Dc[r, r], Dc[r, r + 2] = -1.0 / (2.0 * dz), 1.0 / (2.0 * dz)
+        return D2, D1, Dc
+
+    def soft_modes(self, k, bottom, kind):
+        """Modal profiles minimising the modal form of int |curl J_c|^2 +
+        eps int |J_c|^2 (:6322-6402) with the essential rows exact: for
+        Phi_c |d_z L_k p|^2 + (k^2 + eps) |L_k p|^2, for Az |L_k p|^2 +
+        eps (|d_z p|^2 + k^2 |p|^2), L_k = d_zz - k^2 on the interior
+        planes.  The normal equations of each mode are solved densely (the
+        reference factors the band); a ridge adds ridge * max(1, max|H_ii|)
+        to the diagonal.  k, bottom: arrays of modes; modes with k <= 0 give
+        zero.  Returns (N,) + k.shape profiles in FP64."""
+        k = np.asarray(k, dtype=np.float64)
+        b = np.asarray(bottom, dtype=np.float64)
+        shape = k.shape
+        kf, bf = k.reshape(-1), b.reshape(-1)
+        out = np.zeros((self.Nz, kf.size))
+        act = np.nonzero(kf > 0.0)[0]
+        D2, D1, Dc = self._soft_operators()
+        N = self.Nz
+        eps = self.soft_eps
+        for start in range(0, act.size, 2048):
+            idx = act[start:start + 2048]
+            kk, bb = kf[idx], bf[idx]
+            E, e = self._soft_profile_map(kk, kind)
+            k2 = kk * kk
+            # L_k on the interior planes, per mode: (M, N-2, N)
+            Lk = D2[None] - k2[:, None, None] * np.eye(N)[None, 1:N - 1]
+            if kind == "phi":
+                blocks = [(np.einsum("ij,mjk->mik", D1, Lk), np.ones_like(kk)),
+                          (Lk, k2 + eps)]
+            else:
+                blocks = [(Lk, np.ones_like(kk))]
+                if eps > 0.0:
+                    blocks += [(np.broadcast_to(Dc, (kk.size,) + Dc.shape), np.full_like(kk, eps)),
+                               (np.broadcast_to(np.eye(N)[1:N - 1], (kk.size, N - 2, N)),
+                                eps * k2)]
+            n = E.shape[2]
+            H = np.zeros((kk.size, n, n))
+            g = np.zeros((kk.size, n))
+            for R, w in blocks:
+                A = np.einsum("mij,mjk->mik", R, E)          # rows on the free values
+                c = np.einsum("mij,mj->mi", R, e) * bb[:, None]  # rows' constants
+                H += w[:, None, None] * np.einsum("mri,mrj->mij", A, A)
+                g += w[:, None] * np.einsum("mri,mr->mi", A, c)
+            if self.soft_ridge > 0.0:
+                dmax = np.maximum(np.abs(np.diagonal(H, axis1=1, axis2=2)).max(axis=1), 1.0)
+                H[:, np.arange(n), np.arange(n)] += (self.soft_ridge * dmax)[:, None]
+            u = np.linalg.solve(H, -g[..., None])[..., 0]
+            out[:, idx] = (np.einsum("mzi,mi->mz", E, u) + e * bb[:, None]).T
+        return out.reshape((self.Nz,) + shape)
+
+    def soft_initial_state(self):
+        """Phi_c and Az of the soft_cfcf start and their state
+        (:6423-6484): every DCT mode of Phi_c and every DST mode of Az
+        from soft_modes, the bottom planes and the lateral Az walls fixed
+        afterwards.  Returns B, J, Phi_c, Az and the modal profiles."""
+        if self.Nz < 5:
+            raise ValueError("soft_cfcf requires Nz >= 5")
+        dt = self.dtype
+        pbh = dct2(self.Phi_above_bottom)
+        pbh[0, 0] = 0.0
+        phi_hat = self.soft_modes(self.k, pbh, "phi").astype(dt)
+        phi = idct2(phi_hat).astype(dt)
+        phi[0] = self.Phi_above_bottom
+        abh = dst1(dst1(self.Az_bottom[1:-1, 1:-1], -1), -2)
+        ak = np.sqrt(np.clip(-self.lam_dirichlet(), 0.0, None)).astype(dt)
+        az_hat = self.soft_modes(ak, abh, "az").astype(dt)
+        az = np.zeros((self.Nz, self.Ny, self.Nx), dtype=dt)
+        az[:, 1:-1, 1:-1] = idst1(idst1(az_hat, -1), -2)
+        az[0] = self.Az_bottom
+        B, J = self.reconstruct(phi, az, lap_xy(phi, self.dx, self.dy))
+        self.soft_profiles = (phi_hat, az_hat, ak)
+        return B, J, phi, az
+
+    def soft_diagnostics(self, B, J, phi, az):
+        """The soft_cfcf diagnostics (:6486-6611): the modal energies of the
+        profiles, the reconstructed state's curl and current energies, the
+        bottom rows and the transparent top rows."""
+        phi_hat, az_hat, ak = (np.asarray(a, dtype=np.float64) for a in self.soft_profiles)
+        dz = self.dz
+        kp = np.asarray(self.k, dtype=np.float64)
+        q = (phi_hat[2:] - 2.0 * phi_hat[1:-1] + phi_hat[:-2]) / (dz * dz) - kp ** 2 * phi_hat[1:-1]
+        dq = np.empty_like(q)
+        dq[0] = (-3.0 * q[0] + 4.0 * q[1] - q[2]) / (2.0 * dz)
+        dq[1:-1] = (q[2:] - q[:-2]) / (2.0 * dz)
+        dq[-1] = (3.0 * q[-1] - 4.0 * q[-2] + q[-3]) / (2.0 * dz)
+        w = np.ones((self.Ny, self.Nx))
+        w[[0, -1], :] *= 0.5
+        w[:, [0, -1]] *= 0.5
+        phi_cur = kp ** 2 * (q ** 2).sum(axis=0)
+        phi_curl = kp ** 2 * (dq ** 2 + kp ** 2 * q ** 2).sum(axis=0)
+        qa = (az_hat[2:] - 2.0 * az_hat[1:-1] + az_hat[:-2]) / (dz * dz) - ak ** 2 * az_hat[1:-1]
+        da = (az_hat[2:] - az_hat[:-2]) / (2.0 * dz)
+        az_cur = ak ** 2 * (da ** 2 + ak ** 2 * az_hat[1:-1] ** 2).sum(axis=0)
+        az_curl = ak ** 2 * (qa ** 2).sum(axis=0)
+        f = self.dx * self.dy * dz / (4.0 * (self.Nx - 1) * (self.Ny - 1))
+        curl_e = f * ((w * phi_curl).sum() + az_curl.sum())
+        cur_e = f * ((w * phi_cur).sum() + az_cur.sum())
+        Jd = {"Bx": J[0], "By": J[1], "Bz": J[2]}
+        cj = fd2_curl(Jd, (self.dx, self.dy, dz))
+        vol = self.dx * self.dy * dz
+        rec_curl = float(sum((v.astype(np.float64) ** 2).sum() for v in cj.values()) * vol)
+        rec_cur = float((J.astype(np.float64) ** 2).sum() * vol)
+        rms = lambda v: float(np.sqrt(np.mean(np.asarray(v, dtype=np.float64) ** 2)))
+        target = idct2(self.k * dct2(self.Phi_above_bottom))
+        dphi0 = fd2_derivative(phi, dz, 0)[0]
+        ph = dct2(phi)
+        top_phi = idct2((3.0 * ph[-1] - 4.0 * ph[-2] + ph[-3]) / (2.0 * dz) + self.k * ph[-1])
+        ah = dst1(dst1(az[:, 1:-1, 1:-1], -1), -2)
+        ak_t = np.sqrt(np.clip(-self.lam_dirichlet(), 0.0, None)).astype(self.dtype)
+        top_az = idst1(idst1((3.0 * ah[-1] - 4.0 * ah[-2] + ah[-3]) / (2.0 * dz)
+                             + ak_t * ah[-1], -1), -2)
+        bc = np.stack([-d_dx(target, self.dx) + d_dy(self.Az_bottom, self.dy),
+                       -d_dy(target, self.dy) - d_dx(self.Az_bottom, self.dx)])
+        dp0 = d_dz(phi[:3], dz)[0]
+        bcx0 = -d_dx(dp0, self.dx) + d_dy(az[0], self.dy)
+        bcy0 = -d_dy(dp0, self.dy) - d_dx(az[0], self.dx)
+        return {
+            "curl_current_energy": float(curl_e), "current_energy": float(cur_e),
+            "objective": float(curl_e + self.soft_eps * cur_e),
+            "reconstructed_curl_current_energy": rec_curl,
+            "reconstructed_current_energy": rec_cur,
+            "bottom_phi_rms": rms(phi[0] - self.Phi_above_bottom),
+            "bottom_dphi_rms": rms(dphi0 - target),
+            "bottom_az_rms": rms(az[0] - self.Az_bottom),
+            "bottom_bz_rms": rms(B[2, 0] - self.Bz_obs),
+            "bottom_jz_rms": rms(J[2, 0] - self.Jz_T),
+            "bottom_bh_current_carrying_rms": rms(np.stack([bcx0, bcy0]) - bc),
+            "bottom_bh_observation_rms": rms(B[:2, 0] - np.stack([self.Bx_obs, self.By_obs])),
+            "top_phi_transparent_rms": rms(top_phi),
+            "top_az_transparent_rms": rms(top_az),
+        }
+
     def project_field(self, B, Az_in):
         """Caller-field start: the cfcf handoff of nfgs_solver.py:6080-6147 with
         (B, Az) given.  Phi_c = skip-bottom endpoint-Neumann Poisson of
@@ -1357,6 +1544,8 @@ class OracleNFGS:
         diags = []
         if start is None:
             B, J, phi, az = self.initial_state()
+        elif isinstance(start, str) and start == "soft_cfcf":
+            B, J, phi, az = self.soft_initial_state()
         else:
             phi, az, _, B, J = self.project_field(*start)
         s0, d0, e0 = self.sigma(B, J), self.divB(B), self.energy(B)
