@@ -583,6 +583,32 @@ int nfgs_k0_state_pass(const nfgs_plan* plan, const nfgs_boundary* bnd, int kind
                        void* work, void* stream);
 
 /* ------------------------------------------------------------------------
+ * soft_cfcf start (init_mode="soft_cfcf", nfgs_solver.py:6152-6611).
+ * ------------------------------------------------------------------------ */
+/* Bytes of the work buffer of nfgs_soft_start / nfgs_soft_diagnostics. */
+long nfgs_soft_workspace_bytes(const nfgs_plan* plan);
+/* The start's scalars (replaces _soft_cfcf_initial_state's modal solves and
+ * transforms, :6423-6470): every DCT-I mode of Phi_c and interior DST-I mode
+ * of Az gets the profile minimising the modal curl-current form with weight
+ * eps (soft_cfcf_epsilon) on the current and a diagonal ridge (:6322-6402),
+ * by a banded Cholesky factorisation per mode in FP64.  phi_out, az_out: N
+ * values each; plane 0 is the bottom potential / Az_bottom, the side walls of
+ * Az are zero and its top plane is the profile's (not zero).  out: 6 doubles,
+ * the Phi_c modal curl and current energies (DCT-I end weights applied), the
+ * count of Phi_c modes whose solve was not finite, then the same for Az. */
+int nfgs_soft_start(const nfgs_plan* plan, const nfgs_boundary* bnd, double eps, double ridge,
+                    void* phi_out, void* az_out, void* work, double* out, void* stream);
+/* Sums of squares behind the start's diagnostics (:6542-6611) for a state
+ * (phi, az) with its B and J (3*N each) and the observed B_x, B_y (ny*nx):
+ * out[0..10] = bottom Phi_c, bottom dPhi_c/dz against idct2(k dct2(Phi_above)),
+ * bottom Az, bottom B_z, bottom J_z, bottom current-carrying B_h, bottom
+ * observed B_h, the transparent top rows of Phi_c and Az (in space), |curl J|^2
+ * and |J|^2 over the volume. */
+int nfgs_soft_diagnostics(const nfgs_plan* plan, const nfgs_boundary* bnd, const void* phi,
+                          const void* az, const void* b, const void* j, const void* bx_obs,
+                          const void* by_obs, void* work, double* out, void* stream);
+
+/* ------------------------------------------------------------------------
  * Learned warm start.  No counterpart in the reference: its iterate() starts
  * only from gs_cfcf (nfgs_solver.py:6613-6645).  These are the kernels of a
  * small cosine-basis spectral neural operator that maps the gs_cfcf state

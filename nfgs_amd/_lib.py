@@ -153,6 +153,11 @@ def _declare(lib):
                                       P(K0Step), vp]),
         "nfgs_k0_state_pass": (C.c_int, [vp, P(Boundary), C.c_int, vp, vp, vp, vp, vp, vp, vp,
                                          vp]),
+        # soft_cfcf start (nfgs_soft_*)
+        "nfgs_soft_workspace_bytes": (C.c_long, [vp]),
+        "nfgs_soft_start": (C.c_int, [vp, P(Boundary), C.c_double, C.c_double, vp, vp, vp, vp,
+                                      vp]),
+        "nfgs_soft_diagnostics": (C.c_int, [vp, P(Boundary), vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         # learned start (nfgs_fno_*)
         "nfgs_fno_pointwise": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
                                          C.c_int, C.c_int, vp, vp]),

@@ -565,6 +565,11 @@ class NFGSSolverGPU:
         if self.Nx < 3 or self.Ny < 3:
             raise ValueError("the calculation grid needs at least 3 nodes per lateral axis")
         self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+        h = min(self.dx, self.dy, self.dz)
+        self.soft_cfcf_epsilon = 1.0e-4 / (h * h) if opts.soft_cfcf_epsilon is None \
+            else float(opts.soft_cfcf_epsilon)
+        self.soft_cfcf_mode_batch_size = opts.soft_cfcf_mode_batch_size
+        self.soft_cfcf_normal_ridge = opts.soft_cfcf_normal_ridge
         if opts.horizontal_zero_mode_policy in COUPLED_CLOSURES:
             if self.Nz < 4:
                 raise ValueError("coupled horizontal harmonic modes need Nz >= 4")
@@ -597,6 +602,7 @@ class NFGSSolverGPU:
         self.fov_slice_yx = None
         self._observed_shape_zyx = None
         self._source_zero_mode_history = []
+        self._dirty_top = set()  # slots whose Az top plane is not zero
         self._k0 = None  # coupled closure tables and per-step records
         self._k0_work = None
         self._slot_h = None
@@ -1078,6 +1084,10 @@ class NFGSSolverGPU:
         solver's current profile H_k0) and records its closure diagnostics
         under iteration number `step`."""
         phi, az = self._slots[slot]
+        if slot in self._dirty_top:
+            # a step leaves the top plane of Az alone: it must be zero
+            az[-1].zero_()
+            self._dirty_top.discard(slot)
         mode = L.POLOIDAL_TWO_STAGE if self.opts.poloidal_update_mode == "two_stage" \
             else L.POLOIDAL_DIRECT
         b_out = L.ptr(self._Bwork) if want_b else None
@@ -1207,6 +1217,8 @@ class NFGSSolverGPU:
         self._require_boundary()
         if mode == "field":
             B, J = self._init_from_field(init_field, init_az, init_current, want_fields=True)
+        elif mode == "soft_cfcf":
+            B, J = self._init_soft(want_fields=True)
         elif mode == "learned":
             self._init_learned(learned_start)
             B, J = self._state_fields(0)
@@ -1227,8 +1239,10 @@ class NFGSSolverGPU:
             raise ValueError("NFGS supports init_mode='gs_cfcf', 'cfcf', or 'soft_cfcf' "
                              "(and 'field' for a caller's starting field, 'learned' for the "
                              "learned start)")
-        if mode not in ("gs_cfcf", "field", "learned"):
+        if mode not in ("gs_cfcf", "field", "learned", "soft_cfcf"):
             _unsupported(f"init_mode={init_mode!r}")
+        if mode == "soft_cfcf" and self.Nz < 5:
+            raise ValueError("soft_cfcf requires Nz >= 5")
         return mode
 
     def _gs_cfcf_diagnostics(self) -> dict:
@@ -1299,6 +1313,74 @@ class NFGSSolverGPU:
         diag["gaussian_separation"] = self.gaussian_separation_diagnostics
         diag["coupled_k0"] = self._k0_diagnostics
         self._init_diagnostics = diag
+        if want_fields:
+            return B, J
+        del B, J
+        return None, None
+
+    def _init_soft(self, want_fields=False):
+        """Slot 0 from the soft_cfcf start (nfgs_solver.py:6423-6484): every
+        horizontal mode of Phi_c and Az from its banded modal least-squares
+        solve (nfgs_soft_start), then the state pass of that state (J_par and
+        the monitors) and the start's diagnostics (nfgs_soft_diagnostics,
+        :6486-6611).  Az of this state has a nonzero top plane (the modal
+        profiles meet a transparent top row), so slot 0 is marked for the
+        next step that writes it."""
+        lib, plan, s = self._lib, self._plan, self._stream
+        Nz, Ny, Nx = self.Nz, self.Ny, self.Nx
+        phi, az = self._slots[0]
+        work = torch.empty(int(lib.nfgs_soft_workspace_bytes(plan)), device=self.device,
+                           dtype=torch.uint8)
+        # 6 modal sums (nfgs_soft_start), then 11 diagnostic sums
+        sums = torch.zeros(17, device=self.device, dtype=torch.float64)
+        L.check(lib.nfgs_soft_start(plan, L.C.byref(self._bnd), self.soft_cfcf_epsilon,
+                                    self.soft_cfcf_normal_ridge, L.ptr(phi), L.ptr(az),
+                                    L.ptr(work), L.ptr(sums), s), "soft_cfcf start")
+        self._dirty_top.add(0)
+        if self._slot_h is not None:
+            self._slot_h[0].copy_(self._H_k0)
+        B = self._empty(3, Nz, Ny, Nx)
+        J = self._empty(3, Nz, Ny, Nx)
+        self._pass(phi, az, B, self._ws_t["jp"], J, self._partials, self._h_of(0))
+        L.check(lib.nfgs_reduce(plan, L.ptr(self._partials), L.ptr(self._metrics), s), "reduce")
+        L.check(lib.nfgs_soft_diagnostics(plan, L.C.byref(self._bnd), L.ptr(phi), L.ptr(az),
+                                          L.ptr(B), L.ptr(J), L.ptr(self.Bx_obs),
+                                          L.ptr(self.By_obs), L.ptr(work), L.ptr(sums) + 6 * 8,
+                                          s), "soft_cfcf diagnostics")
+        del work
+        v = sums.cpu().numpy()
+        if v[2] > 0 or v[5] > 0:
+            raise RuntimeError("soft_cfcf modal normal solve produced non-finite values")
+        f = self.dx * self.dy * self.dz / (4.0 * (Nx - 1) * (Ny - 1))
+        curl_e = f * (v[0] + v[3])
+        cur_e = f * (v[1] + v[4])
+        d = v[6:]
+        pl, ip = float(Ny * Nx), float((Ny - 2) * (Nx - 2))
+        vol = self.dx * self.dy * self.dz
+        soft = {
+            "curl_current_energy": float(curl_e), "current_energy": float(cur_e),
+            "objective": float(curl_e + self.soft_cfcf_epsilon * cur_e),
+            "reconstructed_curl_current_energy": float(d[9] * vol),
+            "reconstructed_current_energy": float(d[10] * vol),
+            "bottom_phi_rms": math.sqrt(d[0] / pl),
+            "bottom_dphi_rms": math.sqrt(d[1] / pl),
+            "bottom_az_rms": math.sqrt(d[2] / pl),
+            "bottom_bz_rms": math.sqrt(d[3] / pl),
+            "bottom_jz_rms": math.sqrt(d[4] / pl),
+            "bottom_bh_current_carrying_rms": math.sqrt(d[5] / (2.0 * pl)),
+            "bottom_bh_observation_rms": math.sqrt(d[6] / (2.0 * pl)),
+            "top_phi_transparent_rms": math.sqrt(d[7] / pl),
+            "top_az_transparent_rms": math.sqrt(d[8] / ip),
+        }
+        self._init_diagnostics = {
+            "init_mode": "soft_cfcf", "mfr_count": 0,
+            "soft_cfcf_epsilon": self.soft_cfcf_epsilon,
+            "soft_cfcf_mode_batch_size": self.soft_cfcf_mode_batch_size,
+            "soft_cfcf_normal_ridge": self.soft_cfcf_normal_ridge,
+            "soft_cfcf": soft,
+            "gaussian_separation": self.gaussian_separation_diagnostics,
+            "coupled_k0": self._k0_diagnostics,
+        }
         if want_fields:
             return B, J
         del B, J
@@ -1803,6 +1885,11 @@ class NFGSSolverGPU:
                 self._state_fields(0, B_out=self._Bwork)
         elif mode == "learned":
             self._init_learned(learned_start)
+            sigma0, divB0, E0, _ = self._read_metrics()
+            if diag is not None:
+                self._state_fields(0, B_out=self._Bwork)
+        elif mode == "soft_cfcf":
+            self._init_soft()
             sigma0, divB0, E0, _ = self._read_metrics()
             if diag is not None:
                 self._state_fields(0, B_out=self._Bwork)

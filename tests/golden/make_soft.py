@@ -25,6 +25,8 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 OUT = HERE / "soft"
 sys.path.insert(0, str(HERE))
+sys.path.insert(0, str(HERE.parent))
+sys.path.insert(0, str(HERE.parent.parent))
 from make_golden import boundary, t2n  # noqa: E402  (the same seeded boundary maps)
 
 CASES = [
@@ -98,6 +100,40 @@ def run_case(ref, torch, name, seed, obs, rho, dtype_name, extra):
     print(name, s.Nz, s.Ny, s.Nx, hist["best_iter"], hist["stop_reason"], hist["best_sigma"])
 
 
+def run_digest_case(ref, torch, dtype_name):
+    """The start at a 64^3 observed multipole boundary (66 x 66 x 65
+    calculation grid), held as digests of its volumes (tests/digest.py)."""
+    from digest import digest, pack  # noqa: E402
+    from nfgs_amd.boundary import multipole_boundary  # noqa: E402  (seeded input maps)
+
+    dtype = getattr(torch, dtype_name)
+    np_dt = np.float64 if dtype_name == "float64" else np.float32
+    bx, by, bz, _ = multipole_boundary(64, 64, seed=7, sigma=(4.0, 10.0))
+    bx, by, bz = (a.astype(np_dt) for a in (bx, by, bz))
+    kw = dict(
+        device="cpu", dtype=dtype, lateral_wall="paper_bc1", pin_potential_boundaries=False,
+        buffer_thickness=0, padded_buffer_zone=True, pad_calculation_domain=True,
+        calculation_domain_mode="mudpack", bottom_divergence_robin_weight=0.5,
+        mean_bz_policy="uniform_open", d0_zero_mode_policy="project",
+        source_zero_mode_policy="project", harmonic_residual_policy="diagnose",
+        poloidal_update_mode="direct", vertical_current_operator="staggered_face",
+    )
+    s = ref.NFGSSolverGPU(64, 64, 64, 1.0, 1.0, 1.0, **kw)
+    s.set_bottom_vector_boundary(bx, by, bz, zero_net_current=False, smooth_passes=0, ijzbcor=3)
+    B, J, phi, az, _ = s.compute_initial_conditions(init_mode="soft_cfcf")
+    out = {"in_bx": bx, "in_by": by, "in_bz": bz}
+    for key, t in (("B", B), ("J", J), ("Phi_c", phi), ("Az", az)):
+        out.update(pack(key, digest(t2n(t))))
+    meta = {"case": dict(name=f"soft64_{dtype_name}", observed=[64, 64, 64], rho=0.5,
+                         dtype=dtype_name, extra={}),
+            "soft": dict(s._init_diagnostics["soft_cfcf"]),
+            "init_sigma": float(s.compute_sigma(B, J))}
+    out["meta_json"] = np.array(json.dumps(meta))
+    OUT.mkdir(exist_ok=True)
+    np.savez_compressed(OUT / "digest" / f"soft64_{dtype_name}.npz", **out)
+    print("digest", dtype_name, meta["init_sigma"])
+
+
 def main():
     only = set(sys.argv[2:])
     ref_root = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
@@ -111,6 +147,11 @@ def main():
             if only and case[0] not in only:
                 continue
             run_case(ref, torch, *case)
+        (OUT / "digest").mkdir(parents=True, exist_ok=True)
+        for dtype_name in ("float64", "float32"):
+            if only and f"soft64_{dtype_name}" not in only:
+                continue
+            run_digest_case(ref, torch, dtype_name)
 
 
 if __name__ == "__main__":
