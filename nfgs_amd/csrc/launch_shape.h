@@ -32,20 +32,30 @@ inline bool zchunks_fit(int nz, int count) {
   return count == 1 || last >= 2;
 }
 
+// Workgroups of the fused state pass resident on one CU at a time (36 KB of
+// LDS and about 120 VGPRs per lane allow four), the residency the cost model
+// plans for.
+constexpr int kResidentBlocks = 4;
+
 // Planes per chunk of the cost model.  A chunk re-stages about four planes
-// (its Phi ring and the face below it), and the blocks of a launch share the
-// CUs evenly, so the launch takes about ceil(blocks / CUs) * (chunk + 4)
-// plane-steps on the busiest CU; the chunk count minimises that, keeping the
-// last chunk at least two planes long.
+// (its Phi ring and the face below it), and a CU runs kResidentBlocks
+// workgroups side by side, so the launch takes about
+// ceil(blocks / (CUs * kResidentBlocks)) rounds of (chunk + 4) plane-steps;
+// the chunk count minimises that, keeping the last chunk at least two planes
+// long.  Of equal costs the larger count wins (its last round is the fuller
+// one).  Measured against every count on MI355X: the model's choice is the
+// fastest at 130x130x129, 386x386x129 and 514x514x257 and within 2% of it at
+// 258x258x257.
 inline int cost_model_chunk(int nx, int ny, int nz, long cus) {
   const long xy = tiles_xy(nx, ny);
+  const long slots = cus * kResidentBlocks;
   long best = -1;
   int kc = nz;
   for (int nch = 1; nch <= (nz + 3) / 4; ++nch) {
     const int c = (nz + nch - 1) / nch;
     if (nch > 1 && nz - (nch - 1) * c < 2) continue;
-    const long cost = ((xy * nch + cus - 1) / cus) * (c + 4);
-    if (best < 0 || cost < best) {
+    const long cost = ((xy * nch + slots - 1) / slots) * (c + 4);
+    if (best < 0 || cost <= best) {
       best = cost;
       kc = c;
     }
