@@ -6,9 +6,9 @@ The two knobs (nfgs_plan_set_tuning) change how the work is split over
 workgroups, not what is computed: the transform's results do not depend on
 its launch shape at all, and the z-chunk count changes only the order in
 which the monitor partials are summed (sigma, divB and E move in the last
-bits).  The built-in choices are a cost model (z-chunks) and the occupancy
-limit (transform); on grids far from the 258^2 lateral size the model was
-fitted on, the measured best differs.
+bits).  The built-in choices are a cost model (z-chunks, planned for four
+resident workgroups per CU) and the occupancy limit (transform); the
+measured best can still differ by a few percent.
 """
 from __future__ import annotations
 

@@ -1104,9 +1104,10 @@ class OracleNFGS:
         """The constant DCT-I mode of every plane (nfgs_solver.py:4466-4468)."""
         return dct2(f)[:, 0, 0] / (4.0 * (self.Nx - 1) * (self.Ny - 1))
 
-    def k0_update(self, jp, phi, az):
+    def k0_update(self, jp, phi, az, step_mu=1.0):
         """The closure's profile from the step's parallel current and the new
-        scalars (nfgs_solver.py:5952-6067, :6760-6791); sets self.H."""
+        scalars (nfgs_solver.py:5952-6067, :6760-6791); sets self.H.  The
+        relaxation is k0_update_mu, or the step's mu without it (:6776)."""
         bcz = lap_xy(phi, self.dx, self.dy)
         _, jpt, _ = reconstruct_pt(phi, az, bcz, self.B_background, self.Bz_obs, self.Jz_T,
                                    self.dx, self.dy, self.dz)
@@ -1137,7 +1138,7 @@ class OracleNFGS:
             profile_max_abs=float(np.abs(prof).max()),
             profile_top=[float(v) for v in prof[:, -1]],
             desired_mean_current_rms=float(np.sqrt(np.mean(want ** 2))))
-        mu = 1.0 if self.k0_mu is None else self.k0_mu
+        mu = float(step_mu) if self.k0_mu is None else self.k0_mu
         new = prof if mu >= 1.0 else mu * prof + (1.0 - mu) * self.H.astype(np.float64)
         self.H = new.astype(self.dtype)
         self.H_unrelaxed = prof
@@ -1413,17 +1414,26 @@ class OracleNFGS:
         Bp, Jp = self.reconstruct(phi, az, lap_xy(phi, self.dx, self.dy))
         return phi, az, mean, Bp, Jp
 
-    def sources(self, B, J):
-        """Parallel current, bottom row, and both Poisson sources (nfgs_solver.py:6694-6745)."""
+    def sources(self, B, J, polarity="both"):
+        """Parallel current, bottom row, and both Poisson sources
+        (nfgs_solver.py:6694-6745).  polarity "positive" ("negative") keeps
+        the bottom row only where the observed B_z is positive (negative)
+        (:6696-6701)."""
         jp = self.parallel_current(J, B)
         jp[:, 0] = J[:, 0]
+        if polarity != "both":
+            keep = (self.Bz_obs > 0) if polarity == "positive" else (self.Bz_obs < 0)
+            jp[:, 0] = J[:, 0] * keep.astype(self.dtype)
         s_bcz = d_dy(jp[0], self.dy) - d_dx(jp[1], self.dx)
         s_jz = lap_xy(jp[2], self.dx, self.dy) - d2_dxdz(jp[0], self.dx, self.dz) \
             - d2_dydz(jp[1], self.dy, self.dz)
         return jp, s_bcz, s_jz
 
-    def step(self, B, J, Phi_c, Az):
-        jp, s_bcz, s_jz = self.sources(B, J)
+    def step(self, B, J, Phi_c, Az, mu=1.0, polarity="both"):
+        """One map of the state (nfgs_solver.py:6647-6805).  With mu < 1 the
+        new scalars are mu times the solves' plus (1 - mu) times the passed
+        ones, before the fixed planes are set again (:6778-6790)."""
+        jp, s_bcz, s_jz = self.sources(B, J, polarity)
         phi = self.solve_phi_c(s_bcz)
         az = self.solve_az(s_jz)
         phi[0] = self.Phi_above_bottom
@@ -1431,7 +1441,14 @@ class OracleNFGS:
         az[-1] = 0.0
         self._az_lateral_zero(az)
         if self.k0_policy != "frozen_linear_harmonic":
-            self.k0_update(jp, phi, az)
+            self.k0_update(jp, phi, az, mu)
+        if mu < 1.0:
+            phi = mu * phi + (1.0 - mu) * Phi_c
+            az = mu * az + (1.0 - mu) * Az
+            phi[0] = self.Phi_above_bottom
+            az[0] = self.Az_bottom
+            az[-1] = 0.0
+            self._az_lateral_zero(az)
         B2, J2 = self.reconstruct(phi, az, lap_xy(phi, self.dx, self.dy))
         return B2, J2, phi, az
 
@@ -1527,13 +1544,22 @@ class OracleNFGS:
     def iterate(self, max_iter=200, tol=1e-5, patience=1, sigma_improvement_tol=1e-5,
                 callback=None, start=None, record_diagnostics=False, stop_mode="sigma_best",
                 fixed_point_update_tol=None, fixed_point_pde_tol=1.0e-8,
-                fixed_point_observable_tol=None, fixed_point_patience=10):
+                fixed_point_observable_tol=None, fixed_point_patience=10, mu=1.0,
+                adaptive_mu=False, mu_min=0.02, mu_max=1.0, mu_backoff=0.5, mu_growth=1.1,
+                growth_streak=5, divergence_factor=2.0, max_rollbacks=12, polarity="both"):
         """start=(B, Az): begin from project_field(B, Az) instead of gs_cfcf.
         record_diagnostics: the per-step diagnostics as history lists;
         stop_mode="fixed_point" (needs them) stops after fixed_point_patience
         consecutive steps with the map residual, the PDE residual and the
         sigma/energy drift within their tolerances and selects the final
-        iterate (nfgs_solver.py:3565-3582, :3712-3733, :3766-3776, :3816)."""
+        iterate (nfgs_solver.py:3565-3582, :3712-3733, :3766-3776, :3816).
+        mu < 1 relaxes every step; adaptive_mu rolls a step whose sigma is
+        not finite or above divergence_factor times the best back to the best
+        state with mu backed off, grows mu after growth_streak new bests, and
+        stops after max_rollbacks (:3658-3762); polarity masks the bottom
+        parallel current."""
+        if record_diagnostics and (mu != 1.0 or adaptive_mu or polarity != "both"):
+            raise ValueError("per-step diagnostics are restated at mu = 1 only")
         if stop_mode == "fixed_point" and not record_diagnostics:
             raise ValueError("stop_mode='fixed_point' requires record_iteration_diagnostics=True")
         upd = float(tol if fixed_point_update_tol is None else fixed_point_update_tol)
@@ -1562,13 +1588,32 @@ class OracleNFGS:
         last_imp = None
         reason = "max_iter"
         final_iter = 0
+        mu_cur = float(mu)
+        rollbacks = improve = 0
         for n in range(1, max_iter + 1):
             if record_diagnostics:
-                B, J, phi, az, dg = self.step_diagnostics(B, J, phi, az)
+                Bn, Jn, pn, an, dg = self.step_diagnostics(B, J, phi, az)
                 diags.append(dg)
             else:
-                B, J, phi, az = self.step(B, J, phi, az)
-            s = self.sigma(B, J)
+                Bn, Jn, pn, an = self.step(B, J, phi, az, mu_cur, polarity)
+            s = self.sigma(Bn, Jn)
+            if adaptive_mu and (not np.isfinite(s) or s > divergence_factor * max(best_sigma, 1e-30)):
+                rollbacks += 1
+                mu_cur = max(mu_cur * mu_backoff, mu_min)
+                B, J, phi, az = (a.copy() for a in best)
+                self.H = best_h.copy()
+                improve = 0
+                for key, v in (("sigma", s), ("divB", np.nan), ("energy", np.nan), ("mu", mu_cur),
+                               ("rollback", 1), ("sigma_relative_change", np.nan),
+                               ("energy_relative_change", np.nan),
+                               ("fixed_point_criteria_met", 0)):
+                    h[key].append(v)
+                streak = 0
+                if rollbacks >= max_rollbacks:
+                    reason = "max_rollbacks"
+                    break
+                continue
+            B, J, phi, az = Bn, Jn, pn, an
             d, e = self.divB(B), self.energy(B)
             pe = h["energy"][-1] if h["energy"] else e0
             src = abs(s - prev) / max(abs(prev), 1e-30)
@@ -1576,7 +1621,7 @@ class OracleNFGS:
             h["sigma"].append(s)
             h["divB"].append(d)
             h["energy"].append(e)
-            h["mu"].append(1.0)
+            h["mu"].append(mu_cur)
             h["rollback"].append(0)
             h["sigma_relative_change"].append(src)
             h["energy_relative_change"].append(erc)
@@ -1603,8 +1648,13 @@ class OracleNFGS:
                 best = (B.copy(), J.copy(), phi.copy(), az.copy())
                 best_h = self.H.copy()
                 stall = 0
+                improve += 1
+                if adaptive_mu and improve >= growth_streak:
+                    mu_cur = min(mu_cur * mu_growth, mu_max)
+                    improve = 0
             else:
                 stall += 1
+                improve = 0
             if stop_mode == "fixed_point":
                 if streak >= int(fixed_point_patience):
                     reason = "fixed_point"
@@ -1633,7 +1683,7 @@ class OracleNFGS:
         # the selected state's harmonic profile (nfgs_solver.py:6990-7004)
         self.H = best_h
         h.update(fixed_point_converged=fp_converged, fixed_point_streak=streak)
-        h.update(best_sigma=best_sigma, best_iter=best_iter, rollbacks=0, stop_reason=reason,
+        h.update(best_sigma=best_sigma, best_iter=best_iter, rollbacks=rollbacks, stop_reason=reason,
                  stop_mode=stop_mode,
                  selected_state="final_accepted_iterate" if stop_mode == "fixed_point"
                  else "best_sigma", final_iter=final_iter,
