@@ -572,6 +572,22 @@ int nfgs_k0_update(const nfgs_plan* plan, const nfgs_k0_step* k0, const void* jp
 int nfgs_advance_k0(const nfgs_plan* plan, const nfgs_boundary* bnd, const nfgs_workspace* ws,
                     void* phi_out, void* az_out, void* b_out, int zero_source, int poloidal_mode,
                     const nfgs_k0_step* k0, void* stream);
+/* Polarity masks of the bottom parallel current (nfgs_solver.py:6694-6701). */
+enum { NFGS_POLARITY_BOTH = 0, NFGS_POLARITY_POSITIVE = 1, NFGS_POLARITY_NEGATIVE = 2 };
+/* One step from the state (phi_old, az_old) with relaxation and the polarity
+ * mask (iteration_step(mu, polarity_mask), nfgs_solver.py:6694-6701,
+ * :6776-6790): the bottom row of ws->jp (the old state's J_par) is kept only
+ * where the observed B_z is positive (NFGS_POLARITY_POSITIVE) or negative
+ * (NFGS_POLARITY_NEGATIVE); the sources and solves follow; with k0 (NULL for
+ * the frozen closure) the closure update; then, away from the fixed planes,
+ * phi_out = mu phi* + (1 - mu) phi_old and az_out = mu az* + (1 - mu) az_old
+ * (0 < mu <= 1), and the state pass of the relaxed state.  The fixed planes
+ * of phi_out and az_out (bottom planes, Az's top plane and side walls) must
+ * hold their values, as the state slots of nfgs_advance_ex do. */
+int nfgs_advance_relaxed(const nfgs_plan* plan, const nfgs_boundary* bnd,
+                         const nfgs_workspace* ws, const void* phi_old, const void* az_old,
+                         double mu, int polarity, void* phi_out, void* az_out, void* b_out,
+                         int poloidal_mode, const nfgs_k0_step* k0, void* stream);
 /* nfgs_state_pass of a closure's state (bnd->h_profile: the state's
  * profile).  The coupled closure's current is the frozen rule's; the spline
  * closure's (nfgs_solver.py:5919-5941) has J_z the cell projection of the

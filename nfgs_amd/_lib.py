@@ -40,6 +40,8 @@ LATERAL_EXACT, LATERAL_BF16X3 = 0, 1
 # coupled horizontal-harmonic closures (NFGS_K0_*)
 K0_COUPLED, K0_SPLINE = 1, 2
 K0_STAGGERED, K0_LEGACY = 0, 1
+# polarity masks of the bottom parallel current (NFGS_POLARITY_*)
+POLARITY = {"both": 0, "positive": 1, "negative": 2}
 # learned start (NFGS_FNO_*)
 NFGS_FNO_IDENTITY, NFGS_FNO_GELU = 0, 1
 NFGS_FNO_PROJECT, NFGS_FNO_EXPAND = 0, 1
@@ -153,6 +155,8 @@ def _declare(lib):
                                       P(K0Step), vp]),
         "nfgs_k0_state_pass": (C.c_int, [vp, P(Boundary), C.c_int, vp, vp, vp, vp, vp, vp, vp,
                                          vp]),
+        "nfgs_advance_relaxed": (C.c_int, [vp, P(Boundary), P(Workspace), vp, vp, C.c_double,
+                                           C.c_int, vp, vp, vp, C.c_int, vp, vp]),
         # soft_cfcf start (nfgs_soft_*)
         "nfgs_soft_workspace_bytes": (C.c_long, [vp]),
         "nfgs_soft_start": (C.c_int, [vp, P(Boundary), C.c_double, C.c_double, vp, vp, vp, vp,

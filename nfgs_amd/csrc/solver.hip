@@ -129,6 +129,98 @@ extern "C" int nfgs_advance_k0(const nfgs_plan* p, const nfgs_boundary* bnd,
   return nfgs_reduce(p, ws->partials, ws->metrics, stream);
 }
 
+namespace {
+
+// Keep the bottom row of J_par where the observed B_z has the chosen sign
+// (nfgs_solver.py:6694-6701): J_par[:, 0] = J[:, 0] * mask.
+template <typename T>
+__global__ __launch_bounds__(256) void polarity_mask_kernel(T* __restrict__ jp,
+                                                            const T* __restrict__ bz_obs,
+                                                            long plane, long N, int positive) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= plane) return;
+  const T b = bz_obs[i];
+  const T keep = (positive ? b > T(0) : b < T(0)) ? T(1) : T(0);
+  jp[i] = jp[i] * keep;
+  jp[N + i] = jp[N + i] * keep;
+  jp[2 * N + i] = jp[2 * N + i] * keep;
+}
+
+// phi = mu phi + (1 - mu) phi_old on planes 1..nz-1; az likewise on the
+// interior points of planes 1..nz-2 (nfgs_solver.py:6778-6790).  The fixed
+// planes are left as they are.
+template <typename T>
+__global__ __launch_bounds__(256) void relax_kernel(T* __restrict__ phi,
+                                                    const T* __restrict__ phi_old,
+                                                    T* __restrict__ az,
+                                                    const T* __restrict__ az_old, int nx, int ny,
+                                                    int nz, T mu, T rest) {
+#pragma clang fp contract(off)
+  const long pl = (long)nx * ny;
+  const long i = pl + (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= pl * nz) return;
+  phi[i] = mu * phi[i] + rest * phi_old[i];
+  const int z = (int)(i / pl);
+  const long r = i - (long)z * pl;
+  const int y = (int)(r / nx), x = (int)(r - (long)y * nx);
+  if (z < nz - 1 && x > 0 && x < nx - 1 && y > 0 && y < ny - 1)
+    az[i] = mu * az[i] + rest * az_old[i];
+}
+
+}  // namespace
+
+extern "C" int nfgs_advance_relaxed(const nfgs_plan* p, const nfgs_boundary* bnd,
+                                    const nfgs_workspace* ws, const void* phi_old,
+                                    const void* az_old, double mu, int polarity, void* phi_out,
+                                    void* az_out, void* b_out, int poloidal_mode,
+                                    const nfgs_k0_step* k0, void* stream) {
+  using namespace nfgs;
+  NFGS_REQUIRE(p && bnd && ws && phi_old && az_old && phi_out && az_out, NFGS_ERR_ARG,
+               "advance_relaxed: null argument");
+  NFGS_REQUIRE(mu > 0.0 && mu <= 1.0, NFGS_ERR_ARG, "advance_relaxed: mu must be in (0, 1]");
+  NFGS_REQUIRE(polarity == NFGS_POLARITY_BOTH || polarity == NFGS_POLARITY_POSITIVE ||
+                   polarity == NFGS_POLARITY_NEGATIVE,
+               NFGS_ERR_ARG, "advance_relaxed: unknown polarity");
+  NFGS_REQUIRE(!k0 || k0->h_new == bnd->h_profile, NFGS_ERR_ARG,
+               "advance_relaxed: the boundary's h_profile must be the step's new profile");
+  NFGS_REQUIRE(bnd->bz_obs && ws->jp, NFGS_ERR_STATE, "advance_relaxed: boundary is not set");
+  hipStream_t s = as_stream(stream);
+  const bool f32 = p->d.dtype == NFGS_F32;
+  if (polarity != NFGS_POLARITY_BOTH) {
+    const dim3 g((unsigned)((p->plane + 255) / 256));
+    const int pos = polarity == NFGS_POLARITY_POSITIVE;
+    if (f32)
+      hipLaunchKernelGGL(polarity_mask_kernel<float>, g, dim3(256), 0, s,
+                         static_cast<float*>(ws->jp), static_cast<const float*>(bnd->bz_obs),
+                         p->plane, p->N, pos);
+    else
+      hipLaunchKernelGGL(polarity_mask_kernel<double>, g, dim3(256), 0, s,
+                         static_cast<double*>(ws->jp), static_cast<const double*>(bnd->bz_obs),
+                         p->plane, p->N, pos);
+    NFGS_LAUNCH_CHECK();
+  }
+  NFGS_TRY(advance_solves(p, bnd, ws, phi_out, az_out, 0, poloidal_mode, stream));
+  if (k0) NFGS_TRY(nfgs_k0_update(p, k0, ws->jp, phi_out, az_out, stream));
+  if (mu < 1.0) {
+    const long n = p->N - p->plane;
+    const dim3 g((unsigned)((n + 255) / 256));
+    if (f32)
+      hipLaunchKernelGGL(relax_kernel<float>, g, dim3(256), 0, s, static_cast<float*>(phi_out),
+                         static_cast<const float*>(phi_old), static_cast<float*>(az_out),
+                         static_cast<const float*>(az_old), p->d.nx, p->d.ny, p->d.nz,
+                         (float)mu, (float)(1.0 - mu));
+    else
+      hipLaunchKernelGGL(relax_kernel<double>, g, dim3(256), 0, s, static_cast<double*>(phi_out),
+                         static_cast<const double*>(phi_old), static_cast<double*>(az_out),
+                         static_cast<const double*>(az_old), p->d.nx, p->d.ny, p->d.nz, mu,
+                         1.0 - mu);
+    NFGS_LAUNCH_CHECK();
+  }
+  NFGS_TRY(closure_state_pass(p, bnd, k0 ? k0->kind : 0, phi_out, az_out, b_out, ws->jp,
+                              nullptr, ws->partials, ws->s_bcz, ws->hat0, s));
+  return nfgs_reduce(p, ws->partials, ws->metrics, stream);
+}
+
 extern "C" int nfgs_advance(const nfgs_plan* p, const nfgs_boundary* bnd,
                             const nfgs_workspace* ws, void* phi_out, void* az_out, void* b_out,
                             int zero_source, void* stream) {

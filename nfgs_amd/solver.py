@@ -270,8 +270,19 @@ class _Driver:
         self.fp_streak = 0
         self.fp_converged = False
         self.map_residual = self.pde_residual = None
+        self.rollbacks = 0
 
-    def record(self, n, sigma, divB, energy, diagnostics=None):
+    def rollback(self, sigma, mu):
+        """History of a step that adaptive mu rolled back (:3664-3691): its
+        sigma, the backed-off mu, NaN for the rest; the counters stay."""
+        self.rollbacks += 1
+        for key, v in (("sigma", sigma), ("divB", math.nan), ("energy", math.nan), ("mu", mu),
+                       ("rollback", 1), ("sigma_relative_change", math.nan),
+                       ("energy_relative_change", math.nan), ("fixed_point_criteria_met", 0)):
+            self.history[key].append(v)
+        self.fp_streak = 0
+
+    def record(self, n, sigma, divB, energy, diagnostics=None, mu=1.0):
         """History of step n and its fixed-point criteria (:3697-3734)."""
         h = self.history
         pe = h["energy"][-1] if h["energy"] else self.E0
@@ -280,7 +291,7 @@ class _Driver:
         h["sigma"].append(sigma)
         h["divB"].append(divB)
         h["energy"].append(energy)
-        h["mu"].append(1.0)
+        h["mu"].append(float(mu))
         h["rollback"].append(0)
         h["sigma_relative_change"].append(src)
         h["energy_relative_change"].append(erc)
@@ -347,7 +358,7 @@ class _Driver:
         return False
 
     def summary(self) -> dict:
-        return dict(best_sigma=self.best_sigma, best_iter=self.best_iter, rollbacks=0,
+        return dict(best_sigma=self.best_sigma, best_iter=self.best_iter, rollbacks=self.rollbacks,
                     stop_reason=self.reason, stop_mode=self.stop_mode,
                     selected_state=("final_accepted_iterate" if self.stop_mode == "fixed_point"
                                     else "best_sigma"),
@@ -1077,12 +1088,15 @@ class NFGSSolverGPU:
             raise RuntimeError("NFGS boundary is not set. Call set_bottom_vector_boundary() first.")
 
     def _advance(self, slot: int, zero_source: bool, want_b: bool = False,
-                 src: Optional[int] = None, step: int = 0):
+                 src: Optional[int] = None, step: int = 0, mu: float = 1.0,
+                 polarity: str = "both", old=None):
         """One iteration into `slot`; B of the new state lands in _Bwork only
         when want_b (the iteration itself never needs B in memory).  With a
         coupled closure the step starts from slot `src`'s profile (None: the
         solver's current profile H_k0) and records its closure diagnostics
-        under iteration number `step`."""
+        under iteration number `step`.  mu < 1 relaxes the new scalars
+        towards the old ones (slot `src`, or the pair `old`), and polarity
+        masks the bottom parallel current (nfgs_advance_relaxed)."""
         phi, az = self._slots[slot]
         if slot in self._dirty_top:
             # a step leaves the top plane of Az alone: it must be zero
@@ -1091,6 +1105,9 @@ class NFGSSolverGPU:
         mode = L.POLOIDAL_TWO_STAGE if self.opts.poloidal_update_mode == "two_stage" \
             else L.POLOIDAL_DIRECT
         b_out = L.ptr(self._Bwork) if want_b else None
+        if not zero_source and (float(mu) != 1.0 or polarity != "both"):
+            self._advance_relaxed(slot, b_out, mode, src, step, float(mu), polarity, old)
+            return
         if self._k0 is None:
             L.check(self._lib.nfgs_advance_ex(self._plan, L.C.byref(self._bnd),
                                               L.C.byref(self._ws), L.ptr(phi), L.ptr(az), b_out,
@@ -1108,6 +1125,42 @@ class NFGSSolverGPU:
                                           L.C.byref(self._ws), L.ptr(phi), L.ptr(az), b_out,
                                           int(zero_source), mode, L.C.byref(rec), self._stream),
                 "iteration")
+
+    def _k0_record_for(self, slot, src, step, mu):
+        """The closure record of a step into `slot` from slot `src` (None:
+        the solver's current profile); the relaxation is k0_update_mu, or the
+        step's mu without it (nfgs_solver.py:6776)."""
+        k = self._k0
+        h_old = self._H_k0 if src is None else self._slot_h[src]
+        if step < 0 or step >= k["res"].numel():
+            raise IndexError("closure record index out of range")
+        k_mu = k["mu"] if self.opts.k0_update_mu is not None else mu
+        return L.K0Step(k["kind"], k["disc"], k_mu, L.ptr(k["tables"]),
+                        L.ptr(self._H_k0_bottom_target), L.ptr(h_old),
+                        L.ptr(self._slot_h[slot]), L.ptr(k["work"]),
+                        L.ptr(k["diag"][step & 1]), L.ptr(k["res"]) + 8 * step)
+
+    def _advance_relaxed(self, slot, b_out, mode, src, step, mu, polarity, old):
+        if not self.opts.relax_scalars and mu < 1.0:
+            _unsupported("mu < 1 with relax_scalars=False (the blend of B and J)")
+        phi, az = self._slots[slot]
+        if old is not None:
+            phi_old, az_old = old
+        elif src is not None:
+            phi_old, az_old = self._slots[src]
+        elif mu == 1.0:
+            phi_old, az_old = phi, az  # not read at mu = 1
+        else:
+            raise ValueError("a relaxed step needs the state it starts from")
+        rec = None
+        bnd = self._bnd
+        if self._k0 is not None:
+            rec = self._k0_record_for(slot, src if old is None else None, step, mu)
+            bnd = self._bnd_for(self._slot_h[slot])
+        L.check(self._lib.nfgs_advance_relaxed(
+            self._plan, L.C.byref(bnd), L.C.byref(self._ws), L.ptr(phi_old), L.ptr(az_old), mu,
+            L.POLARITY[polarity], L.ptr(phi), L.ptr(az), b_out, mode,
+            L.C.addressof(rec) if rec is not None else None, self._stream), "relaxed iteration")
 
     def _h_of(self, slot):
         """Profile of a slot index under a coupled closure (else None)."""
@@ -1462,20 +1515,23 @@ class NFGSSolverGPU:
         return out
 
     def iteration_step(self, B, J, Phi_c, Az, B_c=None, mu=1.0, polarity_mask: str = "both"):
-        """One map of the state (nfgs_solver.py:6647-6935) at mu = 1.
+        """One map of the state (nfgs_solver.py:6647-6935).
 
         As in the reference, the parallel current is formed from B and the
         centre Jz = J[2], and its bottom plane is J[:, 0]
-        (nfgs_face_pass_j); Phi_c and Az of the argument are not read at
-        mu = 1.  B_c is not used at mu = 1 and is returned as None.
+        (nfgs_face_pass_j), kept only where the observed B_z has the
+        polarity_mask's sign.  With mu < 1 the new scalars are relaxed
+        towards the passed Phi_c and Az (relax_scalars=True); at mu = 1 they
+        are not read.  B_c is not used and is returned as None.
         """
         self._require_boundary()
-        if float(mu) != 1.0:
-            _unsupported("mu != 1")
-        if polarity_mask != "both":
-            if polarity_mask in ("positive", "negative"):
-                _unsupported(f"polarity_mask={polarity_mask!r}")
+        mu = float(mu)
+        if polarity_mask not in ("positive", "negative", "both"):
             raise ValueError("polarity_mask must be 'positive', 'negative', or 'both'")
+        if not 0.0 < mu <= 1.0:
+            raise ValueError("mu must be in (0, 1]")
+        if self.record_iteration_diagnostics and (mu != 1.0 or polarity_mask != "both"):
+            _unsupported("record_iteration_diagnostics=True with mu < 1 or a polarity mask")
         shape = (3, self.Nz, self.Ny, self.Nx)
         if tuple(B.shape) != shape or tuple(J.shape) != shape:
             raise ValueError(f"B and J must have shape {shape}; got {tuple(B.shape)}, "
@@ -1491,7 +1547,11 @@ class NFGSSolverGPU:
                 "face pass")
         if diag is not None:
             diag.edges()
-        self._advance(1, zero_source=False)
+        old = None
+        if mu != 1.0:
+            shape = (self.Nz, self.Ny, self.Nx)
+            old = (self._dev(Phi_c, shape, "Phi_c"), self._dev(Az, shape, "Az"))
+        self._advance(1, zero_source=False, mu=mu, polarity=polarity_mask, old=old)
         _, _, _, ratio = self._read_metrics()
         self._check_zero_mode(ratio)
         if self._k0 is not None:
@@ -1863,10 +1923,12 @@ class NFGSSolverGPU:
             raise ValueError("fixed_point_patience must be >= 1")
         if polarity_mask not in ("positive", "negative", "both"):
             raise ValueError("polarity_mask must be 'positive', 'negative', or 'both'")
-        if polarity_mask != "both":
-            _unsupported(f"polarity_mask={polarity_mask!r}")
-        if float(mu) != 1.0 or adaptive_mu:
-            _unsupported("mu != 1 or adaptive_mu")
+        relaxed = float(mu) != 1.0 or bool(adaptive_mu) or polarity_mask != "both"
+        if relaxed and self.record_iteration_diagnostics:
+            _unsupported("record_iteration_diagnostics=True with mu < 1, adaptive_mu or a "
+                         "polarity mask")
+        if not 0.0 < float(mu) <= 1.0:
+            raise ValueError("mu must be in (0, 1]")
         self._require_boundary()
         mode = self._init_mode(init_mode)
 
@@ -1908,6 +1970,15 @@ class NFGSSolverGPU:
         drv = _Driver(sigma0, E0, tol, patience, sigma_improvement_tol, stop_mode,
                       dict(update=update_tol, pde=float(fixed_point_pde_tol),
                            observable=observable_tol, patience=int(fixed_point_patience)))
+        if relaxed:
+            return self._iterate_relaxed(
+                drv, int(max_iter), float(mu), bool(adaptive_mu), polarity_mask, verbose,
+                per_step_callback, dict(mu_min=float(mu_min), mu_max=float(mu_max),
+                                        mu_backoff=float(mu_backoff),
+                                        mu_growth=float(mu_growth),
+                                        growth_streak=int(growth_streak),
+                                        divergence_factor=float(divergence_factor),
+                                        max_rollbacks=int(max_rollbacks)))
         best_slot = final_slot = 0
         # The loop keeps one step in flight: step n+1 is enqueued before the
         # monitors of step n are read, so the device never waits on the host.
@@ -1973,6 +2044,66 @@ class NFGSSolverGPU:
         return self._finish(drv, self._slots[best_slot], self._slots[final_slot],
                             best_slot == final_slot, self._h_of(best_slot),
                             self._h_of(final_slot))
+
+    def _iterate_relaxed(self, drv, max_iter, mu, adaptive, polarity, verbose, callback, a):
+        """The driver with mu < 1, adaptive mu or a polarity mask
+        (nfgs_solver.py:3658-3815): steps run one at a time, since with
+        adaptive mu a step's sigma decides where the next one starts.  A step
+        whose sigma is not finite or above divergence_factor times the best is
+        rolled back: mu backs off to max(mu * mu_backoff, mu_min), the next
+        step starts from the best state, and max_rollbacks of them stop the
+        solve.  growth_streak new bests in a row grow mu to
+        min(mu * mu_growth, mu_max)."""
+        best_slot = last = 0
+        mu_cur = mu
+        improve = 0
+        for n in range(1, max_iter + 1):
+            new = next(k for k in range(3) if k not in (best_slot, last))
+            self._advance(new, zero_source=False, want_b=callback is not None, src=last, step=n,
+                          mu=mu_cur, polarity=polarity)
+            sigma, divB, energy, ratio = self._read_metrics()
+            self._check_zero_mode(ratio)
+            if adaptive and (not math.isfinite(sigma) or
+                             sigma > a["divergence_factor"] * max(drv.best_sigma, 1e-30)):
+                mu_cur = max(mu_cur * a["mu_backoff"], a["mu_min"])
+                improve = 0
+                drv.rollback(sigma, mu_cur)
+                if verbose:
+                    print(f"Iter {n:4d} | sigma={sigma:.4e} DIVERGED -> rollback to iter "
+                          f"{drv.best_iter} (best sigma={drv.best_sigma:.4e}), mu={mu_cur:.3f}")
+                # the next step starts from the best state: its J_par
+                bp, ba = self._slots[best_slot]
+                self._pass(bp, ba, None, self._ws_t["jp"], None, None, self._h_of(best_slot))
+                last = best_slot
+                if drv.rollbacks >= a["max_rollbacks"]:
+                    drv.reason = "max_rollbacks"
+                    if verbose:
+                        print(f"Stopping after {drv.rollbacks} rollbacks.")
+                    break
+                continue
+            drv.record(n, sigma, divB, energy, None, mu=mu_cur)
+            if callback is not None:
+                B_cb, J_cb = self._state_fields(new)
+                callback(n, sigma, B_cb, J_cb)
+            if drv.accept(n, sigma):
+                best_slot = new
+                improve += 1
+                if adaptive and improve >= a["growth_streak"]:
+                    mu_cur = min(mu_cur * a["mu_growth"], a["mu_max"])
+                    improve = 0
+            else:
+                improve = 0
+            last = new
+            if verbose and (n == 1 or n % 10 == 0):
+                print(f"Iter {n:4d} | sigma={sigma:.4e} | divB={divB:.4e} | E={energy:.4e} | "
+                      f"mu={mu_cur:.3f}")
+            if drv.check_stop(n, verbose):
+                break
+        if self._k0 is not None:
+            # every step updates the profile, rolled back or not
+            self._k0_record(list(range(1, len(drv.history["sigma"]) + 1)))
+        return self._finish(drv, self._slots[best_slot], self._slots[last], best_slot == last,
+                            self._h_of(best_slot), self._h_of(last))
 
     def _finish(self, drv, best_state, final_state, same: bool, best_h=None, final_h=None):
         """Selected state, final-vs-best, the staggered diagnostic and the
