@@ -66,6 +66,8 @@ def parse():
                         "against FP64)")
     p.add_argument("--precision-n", type=int, default=512)
     p.add_argument("--no-learned", action="store_true", help="skip the learned-start leg")
+    p.add_argument("--no-variants", action="store_true",
+                   help="skip the solves with the other closures, starts and relaxation")
     p.add_argument("--learned-pairs", type=int, default=128,
                    help="solver-generated training pairs of the learned-start leg")
     p.add_argument("--learned-epochs", type=int, default=20)
@@ -97,11 +99,11 @@ ITER_KW = dict(init_mode="gs_cfcf", tol=1e-5, mu=1, patience=40, sigma_improveme
 
 
 def solve(torch, NFGSSolverGPU, planes, n, rho, dtype, max_iter, start=None, extra=None,
-          nz=None):
+          nz=None, iterate_kw=None):
     """One timed solve with the boundary already resident on the device
     (n x n observed columns over nz planes, default n).
     start=(B, Az): init_mode="field" from that state instead of gs_cfcf;
-    extra: more constructor keywords."""
+    extra: more constructor keywords; iterate_kw: more iterate() keywords."""
     bx, by, bz = (p.to(dtype) for p in planes)
     kw = solver_kwargs(rho)
     kw.update(extra or {})
@@ -115,6 +117,7 @@ def solve(torch, NFGSSolverGPU, planes, n, rho, dtype, max_iter, start=None, ext
     kw = dict(ITER_KW)
     if start is not None:
         kw.update(init_mode="field", init_field=start[0], init_az=start[1])
+    kw.update(iterate_kw or {})
     h = s.iterate(max_iter=max_iter, **kw)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
@@ -706,6 +709,49 @@ def main():
             f"B diff vs direct {bd:.3e}")
         del s2
 
+    # the method's other forms on the same boundary: the coupled closures,
+    # the soft_cfcf start, a relaxed step and adaptive mu; each one's time to
+    # converge, its time per iteration and its distance from the plain solve
+    variants = None
+    if not a.no_variants:
+        variants = {}
+        fd = s.fields_to_numpy()
+
+        def bdiff(sv):
+            fv = sv.fields_to_numpy()
+            return float(np.sqrt(sum(np.sum((fv[k].astype(np.float64) - fd[k]) ** 2)
+                                     for k in ("Bx", "By", "Bz")) /
+                                 sum(np.sum(fd[k].astype(np.float64) ** 2)
+                                     for k in ("Bx", "By", "Bz"))))
+        forms = {
+            "coupled_k0": (dict(horizontal_zero_mode_policy="coupled_k0"), {}),
+            "constrained_spline_harmonic": (
+                dict(horizontal_zero_mode_policy="constrained_spline_harmonic"), {}),
+            "soft_cfcf_start": ({}, dict(init_mode="soft_cfcf")),
+            "mu_0.5": ({}, dict(mu=0.5)),
+            "adaptive_mu": ({}, dict(adaptive_mu=True)),
+        }
+        for name, (ctor, it) in forms.items():
+            log(f"variant {name}, fp32 rho=0.5")
+            sv, hv, sv_setup, tv = solve(torch, NFGSSolverGPU, planes, n, 0.5, torch.float32,
+                                         a.max_iter, extra=ctor, iterate_kw=it)
+            rec = dict(sec_to_converge=round(tv, 4), iterations=len(hv["sigma"]),
+                       ms_per_iteration=round((tv - sv_setup) / max(len(hv["sigma"]), 1) * 1e3,
+                                              4),
+                       best_iter=hv["best_iter"], best_sigma=hv["best_sigma"],
+                       stop_reason=hv["stop_reason"], rollbacks=hv.get("rollbacks", 0),
+                       B_rel_diff_vs_plain=bdiff(sv))
+            if name == "soft_cfcf_start":
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                sv.compute_initial_conditions(init_mode="soft_cfcf")
+                torch.cuda.synchronize()
+                rec["start_s"] = round(time.perf_counter() - t0, 4)
+            variants[name] = rec
+            log(f"  {name}: {tv:.3f} s, {rec['iterations']} it, best {hv['best_sigma']:.5e} "
+                f"({hv['stop_reason']}); B diff vs plain {rec['B_rel_diff_vs_plain']:.3e}")
+            del sv
+
     # error of the FP32 solve against an FP64 solve of the same boundary
     err = None
     if not a.no_fp64:
@@ -810,6 +856,7 @@ def main():
         "batched": batched,
         "precision": precision,
         "two_stage": two,
+        "variants": variants,
         "err_vs_fp64": err,
         "cpu_baseline": cpu,
     }

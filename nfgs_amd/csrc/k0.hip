@@ -460,11 +460,16 @@ __global__ __launch_bounds__(256) void k0_profile_kernel(
   __syncthreads();
   const double h0[2] = {(double)h_bottom[0], (double)h_bottom[1]};
   for (int z = threadIdx.x; z < nz; z += 256) {
+    // row z of M is column z of the stored (column-major) table
+    double sx = 0.0, sy = 0.0;
+    for (int j = 0; j < nz; ++j) {
+      const double m = M[(long)j * nz + z];
+      sx += m * q[j];
+      sy += m * q[nz + j];
+    }
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      double s = 0.0;
-      const double* row = M + (long)z * nz;
-      for (int j = 0; j < nz; ++j) s += row[j] * q[(long)c * nz + j];
+      double s = c == 0 ? sx : sy;
       s += v[z] * h0[c];
       const T ht = (T)s;
       hs[(long)c * nz + z] = ht;
@@ -506,58 +511,156 @@ __global__ __launch_bounds__(256) void k0_profile_kernel(
 // J_z is the cell projection of the face average of the PT J_z (zero on the
 // side walls and the top, the toroidal J_z on the bottom), and the side walls
 // carry the harmonic current (-dH_y/dz, dH_x/dz) instead of zero.  One thread
-// per column marches up the column for J_z; the first 2*nz threads of the
-// launch write the wall current.
+// per node forms J_z; the first 2*nz threads of the launch also write the
+// wall current.
 template <typename T>
 __global__ __launch_bounds__(256) void spline_prep_kernel(
     const T* __restrict__ az, const T* __restrict__ jzt, const T* __restrict__ h, int nx, int ny,
     int nz, T idx2, T idy2, T dz, T* __restrict__ jz, T* __restrict__ jh) {
   const long pl = (long)nx * ny;
-  const long r = (long)blockIdx.x * 256 + threadIdx.x;
-  if (r < 2 * nz) {
-    const int z = (int)(r % nz);
-    const bool ycomp = r >= nz;
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < 2 * nz) {
+    const int z = (int)(i % nz);
+    const bool ycomp = i >= nz;
     T v = T(0);
     if (z < nz - 1)
       v = ycomp ? centre_derivative<T>(h, z, nz, dz, false)
                 : -centre_derivative<T>(h + nz, z, nz, dz, false);
-    jh[r] = v;
+    jh[i] = v;
   }
-  if (r >= pl) return;
+  if (i >= pl * nz) return;
+  // one node per thread: the PT J_z of its plane and the two beside it
+  const int z = (int)(i / pl);
+  const long r = i - (long)z * pl;
   const int y = (int)(r / nx), x = (int)(r - (long)y * nx);
-  const T j0 = jzt[r];
-  jz[r] = j0;
-  if (x == 0 || x == nx - 1 || y == 0 || y == ny - 1) {
-    for (int z = 1; z < nz; ++z) jz[(long)z * pl + r] = T(0);
+  if (z == 0) {
+    jz[i] = jzt[r];
     return;
   }
-  // PT J_z = -lap_xy(Az) on the interior planes
-  auto pt = [&](int z) -> T {
-    if (z == nz - 1) return T(0);
-    const T* c = az + (long)z * pl + r;
+  if (z == nz - 1 || x == 0 || x == nx - 1 || y == 0 || y == ny - 1) {
+    jz[i] = T(0);
+    return;
+  }
+  // PT J_z = -lap_xy(Az) on the interior planes, J_z,T on the bottom, 0 on top
+  auto pt = [&](int k) -> T {
+    if (k == 0) return jzt[r];
+    if (k == nz - 1) return T(0);
+    const T* c = az + (long)k * pl + r;
     T acc = T(0);
     acc += (c[1] - T(2) * c[0] + c[-1]) * idx2;
     acc += (c[nx] - T(2) * c[0] + c[-nx]) * idy2;
     return -acc;
   };
-  T lo = j0, mid = pt(1);
-  for (int z = 1; z < nz - 1; ++z) {
-    const T up = pt(z + 1);
-    jz[(long)z * pl + r] = T(0.5) * (T(0.5) * (lo + mid) + T(0.5) * (mid + up));
-    lo = mid;
-    mid = up;
-  }
-  jz[(long)(nz - 1) * pl + r] = T(0);
+  const T lo = pt(z - 1), mid = pt(z), up = pt(z + 1);
+  jz[i] = T(0.5) * (T(0.5) * (lo + mid) + T(0.5) * (mid + up));
 }
 
+// The two fields the PT current of the means is formed from, once per node
+// of planes 0..nz-2: lphi = lap_xy(phi) + d2z(phi) (B_c,z + d2 Phi/dz2, whose
+// lateral gradient is the poloidal part) and dzaz = dAz/dz, with the
+// arithmetic of k0_means_kernel.
+template <typename T>
+__global__ __launch_bounds__(256) void k0_fields_kernel(const T* __restrict__ phi,
+                                                        const T* __restrict__ az, int nx, int ny,
+                                                        int nz, T dz, T idx2, T idy2,
+                                                        T* __restrict__ lphi,
+                                                        T* __restrict__ dzaz) {
+  const long pl = (long)nx * ny;
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= pl * (nz - 1)) return;
+  const int z = (int)(i / pl);
+  const long c = i - (long)z * pl;
+  const int y = (int)(c / nx), x = (int)(c - (long)y * nx);
+  const T dz2 = dz * dz;
+  const T bcz = lap_xy_at(phi + (long)z * pl, y, x, ny, nx, (long)nx, idx2, idy2);
+  lphi[i] = bcz + d2z_at<T>([&](int o) { return phi[(long)(z + o) * pl + c]; }, z, nz, dz2);
+  dzaz[i] = fd1g<T>([&](int o) { return az[(long)(z + o) * pl + c]; }, z, nz, dz);
+}
+
+// The trapezoid plane sums of k0_means_kernel from the precomputed fields,
+// split over gridDim.x chunks of every plane (blockIdx.y): block partials
+// part[((z * chunks) + chunk) * 4 + q].
+template <typename T>
+__global__ __launch_bounds__(256) void k0_sums_kernel(const T* __restrict__ jp,
+                                                      const T* __restrict__ lphi,
+                                                      const T* __restrict__ dzaz, int nx, int ny,
+                                                      int nz, T dx, T dy,
+                                                      double* __restrict__ part) {
+  __shared__ double red[4][256];
+  const int z = blockIdx.y, chunk = blockIdx.x, chunks = gridDim.x;
+  const long pl = (long)nx * ny, N = pl * nz;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long r = (long)chunk * 256 + threadIdx.x; r < pl; r += 256L * chunks) {
+    const int y = (int)(r / nx), x = (int)(r - (long)y * nx);
+    const double w = ((x == 0 || x == nx - 1) ? 1.0 : 2.0) * ((y == 0 || y == ny - 1) ? 1.0 : 2.0);
+    const long i = (long)z * pl + r;
+    s[0] += w * (double)jp[i];
+    s[1] += w * (double)jp[N + i];
+    if (x > 0 && x < nx - 1 && y > 0 && y < ny - 1 && z < nz - 1) {
+      const T dx_dzaz = fd1g<T>([&](int o) { return dzaz[i + o]; }, x, nx, dx);
+      const T dy_dzaz = fd1g<T>([&](int o) { return dzaz[i + (long)o * nx]; }, y, ny, dy);
+      const T dy_l = fd1g<T>([&](int o) { return lphi[i + (long)o * nx]; }, y, ny, dy);
+      const T dx_l = fd1g<T>([&](int o) { return lphi[i + o]; }, x, nx, dx);
+      s[2] += w * (double)(dx_dzaz + dy_l);
+      s[3] += w * (double)(dy_dzaz - dx_l);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = s[c];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) part[((long)z * chunks + chunk) * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// means[q * nz + z] = sum over the chunks of plane z, in chunk order.
+__global__ void k0_sums_final_kernel(const double* __restrict__ part, int chunks, int nx, int ny,
+                                     int nz, double* __restrict__ means) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= nz) return;
+  const double norm = 4.0 * (double)(nx - 1) * (double)(ny - 1);
+  for (int q = 0; q < 4; ++q) {
+    double v = 0.0;
+    for (int c = 0; c < chunks; ++c) v += part[((long)z * chunks + c) * 4 + q];
+    means[(long)q * nz + z] = v / norm;
+  }
+}
+
+// The means, then the profile.  With the scratch of a step (lphi and dzaz:
+// N values each of the plan dtype, part: room for nz * 16 * 4 doubles) the
+// means come from the two-pass kernels, else from the one-pass kernel.
 template <typename T>
 int update_t(const nfgs_plan* p, const nfgs_k0_step* k, const void* jp, const void* phi,
-             const void* az, hipStream_t s) {
+             const void* az, void* lphi, void* dzaz, double* part, int part_doubles,
+             hipStream_t s) {
   const int nx = p->d.nx, ny = p->d.ny, nz = p->d.nz;
-  hipLaunchKernelGGL(k0_means_kernel<T>, dim3(nz), dim3(256), 0, s, static_cast<const T*>(jp),
-                     static_cast<const T*>(phi), static_cast<const T*>(az), nx, ny, nz,
-                     (T)p->d.dx, (T)p->d.dy, (T)p->d.dz, k->work);
-  NFGS_LAUNCH_CHECK();
+  if (lphi && dzaz && part) {
+    const long n = p->plane * (nz - 1);
+    hipLaunchKernelGGL(k0_fields_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const T*>(phi), static_cast<const T*>(az), nx, ny, nz,
+                       (T)p->d.dz, (T)(1.0 / (p->d.dx * p->d.dx)),
+                       (T)(1.0 / (p->d.dy * p->d.dy)), static_cast<T*>(lphi),
+                       static_cast<T*>(dzaz));
+    NFGS_LAUNCH_CHECK();
+    int chunks = part_doubles / (4 * nz);
+    chunks = chunks > 16 ? 16 : (chunks < 1 ? 1 : chunks);
+    hipLaunchKernelGGL(k0_sums_kernel<T>, dim3(chunks, nz), dim3(256), 0, s,
+                       static_cast<const T*>(jp), static_cast<const T*>(lphi),
+                       static_cast<const T*>(dzaz), nx, ny, nz, (T)p->d.dx, (T)p->d.dy, part);
+    NFGS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k0_sums_final_kernel, dim3((nz + 255) / 256), dim3(256), 0, s, part, chunks,
+                       nx, ny, nz, k->work);
+    NFGS_LAUNCH_CHECK();
+  } else {
+    hipLaunchKernelGGL(k0_means_kernel<T>, dim3(nz), dim3(256), 0, s, static_cast<const T*>(jp),
+                       static_cast<const T*>(phi), static_cast<const T*>(az), nx, ny, nz,
+                       (T)p->d.dx, (T)p->d.dy, (T)p->d.dz, k->work);
+    NFGS_LAUNCH_CHECK();
+  }
   const size_t lds = (size_t)2 * nz * sizeof(double) + (size_t)2 * nz * sizeof(T);
   hipLaunchKernelGGL(k0_profile_kernel<T>, dim3(1), dim3(256), lds, s, k->work, k->tables, nz,
                      (T)p->d.dz, k->discretization == NFGS_K0_LEGACY,
@@ -569,6 +672,22 @@ int update_t(const nfgs_plan* p, const nfgs_k0_step* k, const void* jp, const vo
 
 }  // namespace
 
+int k0_update_scratch(const nfgs_plan* p, const nfgs_k0_step* k, const void* jp, const void* phi,
+                      const void* az, void* lphi, void* dzaz, void* part, hipStream_t s) {
+  NFGS_REQUIRE(p && k && jp && phi && az && k->tables && k->h_bottom && k->h_old && k->h_new &&
+                   k->work,
+               NFGS_ERR_ARG, "k0 update: null argument");
+  NFGS_REQUIRE(p->d.nz >= 4 && p->d.nz <= 1024, NFGS_ERR_ARG,
+               "k0 update: Nz must be in [4, 1024]");
+  NFGS_REQUIRE(k->mu > 0.0 && k->mu <= 1.0, NFGS_ERR_ARG, "k0 update: mu must be in (0, 1]");
+  // part holds the N plan-dtype values of a workspace volume
+  const long cap = p->N * p->esize / (long)sizeof(double);
+  const int pd = (int)(cap > (1L << 30) ? (1L << 30) : cap);
+  if (p->d.dtype == NFGS_F32)
+    return update_t<float>(p, k, jp, phi, az, lphi, dzaz, static_cast<double*>(part), pd, s);
+  return update_t<double>(p, k, jp, phi, az, lphi, dzaz, static_cast<double*>(part), pd, s);
+}
+
 // The state pass of a closure's state: the spline closure's J_z and wall
 // current go to the pass through jz_work (N values) and jh_work (2*nz).
 int closure_state_pass(const nfgs_plan* p, const nfgs_boundary* bnd, int kind, const void* phi,
@@ -577,7 +696,7 @@ int closure_state_pass(const nfgs_plan* p, const nfgs_boundary* bnd, int kind, c
   if (kind != NFGS_K0_SPLINE)
     return state_pass_closure(p, bnd, phi, az, b, jp, jout, partials, nullptr, nullptr, s);
   const int nx = p->d.nx, ny = p->d.ny, nz = p->d.nz;
-  const long n = std::max<long>(p->plane, 2L * nz);
+  const long n = std::max<long>(p->N, 2L * nz);
   const dim3 grid((unsigned)((n + 255) / 256));
   if (p->d.dtype == NFGS_F32)
     hipLaunchKernelGGL(spline_prep_kernel<float>, grid, dim3(256), 0, s,
@@ -617,7 +736,12 @@ int nfgs_k0_tables(int nz, double dz, int kind, int discretization, double smoot
   if (const int rc = build_tables(nz, dz, kind, discretization, smoothness, basis_size, M, v,
                                   condition))
     return rc;
-  NFGS_HIP_CHECK(hipMemcpy(tables, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
+  // the device holds M column by column, so the profile kernel's threads
+  // (one output row each) read consecutive doubles
+  Mat Mt(M.size());
+  for (int i = 0; i < nz; ++i)
+    for (int j = 0; j < nz; ++j) Mt[(size_t)j * nz + i] = M[(size_t)i * nz + j];
+  NFGS_HIP_CHECK(hipMemcpy(tables, Mt.data(), Mt.size() * sizeof(double), hipMemcpyHostToDevice));
   NFGS_HIP_CHECK(hipMemcpy(tables + M.size(), v.data(), v.size() * sizeof(double),
                            hipMemcpyHostToDevice));
   return NFGS_OK;
@@ -631,8 +755,9 @@ int nfgs_k0_update(const nfgs_plan* p, const nfgs_k0_step* k, const void* jp, co
   NFGS_REQUIRE(p->d.nz >= 4 && p->d.nz <= 1024, NFGS_ERR_ARG,
                "k0 update: Nz must be in [4, 1024]");
   NFGS_REQUIRE(k->mu > 0.0 && k->mu <= 1.0, NFGS_ERR_ARG, "k0 update: mu must be in (0, 1]");
-  if (p->d.dtype == NFGS_F32) return update_t<float>(p, k, jp, phi, az, as_stream(stream));
-  return update_t<double>(p, k, jp, phi, az, as_stream(stream));
+  if (p->d.dtype == NFGS_F32)
+    return update_t<float>(p, k, jp, phi, az, nullptr, nullptr, nullptr, 0, as_stream(stream));
+  return update_t<double>(p, k, jp, phi, az, nullptr, nullptr, nullptr, 0, as_stream(stream));
 }
 
 int nfgs_k0_state_pass(const nfgs_plan* p, const nfgs_boundary* bnd, int kind, const void* phi,

@@ -83,6 +83,10 @@ int divide_by_lambda(const nfgs_plan* p, void* hat, int nplanes, hipStream_t s);
 int state_pass_closure(const nfgs_plan* p, const nfgs_boundary* bnd, const void* phi,
                        const void* az, void* b, void* jp, void* jout, double* partials,
                        const void* jzc, const void* jh, hipStream_t s);
+// The closure update of a step with workspace scratch: lphi and dzaz take N
+// values each of the plan dtype, part a workspace volume (as doubles).
+int k0_update_scratch(const nfgs_plan* p, const nfgs_k0_step* k, const void* jp, const void* phi,
+                      const void* az, void* lphi, void* dzaz, void* part, hipStream_t s);
 // The state pass of a closure (kind NFGS_K0_COUPLED or NFGS_K0_SPLINE);
 // the spline closure's inputs to state_pass_closure are built in jz_work
 // (N values) and jh_work (2*nz values), both of the plan dtype.

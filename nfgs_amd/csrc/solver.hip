@@ -120,7 +120,8 @@ extern "C" int nfgs_advance_k0(const nfgs_plan* p, const nfgs_boundary* bnd,
       NFGS_HIP_CHECK(hipMemcpyAsync(k0->h_new, k0->h_old, (size_t)2 * p->d.nz * p->esize,
                                     hipMemcpyDeviceToDevice, as_stream(stream)));
   } else {
-    NFGS_TRY(nfgs_k0_update(p, k0, ws->jp, phi_out, az_out, stream));
+    NFGS_TRY(k0_update_scratch(p, k0, ws->jp, phi_out, az_out, ws->s_bcz, ws->hat0, ws->hat1,
+                               as_stream(stream)));
   }
   // the sources and the solves are done with s_bcz and hat0: the spline
   // closure's J_z and wall current are built there
@@ -200,7 +201,8 @@ extern "C" int nfgs_advance_relaxed(const nfgs_plan* p, const nfgs_boundary* bnd
     NFGS_LAUNCH_CHECK();
   }
   NFGS_TRY(advance_solves(p, bnd, ws, phi_out, az_out, 0, poloidal_mode, stream));
-  if (k0) NFGS_TRY(nfgs_k0_update(p, k0, ws->jp, phi_out, az_out, stream));
+  if (k0)
+    NFGS_TRY(k0_update_scratch(p, k0, ws->jp, phi_out, az_out, ws->s_bcz, ws->hat0, ws->hat1, s));
   if (mu < 1.0) {
     const long n = p->N - p->plane;
     const dim3 g((unsigned)((n + 255) / 256));

@@ -1970,7 +1970,7 @@ class NFGSSolverGPU:
         drv = _Driver(sigma0, E0, tol, patience, sigma_improvement_tol, stop_mode,
                       dict(update=update_tol, pde=float(fixed_point_pde_tol),
                            observable=observable_tol, patience=int(fixed_point_patience)))
-        if relaxed:
+        if adaptive_mu:
             return self._iterate_relaxed(
                 drv, int(max_iter), float(mu), bool(adaptive_mu), polarity_mask, verbose,
                 per_step_callback, dict(mu_min=float(mu_min), mu_max=float(mu_max),
@@ -1996,7 +1996,7 @@ class NFGSSolverGPU:
             if diag is not None:
                 diag.edges()
             self._advance(s_new, zero_source=False, want_b=not ahead or diag is not None,
-                          src=old, step=n_step)
+                          src=old, step=n_step, mu=float(mu), polarity=polarity_mask)
             if diag is not None:
                 diag.after_step(self._slots[old], self._slots[s_new], self._Bwork)
                 diag.post(n_step & 1)
@@ -2019,7 +2019,7 @@ class NFGSSolverGPU:
                 dg = diag.take(n & 1)
                 self._last_iteration_diagnostics = dg
                 self._iteration_diagnostics_history.append(dg)
-            drv.record(n, sigma, divB, energy, dg)
+            drv.record(n, sigma, divB, energy, dg, mu=float(mu))
             if per_step_callback is not None:
                 # the callback owns fresh B and J of this step, as the
                 # reference hands it the new state tensors (:3736-3737)
@@ -2036,7 +2036,7 @@ class NFGSSolverGPU:
                 best_slot = slot
             if verbose and (n == 1 or n % 10 == 0):
                 print(f"Iter {n:4d} | sigma={sigma:.4e} | divB={divB:.4e} | E={energy:.4e} | "
-                      f"mu={1.0:.3f}")
+                      f"mu={float(mu):.3f}")
             if drv.check_stop(n, verbose):
                 break
         if self._k0 is not None:
@@ -2046,9 +2046,9 @@ class NFGSSolverGPU:
                             self._h_of(final_slot))
 
     def _iterate_relaxed(self, drv, max_iter, mu, adaptive, polarity, verbose, callback, a):
-        """The driver with mu < 1, adaptive mu or a polarity mask
-        (nfgs_solver.py:3658-3815): steps run one at a time, since with
-        adaptive mu a step's sigma decides where the next one starts.  A step
+        """The driver with adaptive mu (nfgs_solver.py:3658-3815): steps run
+        one at a time, since a step's sigma decides where the next one starts
+        (a fixed mu < 1 or a polarity mask alone keeps the pipelined loop).  A step
         whose sigma is not finite or above divergence_factor times the best is
         rolled back: mu backs off to max(mu * mu_backoff, mu_min), the next
         step starts from the best state, and max_rollbacks of them stop the
