@@ -695,10 +695,32 @@ __device__ __forceinline__ void state_pass_body(
 #pragma unroll
       for (int i = 0; i < R1PT; ++i) {
         const int r = threadIdx.x + 256 * i;
-        if (r < R1N) {
+        int ly, lx;
+        bool on;
+        if (!IX && t.narrow) {
           // narrow tiles: column-major, so the in-grid ring points share a wave
-          const bool cm = !IX && t.narrow;
-          const int ly = 1 + (cm ? r % R1H : r / R1W), lx = 1 + (cm ? r / R1H : r % R1W);
+          on = r < R1N;
+          ly = 1 + r % R1H;
+          lx = 1 + r / R1H;
+        } else if (i == 0) {
+          // the tile's own columns, one row per half-wave (conflict-free LDS
+          // banks); the first TY ring rows
+          on = true;
+          ly = 1 + (int)threadIdx.x / TX;
+          lx = HH + (int)threadIdx.x % TX;
+        } else if ((int)threadIdx.x < (R1H - TY) * TX) {
+          // the ring's last two rows, own columns
+          on = true;
+          ly = 1 + TY + (int)threadIdx.x / TX;
+          lx = HH + (int)threadIdx.x % TX;
+        } else {
+          // the ring's two side columns
+          const int j = (int)threadIdx.x - (R1H - TY) * TX;
+          on = j < 2 * R1H;
+          ly = 1 + (j >> 1);
+          lx = (j & 1) ? R1W : 1;
+        }
+        if (on) {
           const int e = ly * PW + lx;
           const int y = t.y0 + ly - HH, x = t.x0 + lx - HH;
           T vbx = T(0), vby = T(0), vcz = T(0);
